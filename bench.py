@@ -1,0 +1,127 @@
+"""Benchmark: ms per plate solve to relative residual 1e-8 on one MI355X.
+
+Workload (default): 142 x 142 Q4 Mindlin plate, 61,347 DOF, clamped edges,
+E = 210 GPa, nu = 0.3, t = 0.1 m, 10 kPa + 50 kPa Gaussian patch (sigma 1 m,
+centre from SplitMix64 seed 1); FP64 block-Jacobi PCG, whole loop on the
+device.  A step is one full solve of the resident load vector.
+
+Prints one JSON line (rank 0).  Extra fields: the roofline of the persistent
+PCG kernel and the CPU baseline (the float32 restatement of solver.ts run in
+Node on a bounded number of iterations, extrapolated to its full count).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-gauntlet_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per DOF per FP64 PCG iteration: read p, x, r and the
+# 3x3 block inverse (3 words per DOF), write p, x, r = 9 words of 8 bytes.
+BYTES_PER_DOF_ITER_FP64 = 72
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-sample-iters", type=int, default=1200,
+                    help="iterations of the CPU baseline sample (0 = skip)")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, F, full_iters, sample_iters):
+    """Float32 restatement of solver.ts (oracle/solver_f32.js) in Node, one
+    thread, on the first `sample_iters` iterations of the same solve."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+
+    n = w["mesh"]["gridNx"]
+    _, st = O.run_js_solver(n, F, w["E"], w["nu"], w["t"],
+                            [{"type": "fixed", "location": "all_edges"}], tol=1e-8,
+                            max_iter=sample_iters)
+    ms_per_iter = st["solveTimeMs"] / max(1, st["iterations"])
+    return {
+        "value": ms_per_iter * full_iters, "unit": "ms/solve", "cores": 1, "kind": "port",
+        "sample": (f"float32 solver.ts restatement in Node {st['node']} on "
+                   f"{st['cpuModel']}: {st['iterations']} PCG iterations timed "
+                   f"({st['solveTimeMs']:.0f} ms), x {full_iters} iterations it needs to "
+                   f"reach 1e-8"),
+        "ms_per_iteration": ms_per_iter,
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+
+    from platepcg import Context, Plan, _lib
+    from platepcg import problem as P
+
+    w = P.workload(a.workload)
+    ctx = Context(0)
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    F = plan.pressure_load(w["q0"], w["patches"])
+    plan.set_rhs(F)
+    opts = plan.options(tolerance=1e-8, max_iterations=100000,
+                        flags=_lib.FLAG_NO_TRUE_RESIDUAL)
+    for _ in range(a.warmup):
+        plan.solve_resident(opts)
+    dev_ms, iters = [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        st = plan.solve_resident(opts)
+        dev_ms.append(st.device_ms)
+        iters.append(st.iterations)
+    wall = (time.perf_counter() - t0) * 1e3
+    chk = plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=100000))
+    ms_per_step = wall / a.steps
+    n_dof = plan.n_dof
+    it = int(np.median(iters))
+    kern_ms = float(np.mean(dev_ms))
+    bytes_per_launch = BYTES_PER_DOF_ITER_FP64 * n_dof * it
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tr_path = os.path.join(ROOT, "profiles", f"pcg_fp64_{a.workload}_traffic.json")
+    if os.path.exists(tr_path):
+        with open(tr_path) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "ms/solve to 1e-8 rel. residual (61k DOF, FP64 block-Jacobi PCG)",
+        "value": ms_per_step, "unit": "ms/solve", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {w['mesh']['gridNx']}x{w['mesh']['gridNy']} Q4 "
+                   f"Mindlin plate, {n_dof} DOF, clamped, 10 kPa + Gaussian patch",
+                   "dof": n_dof, "iterations": it, "precond": "block-jacobi-3x3",
+                   "precision": "fp64"},
+        "device_ms_mean": kern_ms, "device_ms_min": float(np.min(dev_ms)),
+        "true_rel_residual": chk.true_rel_residual, "converged": bool(chk.converged),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_pcg (persistent FP64 PCG)",
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if a.cpu_sample_iters > 0:
+        g = np.load(os.path.join(ROOT, "tests", "golden", "c2_142.npz")) if a.workload == "c2" else None
+        full = int(g["it_js"]) if g is not None else it
+        try:
+            out["cpu_baseline"] = cpu_baseline(w, F, full, a.cpu_sample_iters)
+        except (OSError, subprocess.CalledProcessError) as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

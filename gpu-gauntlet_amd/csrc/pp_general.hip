@@ -1,0 +1,572 @@
+// Coloured (general-mesh) path: per-element FP64 stiffness stored in HBM and
+// applied one colour at a time, thread = element, race-free scatter because
+// no two elements of a colour share a node (the scheme of
+// gpu/solver.ts:413-461 + apply_k_q4.wgsl:133-350).  Used for meshes that are
+// not tensor grids (triangles, general quads) and as the parity path that
+// mirrors the reference's own scatter order.
+//
+// The PCG loop is driven from the host without per-iteration read-back: all
+// scalars (alpha, beta, |r|, the iteration counter and the stop flag) live on
+// the device, every kernel returns early once the stop flag is set, and the
+// host only polls the flag once per batch of iterations.
+
+#include <math.h>
+
+#include <algorithm>
+
+#include "pp_device.h"
+
+namespace pp {
+
+namespace {
+
+constexpr int kRedBlocks = 512;  // fixed partial count: deterministic sums
+constexpr int kBatch = 64;       // iterations enqueued between flag polls
+
+// device scalar slots
+enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BNORM, S_ITER, S_DONE, S_CONV, S_BREAK, S_N };
+
+// ------------------------------------------------ host element stiffness
+
+// Q4 Mindlin 12x12 for a general quadrilateral (element.ts:258-483).
+void ke_q4(const double* xy, double E, double nu, double t, double* Ke) {
+  const double D = E * t * t * t / (12.0 * (1.0 - nu * nu));
+  const double D11 = D, D12 = D * nu, D33 = D * (1.0 - nu) / 2.0;
+  const double Ds = (5.0 / 6.0) * E / (2.0 * (1.0 + nu)) * t;
+  std::fill(Ke, Ke + 144, 0.0);
+  auto phys = [&](double xi, double eta, double* dx, double* dy) {
+    const double dxi[4] = {-(1 - eta) / 4, (1 - eta) / 4, (1 + eta) / 4, -(1 + eta) / 4};
+    const double deta[4] = {-(1 - xi) / 4, -(1 + xi) / 4, (1 + xi) / 4, (1 - xi) / 4};
+    double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+    for (int i = 0; i < 4; i++) {
+      J11 += dxi[i] * xy[2 * i]; J12 += dxi[i] * xy[2 * i + 1];
+      J21 += deta[i] * xy[2 * i]; J22 += deta[i] * xy[2 * i + 1];
+    }
+    const double det = J11 * J22 - J12 * J21, inv = 1.0 / det;
+    for (int i = 0; i < 4; i++) {
+      dx[i] = inv * (J22 * dxi[i] - J12 * deta[i]);
+      dy[i] = inv * (-J21 * dxi[i] + J11 * deta[i]);
+    }
+    return det;
+  };
+  const double g = 0.5773502691896257;
+  const double gp[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
+  for (int q = 0; q < 4; q++) {
+    double dx[4], dy[4];
+    const double f = fabs(phys(gp[q][0], gp[q][1], dx, dy));
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        const int ti = 3 * i + 1, yi = 3 * i + 2, tj = 3 * j + 1, yj = 3 * j + 2;
+        Ke[ti * 12 + tj] += f * (dy[i] * D11 * dy[j] + dx[i] * D33 * dx[j]);
+        Ke[yi * 12 + yj] += f * (dx[i] * D11 * dx[j] + dy[i] * D33 * dy[j]);
+        Ke[ti * 12 + yj] += f * (-dy[i] * D12 * dx[j] - dx[i] * D33 * dy[j]);
+        Ke[yi * 12 + tj] += f * (-dx[i] * D12 * dy[j] - dy[i] * D33 * dx[j]);
+      }
+  }
+  double dx[4], dy[4];
+  const double f = 4.0 * fabs(phys(0, 0, dx, dy));
+  const double N = 0.25;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      const int wi = 3 * i, wj = 3 * j;
+      Ke[wi * 12 + wj] += f * Ds * (dx[i] * dx[j] + dy[i] * dy[j]);
+      Ke[wi * 12 + wj + 1] += f * Ds * dy[i] * (-N);
+      Ke[wi * 12 + wj + 2] += f * Ds * dx[i] * N;
+      Ke[(wi + 1) * 12 + wj] += f * Ds * (-N) * dy[j];
+      Ke[(wi + 1) * 12 + wj + 1] += f * Ds * N * N;
+      Ke[(wi + 2) * 12 + wj] += f * Ds * N * dx[j];
+      Ke[(wi + 2) * 12 + wj + 2] += f * Ds * N * N;
+    }
+}
+
+// The reference's 9x9 triangle (element.ts:642-943) exactly as written,
+// including its signed 1/(2A) and its B columns.
+void ke_dkt(const double* xy, double E, double nu, double t, double* Ke) {
+  std::fill(Ke, Ke + 81, 0.0);
+  const double x12 = xy[0] - xy[2], y12 = xy[1] - xy[3];
+  const double x23 = xy[2] - xy[4], y23 = xy[3] - xy[5];
+  const double x31 = xy[4] - xy[0], y31 = xy[5] - xy[1];
+  const double twoA = x12 * y31 - x31 * y12;
+  const double area = fabs(twoA) / 2;
+  if (area < 1e-15) return;
+  const double D = E * t * t * t / (12.0 * (1.0 - nu * nu));
+  const double D11 = D, D12 = D * nu, D33 = D * (1.0 - nu) / 2.0;
+  // edges k = 4 (2-3), 5 (3-1), 6 (1-2)
+  const double ex[3] = {x23, x31, x12}, ey[3] = {y23, y31, y12};
+  double a[3], b[3], c[3], d[3], e[3];
+  for (int k = 0; k < 3; k++) {
+    const double L = ex[k] * ex[k] + ey[k] * ey[k];
+    a[k] = -ex[k] / L;
+    b[k] = 3 * ex[k] * ey[k] / (4 * L);
+    c[k] = (ex[k] * ex[k] - 2 * ey[k] * ey[k]) / (4 * L);
+    d[k] = -ey[k] / L;
+    e[k] = (ey[k] * ey[k] - 2 * ex[k] * ex[k]) / (4 * L);
+  }
+  const double P4 = -6 * a[0], P5 = -6 * a[1], P6 = -6 * a[2];
+  const double t4 = -6 * d[0], t5 = -6 * d[1], t6 = -6 * d[2];
+  const double q4 = 3 * a[0], q5 = 3 * a[1], q6 = 3 * a[2];
+  const double r4 = 3 * d[0], r5 = 3 * d[1], r6 = 3 * d[2];
+  const double b4 = b[0], b5 = b[1], b6 = b[2], c4 = c[0], c5 = c[1], c6 = c[2];
+  const double e4 = e[0], e5 = e[1], e6 = e[2];
+  const double gL[3][2] = {{1.0 / 6, 1.0 / 6}, {2.0 / 3, 1.0 / 6}, {1.0 / 6, 2.0 / 3}};
+  for (int gq = 0; gq < 3; gq++) {
+    const double L1 = gL[gq][0], L2 = gL[gq][1];
+    const double m1 = 1 - 2 * L1, m2 = 1 - 2 * L2;
+    const double hx1[9] = {P6 * m1 + (P5 - P6) * L2, q6 * m1 - (q5 + q6) * L2,
+                           -4 + 6 * (L1 + L2) + (c5 - c6) * L2 + c6 * m1,
+                           P4 * L2 + P6 * m1, q4 * L2 - q6 * m1, c4 * L2 - c6 * m1,
+                           -P5 * L2 - P6 * m1, q5 * L2 + q6 * m1, -c5 * L2 + c6 * m1};
+    const double hx2[9] = {-P6 * m2 + (P4 - P5) * L1 - P5 * m2,
+                           q6 * m2 - (q4 + q5) * L1 + q5 * m2,
+                           -4 + 6 * (L1 + L2) + (c4 - c5) * L1 + c5 * m2,
+                           P4 * m2 + P5 * L1, q4 * m2 - q5 * L1, c4 * m2 - c5 * L1,
+                           -P4 * L1 - P5 * m2, q4 * L1 + q5 * m2, -c4 * L1 + c5 * m2};
+    const double hy1[9] = {t6 * m1 + (t5 - t6) * L2, 1 + r6 * m1 - (r5 + r6) * L2,
+                           -e6 * m1 + (b5 - b6) * L2,
+                           t4 * L2 + t6 * m1, -1 + r4 * L2 - r6 * m1, -e4 * L2 - b6 * m1,
+                           -t5 * L2 - t6 * m1, r5 * L2 + r6 * m1, e5 * L2 + b6 * m1};
+    const double hy2[9] = {-t6 * m2 + (t4 - t5) * L1 - t5 * m2,
+                           1 + r6 * m2 - (r4 + r5) * L1 + r5 * m2,
+                           e6 * m2 + (b4 - b5) * L1 - b5 * m2,
+                           t4 * m2 + t5 * L1, -1 + r4 * m2 - r5 * L1, -e4 * m2 + b5 * L1,
+                           -t4 * L1 - t5 * m2, r4 * L1 + r5 * m2, e4 * L1 - b5 * m2};
+    const double inv2A = 1 / twoA;
+    double B[3][9], DB[3][9];
+    for (int j = 0; j < 9; j++) {
+      B[0][j] = inv2A * (-y23 * hx1[j] - y31 * hx2[j]);
+      B[1][j] = inv2A * (x23 * hy1[j] + x31 * hy2[j]);
+      B[2][j] = inv2A * (x23 * hx1[j] + x31 * hx2[j] - y23 * hy1[j] - y31 * hy2[j]);
+    }
+    for (int j = 0; j < 9; j++) {
+      DB[0][j] = D11 * B[0][j] + D12 * B[1][j];
+      DB[1][j] = D12 * B[0][j] + D11 * B[1][j];
+      DB[2][j] = D33 * B[2][j];
+    }
+    const double f = area / 3.0;
+    for (int i = 0; i < 9; i++)
+      for (int j = 0; j < 9; j++)
+        Ke[i * 9 + j] += f * (B[0][i] * DB[0][j] + B[1][i] * DB[1][j] + B[2][i] * DB[2][j]);
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <int NPE>
+__global__ void k_colour(const uint32_t* list, int count, const uint32_t* elems, const double* Ke,
+                         const uint8_t* cdof, const double* x, double* y, int skip_fixed,
+                         const double* sc) {
+  if (sc && sc[S_DONE] != 0.0) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  constexpr int ND = 3 * NPE;
+  const uint32_t e = list[k];
+  int64_t dof[ND];
+  double u[ND];
+#pragma unroll
+  for (int i = 0; i < NPE; i++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) dof[3 * i + d] = (int64_t)elems[(size_t)e * NPE + i] * 3 + d;
+#pragma unroll
+  for (int j = 0; j < ND; j++) u[j] = x[dof[j]];
+  const double* K = Ke + (size_t)e * ND * ND;
+#pragma unroll
+  for (int r = 0; r < ND; r++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < ND; c++) s += K[r * ND + c] * u[c];
+    if (!(skip_fixed && cdof[dof[r]])) y[dof[r]] += s;
+  }
+}
+
+__global__ void k_zero(int64_t n, double* y, const double* sc) {
+  if (sc && sc[S_DONE] != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = 0.0;
+}
+
+__global__ void k_bc_rows(int64_t n, const uint8_t* cdof, const double* x, double* y,
+                          const double* sc) {
+  if (sc && sc[S_DONE] != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && cdof[i]) y[i] = x[i];
+}
+
+__global__ void k_precond6(int64_t nn, const double* minv, const double* r, double* z,
+                           const double* sc) {
+  if (sc && sc[S_DONE] != 0.0) return;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const double rv[3] = {r[3 * n], r[3 * n + 1], r[3 * n + 2]};
+  double zv[3];
+  sym3_apply(minv + 6 * n, rv, zv);
+  for (int d = 0; d < 3; d++) z[3 * n + d] = zv[d];
+}
+
+// partial sums of a.b (and c.d when c != nullptr) over a fixed grid
+__global__ __launch_bounds__(256) void k_dot2(int64_t n, const double* a, const double* b,
+                                              const double* c, const double* d, double* part,
+                                              const double* sc) {
+  if (sc && sc[S_DONE] != 0.0) return;
+  __shared__ double red[16];
+  double v[2] = {0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    v[0] += a[i] * b[i];
+    if (c) v[1] += c[i] * d[i];
+  }
+  block_sum<2, 256>(v, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v[0];
+    part[kRedBlocks + blockIdx.x] = v[1];
+  }
+}
+
+__device__ double sum_parts(const double* part, double* red) {
+  double v[1] = {0};
+  for (int i = threadIdx.x; i < kRedBlocks; i += 256) v[0] += part[i];
+  block_sum<1, 256>(v, red);
+  return v[0];
+}
+
+// init: rz = r.z, bnorm = |F|
+__global__ __launch_bounds__(256) void k_fin_init(const double* part, double* sc) {
+  __shared__ double red[16];
+  const double rz = sum_parts(part, red);
+  const double ff = sum_parts(part + kRedBlocks, red);
+  if (threadIdx.x == 0) {
+    sc[S_RZ] = rz;
+    sc[S_RR] = ff;
+    sc[S_BNORM] = sqrt(ff);
+    sc[S_ITER] = 0;
+    sc[S_CONV] = 0;
+    sc[S_BREAK] = 0;
+    sc[S_DONE] = sqrt(ff) < 1e-30 ? 1.0 : 0.0;
+    if (sqrt(ff) < 1e-30) { sc[S_CONV] = 1; sc[S_RR] = 0; }
+  }
+}
+
+// alpha = rz / pAp, breakdown test (pcg.ts:151-164)
+__global__ __launch_bounds__(256) void k_fin_pap(const double* part, double* sc, double maxit) {
+  if (sc[S_DONE] != 0.0) return;
+  __shared__ double red[16];
+  const double pap = sum_parts(part, red);
+  if (threadIdx.x == 0) {
+    sc[S_PAP] = pap;
+    if (fabs(pap) < 1e-30) {
+      sc[S_ITER] += 1;
+      sc[S_BREAK] = 1;
+      sc[S_DONE] = 1;
+    } else {
+      sc[S_ALPHA] = sc[S_RZ] / pap;
+    }
+  }
+}
+
+// x += alpha p; r -= alpha Ap; z = M^-1 r; partial r.z, r.r
+__global__ __launch_bounds__(256) void k_update_xrz(int64_t nn, double* x, double* r, double* z,
+                                                    const double* p, const double* Ap,
+                                                    const double* minv, double* part,
+                                                    const double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  __shared__ double red[16];
+  const double alpha = sc[S_ALPHA];
+  double v[2] = {0, 0};
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < nn; n += (int64_t)gridDim.x * 256) {
+    double rv[3], zv[3];
+    for (int d = 0; d < 3; d++) {
+      x[3 * n + d] += alpha * p[3 * n + d];
+      rv[d] = r[3 * n + d] - alpha * Ap[3 * n + d];
+      r[3 * n + d] = rv[d];
+    }
+    sym3_apply(minv + 6 * n, rv, zv);
+    for (int d = 0; d < 3; d++) {
+      z[3 * n + d] = zv[d];
+      v[0] += rv[d] * zv[d];
+      v[1] += rv[d] * rv[d];
+    }
+  }
+  block_sum<2, 256>(v, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v[0];
+    part[kRedBlocks + blockIdx.x] = v[1];
+  }
+}
+
+// convergence (pcg.ts:173-181) and beta (pcg.ts:194-198)
+__global__ __launch_bounds__(256) void k_fin_rz(const double* part, double* sc, double tol,
+                                                double maxit) {
+  if (sc[S_DONE] != 0.0) return;
+  __shared__ double red[16];
+  const double rz = sum_parts(part, red);
+  const double rr = sum_parts(part + kRedBlocks, red);
+  if (threadIdx.x == 0) {
+    sc[S_ITER] += 1;
+    sc[S_RR] = rr;
+    const double rn = sqrt(rr);
+    if (rn < tol || rn / sc[S_BNORM] < tol) {
+      sc[S_CONV] = 1;
+      sc[S_DONE] = 1;
+    } else if (sc[S_ITER] >= maxit) {
+      sc[S_DONE] = 1;
+    } else {
+      sc[S_BETA] = rz / sc[S_RZ];
+      sc[S_RZ] = rz;
+    }
+  }
+}
+
+__global__ void k_update_p(int64_t n, double* p, const double* z, const double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + sc[S_BETA] * p[i];
+}
+
+__global__ void k_copy(int64_t n, const double* s, double* d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int colour_pass(pp_plan* pl, int c, const double* dx, double* dy, bool skip_fixed,
+                const double* sc) {
+  const uint32_t off = pl->h_colour_off[c], cnt = pl->h_colour_off[c + 1] - off;
+  if (cnt == 0) return PP_OK;
+  hipStream_t s = pl->ctx->stream;
+  if (pl->npe == 4)
+    hipLaunchKernelGGL(k_colour<4>, dim3(nblk(cnt, 64)), dim3(64), 0, s, pl->by_colour.p + off,
+                       (int)cnt, pl->elems.p, pl->ke.p, pl->cmask_dof.p, dx, dy, (int)skip_fixed, sc);
+  else
+    hipLaunchKernelGGL(k_colour<3>, dim3(nblk(cnt, 64)), dim3(64), 0, s, pl->by_colour.p + off,
+                       (int)cnt, pl->elems.p, pl->ke.p, pl->cmask_dof.p, dx, dy, (int)skip_fixed, sc);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int apply_all(pp_plan* pl, const double* dx, double* dy, const double* sc) {
+  hipStream_t s = pl->ctx->stream;
+  hipLaunchKernelGGL(k_zero, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, (int64_t)pl->n_dof, dy, sc);
+  for (size_t c = 0; c + 1 < pl->h_colour_off.size(); c++) {
+    int rc = colour_pass(pl, (int)c, dx, dy, false, sc);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_bc_rows, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, (int64_t)pl->n_dof,
+                     (const uint8_t*)pl->cmask_dof.p, dx, dy, sc);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+}  // namespace
+
+int general_setup(pp_plan* pl, const pp_colouring* col) {
+  const int npe = pl->npe, nd = 3 * npe;
+  const int64_t ne = pl->n_elems, nn = pl->n_nodes;
+  std::vector<double> Ke((size_t)ne * nd * nd);
+  for (int64_t e = 0; e < ne; e++) {
+    double xy[8];
+    for (int i = 0; i < npe; i++) {
+      const uint32_t n = pl->h_elems[(size_t)e * npe + i];
+      xy[2 * i] = pl->h_nodes[2 * (size_t)n];
+      xy[2 * i + 1] = pl->h_nodes[2 * (size_t)n + 1];
+    }
+    const double t = pl->var_t ? (double)pl->h_elem_t[e] : pl->t;
+    if (npe == 4) ke_q4(xy, pl->E, pl->nu, t, &Ke[(size_t)e * nd * nd]);
+    else ke_dkt(xy, pl->E, pl->nu, t, &Ke[(size_t)e * nd * nd]);
+  }
+  // block diagonal (solver.ts:188-225) and inverse (solver.ts:231-304)
+  std::vector<double> blk((size_t)nn * 9, 0.0), minv((size_t)nn * 6);
+  for (int64_t e = 0; e < ne; e++) {
+    const double* K = &Ke[(size_t)e * nd * nd];
+    for (int i = 0; i < npe; i++) {
+      const uint32_t n = pl->h_elems[(size_t)e * npe + i];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) blk[(size_t)n * 9 + r * 3 + c] += K[(3 * i + r) * nd + 3 * i + c];
+    }
+  }
+  for (int64_t n = 0; n < nn; n++) {
+    double* A = &blk[(size_t)n * 9];
+    const unsigned fix = pl->h_cmask[n];
+    const bool cw = fix & PP_FIX_W, cx = fix & PP_FIX_THX, cy = fix & PP_FIX_THY;
+    double* o = &minv[(size_t)n * 6];
+    if (cw && cx && cy) { o[0] = 1; o[1] = 0; o[2] = 0; o[3] = 1; o[4] = 0; o[5] = 1; continue; }
+    if (cw) { A[0] = 1; A[1] = 0; A[2] = 0; A[3] = 0; A[6] = 0; }
+    if (cx) { A[4] = 1; A[1] = 0; A[7] = 0; A[3] = 0; A[5] = 0; }
+    if (cy) { A[8] = 1; A[2] = 0; A[5] = 0; A[6] = 0; A[7] = 0; }
+    const double a00 = A[0], a01 = A[1], a02 = A[2], a10 = A[3], a11 = A[4], a12 = A[5];
+    const double a20 = A[6], a21 = A[7], a22 = A[8];
+    const double det = a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20) +
+                       a02 * (a10 * a21 - a11 * a20);
+    if (fabs(det) < 1e-30) { o[0] = 1; o[1] = 0; o[2] = 0; o[3] = 1; o[4] = 0; o[5] = 1; continue; }
+    const double id = 1.0 / det;
+    o[0] = id * (a11 * a22 - a12 * a21);
+    o[1] = id * (a02 * a21 - a01 * a22);
+    o[2] = id * (a01 * a12 - a02 * a11);
+    o[3] = id * (a00 * a22 - a02 * a20);
+    o[4] = id * (a02 * a10 - a00 * a12);
+    o[5] = id * (a00 * a11 - a01 * a10);
+  }
+  std::vector<uint8_t> cdof((size_t)nn * 3);
+  for (int64_t n = 0; n < nn; n++)
+    for (int d = 0; d < 3; d++) cdof[(size_t)n * 3 + d] = (pl->h_cmask[n] >> d) & 1u;
+  int rc;
+  if ((rc = pl->ke.alloc(Ke.size())) || (rc = pl->minv.alloc(minv.size())) ||
+      (rc = pl->cmask_dof.alloc(cdof.size())) || (rc = pl->elems.alloc(pl->h_elems.size())) ||
+      (rc = pl->by_colour.alloc(std::max<size_t>(1, col->colour_offsets[col->colour_count]))) ||
+      (rc = pl->scal.alloc(S_N)) || (rc = pl->partials.alloc(2 * kRedBlocks)))
+    return rc;
+  hipStream_t s = pl->ctx->stream;
+  PP_HIP(hipMemcpyAsync(pl->ke.p, Ke.data(), Ke.size() * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(pl->minv.p, minv.data(), minv.size() * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(pl->cmask_dof.p, cdof.data(), cdof.size(), hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(pl->elems.p, pl->h_elems.data(), pl->h_elems.size() * 4,
+                        hipMemcpyHostToDevice, s));
+  const uint32_t total = col->colour_offsets[col->colour_count];
+  if (total)
+    PP_HIP(hipMemcpyAsync(pl->by_colour.p, col->elements_by_colour, (size_t)total * 4,
+                          hipMemcpyHostToDevice, s));
+  PP_HIP(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+int general_apply_k(pp_plan* pl, const double* dx, double* dy, int colour, bool bc_rows) {
+  if (colour < 0) return apply_all(pl, dx, dy, nullptr);
+  hipStream_t s = pl->ctx->stream;
+  hipLaunchKernelGGL(k_zero, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, (int64_t)pl->n_dof, dy,
+                     (const double*)nullptr);
+  int rc = colour_pass(pl, colour, dx, dy, !bc_rows, nullptr);
+  if (rc) return rc;
+  if (bc_rows)
+    hipLaunchKernelGGL(k_bc_rows, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, (int64_t)pl->n_dof,
+                       (const uint8_t*)pl->cmask_dof.p, dx, dy, (const double*)nullptr);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int general_apply_precond(pp_plan* pl, const double* dr, double* dz) {
+  hipLaunchKernelGGL(k_precond6, dim3(nblk(pl->n_nodes, 256)), dim3(256), 0, pl->ctx->stream,
+                     (int64_t)pl->n_nodes, (const double*)pl->minv.p, dr, dz, (const double*)nullptr);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int general_true_residual(pp_plan* pl, const double* dx, double* rel) {
+  int rc = apply_all(pl, dx, pl->tmp.p, nullptr);
+  if (rc) return rc;
+  return launch_norm_ratio(pl, pl->F.p, pl->tmp.p, rel);
+}
+
+// Host-driven device-resident PCG (pcg.ts:98-215), no per-iteration read-back.
+int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  hipStream_t s = pl->ctx->stream;
+  const int64_t nd = pl->n_dof, nn = pl->n_nodes;
+  double* sc = pl->scal.p;
+  double* part = pl->partials.p;
+  const int gb = nblk(nd, 256), gn = nblk(nn, 256);
+  PP_HIP(hipEventRecord(pl->ctx->ev0, s));
+  // x = 0, r = F, z = M^-1 r, p = z, rz = r.z, |F|
+  PP_HIP(hipMemsetAsync(pl->x.p, 0, nd * 8, s));
+  hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, s, nd, (const double*)pl->F.p, pl->r.p);
+  hipLaunchKernelGGL(k_precond6, dim3(gn), dim3(256), 0, s, nn, (const double*)pl->minv.p,
+                     (const double*)pl->r.p, pl->z.p, (const double*)nullptr);
+  hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, s, nd, (const double*)pl->z.p, pl->p0.p);
+  hipLaunchKernelGGL(k_dot2, dim3(kRedBlocks), dim3(256), 0, s, nd, (const double*)pl->r.p,
+                     (const double*)pl->z.p, (const double*)pl->F.p, (const double*)pl->F.p, part,
+                     (const double*)nullptr);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, s, (const double*)part, sc);
+  PP_HIP(hipGetLastError());
+  const double maxit = o->max_iterations;
+  double h_sc[S_N];
+  for (int done_it = 0;;) {
+    const int n_it = std::min(kBatch, o->max_iterations - done_it);
+    for (int k = 0; k < n_it; k++) {
+      int rc = apply_all(pl, pl->p0.p, pl->Ap.p, sc);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_dot2, dim3(kRedBlocks), dim3(256), 0, s, nd, (const double*)pl->p0.p,
+                         (const double*)pl->Ap.p, (const double*)nullptr, (const double*)nullptr,
+                         part, (const double*)sc);
+      hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(256), 0, s, (const double*)part, sc, maxit);
+      hipLaunchKernelGGL(k_update_xrz, dim3(kRedBlocks), dim3(256), 0, s, nn, pl->x.p, pl->r.p,
+                         pl->z.p, (const double*)pl->p0.p, (const double*)pl->Ap.p,
+                         (const double*)pl->minv.p, part, (const double*)sc);
+      hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(256), 0, s, (const double*)part, sc,
+                         o->tolerance, maxit);
+      hipLaunchKernelGGL(k_update_p, dim3(gb), dim3(256), 0, s, nd, pl->p0.p,
+                         (const double*)pl->z.p, (const double*)sc);
+      PP_HIP(hipGetLastError());
+    }
+    done_it += n_it;
+    PP_HIP(hipMemcpyAsync(h_sc, sc, sizeof(h_sc), hipMemcpyDeviceToHost, s));
+    PP_HIP(hipStreamSynchronize(s));
+    if (h_sc[S_DONE] != 0.0 || done_it >= o->max_iterations) break;
+  }
+  PP_HIP(hipEventRecord(pl->ctx->ev1, s));
+  PP_HIP(hipEventSynchronize(pl->ctx->ev1));
+  float ms = 0;
+  PP_HIP(hipEventElapsedTime(&ms, pl->ctx->ev0, pl->ctx->ev1));
+  st->iterations = (int)h_sc[S_ITER];
+  st->converged = h_sc[S_CONV] != 0.0;
+  st->breakdown = h_sc[S_BREAK] != 0.0;
+  st->final_residual = sqrt(h_sc[S_RR]);
+  st->device_ms = ms;
+  st->sweeps = 1;
+  st->sweep_iterations[0] = st->iterations;
+  return PP_OK;
+}
+
+// Consistent pressure load on a general mesh, on the host (set-up work).
+int general_pressure(pp_plan* pl, double q0, const double* pt, int np, float* F) {
+  const int npe = pl->npe;
+  std::vector<double> f((size_t)pl->n_dof, 0.0);
+  auto pres = [&](double x, double y) {
+    double p = q0;
+    for (int k = 0; k < np; k++) {
+      const double dx = x - pt[4 * k + 1], dy = y - pt[4 * k + 2], s = pt[4 * k + 3];
+      p += pt[4 * k] * exp(-(dx * dx + dy * dy) / (2 * s * s));
+    }
+    return p;
+  };
+  for (int64_t e = 0; e < pl->n_elems; e++) {
+    double xy[8];
+    uint32_t nodes[4];
+    for (int i = 0; i < npe; i++) {
+      nodes[i] = pl->h_elems[(size_t)e * npe + i];
+      xy[2 * i] = pl->h_nodes[2 * (size_t)nodes[i]];
+      xy[2 * i + 1] = pl->h_nodes[2 * (size_t)nodes[i] + 1];
+    }
+    if (npe == 4) {
+      const double g = 0.5773502691896257;
+      const double gp[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
+      for (int q = 0; q < 4; q++) {
+        const double xi = gp[q][0], eta = gp[q][1];
+        const double N[4] = {(1 - xi) * (1 - eta) / 4, (1 + xi) * (1 - eta) / 4,
+                             (1 + xi) * (1 + eta) / 4, (1 - xi) * (1 + eta) / 4};
+        const double dxi[4] = {-(1 - eta) / 4, (1 - eta) / 4, (1 + eta) / 4, -(1 + eta) / 4};
+        const double deta[4] = {-(1 - xi) / 4, -(1 + xi) / 4, (1 + xi) / 4, (1 - xi) / 4};
+        double J11 = 0, J12 = 0, J21 = 0, J22 = 0, px = 0, py = 0;
+        for (int i = 0; i < 4; i++) {
+          J11 += dxi[i] * xy[2 * i]; J12 += dxi[i] * xy[2 * i + 1];
+          J21 += deta[i] * xy[2 * i]; J22 += deta[i] * xy[2 * i + 1];
+          px += N[i] * xy[2 * i]; py += N[i] * xy[2 * i + 1];
+        }
+        const double w = pres(px, py) * fabs(J11 * J22 - J12 * J21);
+        for (int i = 0; i < 4; i++) f[3 * (size_t)nodes[i]] += N[i] * w;
+      }
+    } else {
+      const double gL[3][3] = {{1.0 / 6, 1.0 / 6, 2.0 / 3}, {2.0 / 3, 1.0 / 6, 1.0 / 6},
+                               {1.0 / 6, 2.0 / 3, 1.0 / 6}};
+      const double twoA = (xy[0] - xy[2]) * (xy[5] - xy[1]) - (xy[4] - xy[0]) * (xy[1] - xy[3]);
+      const double area = fabs(twoA) / 2;
+      for (int q = 0; q < 3; q++) {
+        const double px = gL[q][0] * xy[0] + gL[q][1] * xy[2] + gL[q][2] * xy[4];
+        const double py = gL[q][0] * xy[1] + gL[q][1] * xy[3] + gL[q][2] * xy[5];
+        const double w = pres(px, py) * area / 3.0;
+        for (int i = 0; i < 3; i++) f[3 * (size_t)nodes[i]] += gL[q][i] * w;
+      }
+    }
+  }
+  for (int64_t n = 0; n < pl->n_nodes; n++)
+    for (int d = 0; d < 3; d++)
+      F[3 * n + d] = (d == 0 && !(pl->h_cmask[n] & PP_FIX_W)) ? (float)f[3 * n] : 0.0f;
+  return PP_OK;
+}
+
+}  // namespace pp

@@ -1,0 +1,146 @@
+// Internal declarations shared by the platepcg translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/platepcg.h"
+
+namespace pp {
+
+// ------------------------------------------------------------ errors
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define PP_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::pp::fail(PP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ----------------------------------------------------- device buffers
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    release();
+    if (count == 0) return PP_OK;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(PP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    n = count;
+    return PP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Grid barrier state for the persistent kernels: a monotonically increasing
+// arrival counter and an abort flag raised by a workgroup that gave up
+// waiting.  Zeroed before every persistent launch.
+struct GridSync {
+  unsigned int count;
+  unsigned int abort;
+  unsigned int pad[62];
+};
+
+// Device-resident solve statistics written by the persistent kernels.
+struct DevStats {
+  int iterations;
+  int converged;
+  int breakdown;
+  int pad;
+  double final_residual;
+  double bnorm;
+};
+
+}  // namespace pp
+
+struct pp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cu_count = 0;
+};
+
+struct pp_plan {
+  pp_ctx* ctx = nullptr;
+  int64_t n_nodes = 0, n_elems = 0, n_dof = 0;
+  int npe = 4;
+  bool structured = false;
+  int nx = 0, ny = 0;
+  double E = 0, nu = 0, t = 0;
+  bool var_t = false;
+
+  // host copies
+  std::vector<float> h_nodes;
+  std::vector<uint32_t> h_elems;
+  std::vector<uint32_t> h_colour_off;
+  std::vector<uint8_t> h_cmask;
+  std::vector<float> h_elem_t;
+
+  // device data
+  pp::DevBuf<double> xc, yc;          // structured: grid coordinates
+  pp::DevBuf<float> elem_t;           // per-element thickness (optional)
+  pp::DevBuf<uint8_t> cmask;          // per-node PP_FIX bits
+  pp::DevBuf<uint8_t> cmask_dof;      // per-DOF 0/1 (coloured path)
+  pp::DevBuf<double> minv;            // 6 per node (symmetric 3x3 inverse)
+  pp::DevBuf<double> ke;              // coloured path: per-element FP64 Ke
+  pp::DevBuf<uint32_t> elems;         // coloured path: connectivity
+  pp::DevBuf<uint32_t> by_colour;     // coloured path: elements by colour
+
+  // solve state (FP64)
+  pp::DevBuf<double> F, x, r, z, p0, p1, Ap, tmp;
+  pp::DevBuf<float> F32;
+  pp::DevBuf<double> partials;        // grid-reduction partials
+  pp::DevBuf<pp::GridSync> gsync;
+  pp::DevBuf<pp::DevStats> dstats;
+  pp::DevBuf<double> scal;            // coloured path device scalars
+
+  // mixed precision state (FP32 inner solve)
+  pp::DevBuf<float> xs, rs, zs, ps0, ps1, Aps;
+  pp::DevBuf<double> xacc;
+
+  bool rhs_set = false;
+  double rhs_norm = 0;
+};
+
+namespace pp {
+
+// structured kernels (pp_struct.hip)
+int struct_setup(pp_plan* pl);
+int struct_apply_k(pp_plan* pl, const double* dx, double* dy);
+int struct_apply_precond(pp_plan* pl, const double* dr, double* dz);
+int struct_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF);
+int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+int struct_solve_mixed(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+int struct_true_residual(pp_plan* pl, const double* dx, double* rel);
+
+// coloured kernels (pp_general.hip)
+int general_setup(pp_plan* pl, const pp_colouring* col);
+int general_apply_k(pp_plan* pl, const double* dx, double* dy, int colour, bool bc_rows);
+int general_apply_precond(pp_plan* pl, const double* dr, double* dz);
+int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+int general_true_residual(pp_plan* pl, const double* dx, double* rel);
+int general_pressure(pp_plan* pl, double q0, const double* patches, int np, float* F);
+
+// shared helpers (pp_api.hip)
+int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel);
+
+}  // namespace pp

@@ -1,0 +1,11 @@
+"""platepcg: host side of the MI355X plate-bending PCG library.
+
+The compute runs in the HIP library ``lib/libplatepcg.so`` (gfx950); this
+package binds its C ABI and builds problem inputs.
+"""
+from . import _lib, problem
+from ._lib import PlatePCGError, load
+from .solver import Context, Plan, solve_gpu, stats_dict
+
+__all__ = ["Context", "Plan", "solve_gpu", "stats_dict", "PlatePCGError", "load", "problem",
+           "_lib"]

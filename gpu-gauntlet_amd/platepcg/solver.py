@@ -1,0 +1,193 @@
+"""Python host API over the C ABI, with the reference's solver call shape.
+
+``solve_gpu(mesh, material, coloring, F, constrained_dofs, options)`` mirrors
+``solveGPU`` (gpu/solver.ts:80-104) and returns the fields of
+``GPUSolveResult`` (gpu/solver.ts:47-60) plus ``solution64`` and
+``trueRelResidual``.  Unlike the reference there is no CPU fallback: a GPU or
+library failure raises ``PlatePCGError`` (``forceCPU`` is rejected).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One HIP device + stream (gpu/context.ts:31-106)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.pp_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.pp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Set-up for one mesh/material/support set; solve many load cases."""
+
+    def __init__(self, ctx, mesh, E, nu, t, mask, colouring=None, element_t=None):
+        self.ctx, self.lib = ctx, ctx.lib
+        self._keep = []
+        nodes = np.ascontiguousarray(mesh["nodes"], np.float32)
+        elems = np.ascontiguousarray(mesh["elements"], np.uint32)
+        self.n_nodes = int(mesh["nodeCount"])
+        self.n_dof = 3 * self.n_nodes
+        m = _lib.Mesh(nodes.ctypes.data, elems.ctypes.data, self.n_nodes, int(mesh["elementCount"]),
+                      int(mesh.get("nodesPerElement", 4)), int(mesh.get("gridNx", 0)),
+                      int(mesh.get("gridNy", 0)))
+        et = None if element_t is None else np.ascontiguousarray(element_t, np.float32)
+        mat = _lib.Material(float(E), float(nu), float(t), et.ctypes.data if et is not None else None)
+        mask = np.ascontiguousarray(mask, np.uint8)
+        col_p = None
+        if colouring is not None:
+            order = np.ascontiguousarray(colouring["elements_by_colour"], np.uint32)
+            offs = np.ascontiguousarray(colouring["colour_offsets"], np.uint32)
+            col = _lib.Colouring(order.ctypes.data, offs.ctypes.data, int(colouring["colour_count"]))
+            col_p = C.byref(col)
+            self._keep += [order, offs]
+        h = C.c_void_p()
+        check(self.lib.pp_plan_create(ctx.h, C.byref(m), C.byref(mat), col_p, _ptr(mask), C.byref(h)))
+        self.h = h
+        self.structured = bool(self.lib.pp_plan_is_structured(h))
+
+    def close(self):
+        if self.h:
+            self.lib.pp_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- unit entry points
+    def apply_k(self, x, path=_lib.PATH_AUTO):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.n_dof, np.float64)
+        check(self.lib.pp_apply_k(self.h, _ptr(x), _ptr(y), path))
+        return y
+
+    def apply_k_colour(self, colour, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.n_dof, np.float64)
+        check(self.lib.pp_apply_k_colour(self.h, colour, _ptr(x), _ptr(y)))
+        return y
+
+    def apply_precond(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty(self.n_dof, np.float64)
+        check(self.lib.pp_apply_precond(self.h, _ptr(r), _ptr(z)))
+        return z
+
+    def block_inverse(self):
+        b = np.empty(9 * self.n_nodes, np.float64)
+        check(self.lib.pp_get_block_inverse(self.h, _ptr(b)))
+        return b
+
+    def pressure_load(self, q0, patches=None):
+        pt = np.ascontiguousarray(np.zeros((0, 4)) if patches is None else patches, np.float64)
+        F = np.empty(self.n_dof, np.float32)
+        check(self.lib.pp_pressure_load(self.h, float(q0), _ptr(pt) if pt.size else None,
+                                        int(pt.shape[0]), _ptr(F)))
+        return F
+
+    # ---- solve
+    @staticmethod
+    def options(tolerance=1e-8, max_iterations=20000, precision=_lib.PRECISION_FP64,
+                precond=_lib.PRECOND_BLOCK_JACOBI, path=_lib.PATH_AUTO, inner_tolerance=1e-4,
+                max_refinements=12, flags=0):
+        return _lib.SolveOpts(tolerance, max_iterations, precision, precond, path,
+                              inner_tolerance, max_refinements, flags)
+
+    def set_rhs(self, F):
+        F = np.ascontiguousarray(F, np.float32)
+        check(self.lib.pp_set_rhs(self.h, _ptr(F)))
+
+    def solve_resident(self, opts):
+        st = _lib.SolveStats()
+        check(self.lib.pp_solve_resident(self.h, C.byref(opts), C.byref(st)))
+        return st
+
+    def solution(self):
+        x64 = np.empty(self.n_dof, np.float64)
+        x32 = np.empty(self.n_dof, np.float32)
+        check(self.lib.pp_get_solution(self.h, _ptr(x64), _ptr(x32)))
+        return x64, x32
+
+    def solve(self, F, **kw):
+        self.set_rhs(F)
+        st = self.solve_resident(self.options(**kw))
+        x64, x32 = self.solution()
+        return x64, x32, stats_dict(st)
+
+
+def stats_dict(st):
+    n = max(1, st.sweeps)
+    return {
+        "iterations": st.iterations, "converged": bool(st.converged),
+        "breakdown": bool(st.breakdown), "sweeps": st.sweeps,
+        "finalResidual": st.final_residual, "trueRelResidual": st.true_rel_residual,
+        "deviceMs": st.device_ms,
+        "sweepIterations": [st.sweep_iterations[i] for i in range(min(n, _lib.MAX_SWEEPS))],
+        "sweepResidual": [st.sweep_residual[i] for i in range(min(n, _lib.MAX_SWEEPS))],
+    }
+
+
+_ctx_cache = {}
+
+
+def solve_gpu(mesh, material, coloring, F, constrained_dofs, options=None):
+    """solveGPU (gpu/solver.ts:80-104) over the HIP library.
+
+    mesh: dict as PlateMesh; material: {E, nu, t}; coloring: {colors: [..]}
+    or the packed form; F: float32 load vector; constrained_dofs: iterable of
+    DOF indices; options: {tolerance, maxIterations, forceCPU}.
+    """
+    from .problem import dofs_to_mask
+
+    options = options or {}
+    if options.get("forceCPU"):
+        raise _lib.PlatePCGError(_lib.PP_ERR_UNSUPPORTED, "forceCPU: this library has no CPU path")
+    t0 = time.perf_counter()
+    ctx = _ctx_cache.get(0)
+    if ctx is None:
+        ctx = _ctx_cache[0] = Context(0)
+    col = coloring
+    if coloring is not None and "colors" in coloring:
+        order = np.concatenate([np.asarray(c, np.uint32) for c in coloring["colors"]])
+        offs = np.concatenate([[0], np.cumsum([len(c) for c in coloring["colors"]])])
+        col = {"elements_by_colour": order, "colour_offsets": offs.astype(np.uint32),
+               "colour_count": len(coloring["colors"])}
+    mask = dofs_to_mask(int(mesh["nodeCount"]), constrained_dofs)
+    plan = Plan(ctx, mesh, material["E"], material["nu"], material["t"], mask, col,
+                material.get("element_t"))
+    try:
+        x64, x32, st = plan.solve(F, tolerance=options.get("tolerance", 1e-6),
+                                  max_iterations=options.get("maxIterations", 1000))
+    finally:
+        plan.close()
+    return {
+        "solution": x32, "iterations": st["iterations"], "finalResidual": st["finalResidual"],
+        "converged": st["converged"], "gpuTimeMs": (time.perf_counter() - t0) * 1e3,
+        "usedGPU": True, "solution64": x64, "trueRelResidual": st["trueRelResidual"],
+        "deviceMs": st["deviceMs"],
+    }

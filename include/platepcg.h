@@ -1,0 +1,179 @@
+/*
+ * platepcg: MI355X (gfx950) plate-bending PCG solve behind a C ABI.
+ *
+ * Every entry point replaces a piece of the reference's GPU solve path
+ * (cseifert512/gpu-gauntlet, src/lib/plate/gpu).  Plain pointers and sizes
+ * only; all host pointers are borrowed for the duration of the call and every
+ * output buffer is caller-allocated.  Functions return PP_OK (0) or a
+ * negative PP_ERR_* code; pp_last_error() then holds a message for the
+ * calling thread.
+ *
+ * Vectors use the reference's DOF numbering: 3 DOFs per node [w, thx, thy],
+ * DOF index = 3*node + k (types.ts:175-188).
+ */
+#ifndef PLATEPCG_H
+#define PLATEPCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_OK 0
+#define PP_ERR_INVALID (-1)     /* bad argument or inconsistent mesh */
+#define PP_ERR_HIP (-2)         /* HIP runtime error */
+#define PP_ERR_UNSUPPORTED (-3) /* option not available for this plan */
+#define PP_ERR_NOMEM (-4)       /* device allocation failed */
+#define PP_ERR_TIMEOUT (-5)     /* a persistent kernel's grid barrier gave up */
+#define PP_ERR_NODEVICE (-6)    /* no HIP device */
+
+/* Constrained-DOF bits per node in the mask arrays (solver.ts:92-134). */
+#define PP_FIX_W 1u
+#define PP_FIX_THX 2u
+#define PP_FIX_THY 4u
+
+/* Mesh, as PlateMesh (types.ts:73-89).  boundaryNodes is not needed here:
+ * constraints arrive as a per-DOF mask (gpu/solver.ts:80-87 receives them as
+ * a Set<number>; buffers.ts:225-235 turns that into a per-DOF mask). */
+typedef struct {
+  const float* nodes;        /* 2 per node: x, y (float32, as mesher.ts) */
+  const uint32_t* elements;  /* nodes_per_element per element, CCW */
+  uint32_t node_count;
+  uint32_t element_count;
+  uint32_t nodes_per_element; /* 4 = Q4 Mindlin, 3 = DKT triangle */
+  uint32_t grid_nx;           /* structured grid size, 0 when unstructured */
+  uint32_t grid_ny;
+} pp_mesh;
+
+/* Material, as PlateMaterial (types.ts:14-21).  element_t is an extension
+ * with no reference counterpart: per-element thickness, NULL = uniform t. */
+typedef struct {
+  double E;
+  double nu;
+  double t;
+  const float* element_t;
+} pp_material;
+
+/* Element colouring, as ElementColoring (types.ts:95-102): colour c holds
+ * elements_by_colour[colour_offsets[c] .. colour_offsets[c+1]). */
+typedef struct {
+  const uint32_t* elements_by_colour;
+  const uint32_t* colour_offsets; /* colour_count + 1 entries */
+  uint32_t colour_count;
+} pp_colouring;
+
+#define PP_PRECISION_FP64 0  /* FP64 PCG throughout */
+#define PP_PRECISION_MIXED 1 /* FP32 inner PCG, FP64 iterative refinement */
+
+#define PP_PRECOND_BLOCK_JACOBI 0 /* 3x3 nodal block Jacobi (solver.ts:188-339) */
+#define PP_PRECOND_LEARNED 1      /* CNN preconditioner, flexible PCG */
+
+#define PP_PATH_AUTO 0       /* structured kernels when the mesh is a grid */
+#define PP_PATH_STRUCTURED 1 /* tensor-grid tile kernels (error if not a grid) */
+#define PP_PATH_COLOURED 2   /* stored-Ke element kernels, one pass per colour */
+
+/* Solver options, as GPUSolverOptions (gpu/solver.ts:35-42) plus the
+ * precision / preconditioner / kernel-path choices of this library. */
+typedef struct {
+  double tolerance;      /* stop when |r| < tol or |r|/|F| < tol (pcg.ts:174) */
+  int32_t max_iterations;
+  int32_t precision;     /* PP_PRECISION_* */
+  int32_t precond;       /* PP_PRECOND_* */
+  int32_t path;          /* PP_PATH_* */
+  double inner_tolerance;  /* mixed: relative tolerance of each inner solve */
+  int32_t max_refinements; /* mixed: outer sweeps */
+  int32_t flags;           /* PP_FLAG_* */
+} pp_solve_opts;
+
+/* Skip the FP64 |F - K x| / |F| recomputation after the solve
+ * (stats.true_rel_residual is then 0). */
+#define PP_FLAG_NO_TRUE_RESIDUAL 1
+
+#define PP_MAX_SWEEPS 32
+
+/* Solve statistics, as GPUSolveResult (gpu/solver.ts:47-60) minus the
+ * solution, plus what the FP64 check needs. */
+typedef struct {
+  int32_t iterations;         /* total PCG iterations (all sweeps) */
+  int32_t converged;
+  int32_t breakdown;          /* |pAp| < 1e-30 (pcg.ts:154-161) */
+  int32_t sweeps;             /* mixed: refinement sweeps done */
+  double final_residual;      /* |r| of the recurrence at exit */
+  double true_rel_residual;   /* |F - K x| / |F| recomputed in FP64 */
+  double device_ms;           /* HIP-event time of the resident solve */
+  int32_t sweep_iterations[PP_MAX_SWEEPS];
+  double sweep_residual[PP_MAX_SWEEPS]; /* true FP64 rel. residual after each sweep */
+} pp_solve_stats;
+
+typedef struct pp_ctx pp_ctx;
+typedef struct pp_plan pp_plan;
+
+/* Capability probe; replaces isWebGPUAvailable (gpu/context.ts:19-21).
+ * Returns the number of HIP devices (0 = none). */
+int pp_device_count(void);
+
+/* Message of the last failing call on this thread. */
+const char* pp_last_error(void);
+
+/* Device + stream; replaces initGPU / the cached context
+ * (gpu/context.ts:31-106). */
+int pp_ctx_create(int device, pp_ctx** out);
+int pp_ctx_destroy(pp_ctx* ctx);
+
+/* One-time set-up for a mesh: replaces the set-up half of solveGPUInternal
+ * (gpu/solver.ts:118-150: block diagonal + inverse, buffer creation and
+ * uploads, colour offsets).  The FP64 3x3 block inverses are built on the
+ * device.  constrained_mask holds PP_FIX_* bits per node. */
+int pp_plan_create(pp_ctx* ctx, const pp_mesh* mesh, const pp_material* material,
+                   const pp_colouring* colouring, const uint8_t* constrained_mask,
+                   pp_plan** out);
+int pp_plan_destroy(pp_plan* plan);
+
+/* 1 when the plan runs the structured tile kernels, 0 for the coloured path. */
+int pp_plan_is_structured(const pp_plan* plan);
+
+/* Upload F and make it the resident right-hand side: replaces
+ * device.queue.writeBuffer(buffers.r, F) (gpu/solver.ts:153). */
+int pp_set_rhs(pp_plan* plan, const float* F);
+
+/* PCG on the resident F; x stays resident.  Replaces runGPUPCG
+ * (gpu/solver.ts:190-553), with the whole loop on the device. */
+int pp_solve_resident(pp_plan* plan, const pp_solve_opts* opts, pp_solve_stats* stats);
+
+/* Read the resident solution: replaces readFromBuffer(x) (buffers.ts:315).
+ * Either pointer may be NULL. */
+int pp_get_solution(pp_plan* plan, double* x64, float* x32);
+
+/* set_rhs + solve_resident + get_solution: the whole of solveGPU
+ * (gpu/solver.ts:80-185) for one load case. */
+int pp_solve(pp_plan* plan, const float* F, const pp_solve_opts* opts, float* x32, double* x64,
+             pp_solve_stats* stats);
+
+/* y = K x with the reference's BC rows (y[c] = x[c] on constrained DOFs):
+ * replaces gpuApplyK (gpu/solver.ts:413-461) + apply_k_q4/apply_k_dkt +
+ * apply_bc.  path picks the kernels (PP_PATH_*). */
+int pp_apply_k(pp_plan* plan, const double* x, double* y, int path);
+
+/* One colour only, scatter skipping constrained DOFs and no BC rows: the
+ * GPU side of applySingleColorKx (gpu/single_color_test.ts:44-171). */
+int pp_apply_k_colour(pp_plan* plan, int colour, const double* x, double* y);
+
+/* z = M^-1 r with the FP64 block inverses: replaces the block
+ * preconditioner shader (gpu/shaders/index.ts:214-262). */
+int pp_apply_precond(pp_plan* plan, const double* r, double* z);
+
+/* The FP64 block inverses, 9 per node row-major (solver.ts:231-304). */
+int pp_get_block_inverse(pp_plan* plan, double* blocks);
+
+/* Consistent nodal pressure load for p(x,y) = q0 + sum_k A_k
+ * exp(-|x-c_k|^2 / (2 s_k^2)); patches holds (A, cx, cy, s) per patch.  F is
+ * zero on constrained DOFs and on the rotation DOFs.  No reference
+ * counterpart: the reference has only point loads (solver.ts:363-392). */
+int pp_pressure_load(pp_plan* plan, double q0, const double* patches, int n_patches, float* F);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLATEPCG_H */

@@ -1,0 +1,155 @@
+"""Parity of the HIP library with the CPU oracles, through the C ABI."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_api as O
+from platepcg import Context, Plan, _lib
+from platepcg import problem as P
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+E, NU = 210e9, 0.3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def make(ctx, w, coloured=True, triangles=False):
+    mesh = w["mesh"]
+    if triangles:
+        mesh = P.split_quads(mesh)
+    col = None
+    if coloured:
+        col = P.colour_greedy(mesh) if triangles else P.colour_structured(mesh)
+    plan = Plan(ctx, mesh, w["E"], w["nu"], w["t"], w["mask"], col)
+    op = O.OracleProblem(mesh, w["E"], w["nu"], w["t"], w["mask"],
+                         order=None if col is None else col["elements_by_colour"])
+    return plan, op, mesh, col
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_apply_k_matches_oracle(ctx, name):
+    plan, op, _, _ = make(ctx, P.workload(name))
+    assert plan.structured
+    x = np.random.default_rng(1).standard_normal(plan.n_dof)
+    y_ref = op.apply_k(x)
+    assert rel(plan.apply_k(x, _lib.PATH_STRUCTURED), y_ref) < 1e-12
+    assert rel(plan.apply_k(x, _lib.PATH_COLOURED), y_ref) < 1e-12
+
+
+def test_single_colour_apply_matches_oracle(ctx):
+    """gpu/single_color_test.ts:175-237: one colour's K.x, constrained DOFs
+    skipped in the scatter."""
+    plan, op, mesh, col = make(ctx, P.workload("c1"))
+    x = np.random.default_rng(2).standard_normal(plan.n_dof)
+    nd = 12
+    Ke = op.Ke.reshape(-1, nd, nd)
+    el = mesh["elements"].reshape(-1, 4).astype(np.int64)
+    off = col["colour_offsets"]
+    for c in range(col["colour_count"]):
+        y = np.zeros(plan.n_dof)
+        for e in col["elements_by_colour"][off[c]:off[c + 1]]:
+            dofs = (el[e][:, None] * 3 + np.arange(3)).ravel()
+            ye = Ke[e] @ x[dofs]
+            keep = op.cdof[dofs] == 0
+            y[dofs[keep]] += ye[keep]
+        assert rel(plan.apply_k_colour(c, x), y) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_block_inverse_and_precond_match_oracle(ctx, name):
+    plan, op, _, _ = make(ctx, P.workload(name), coloured=False)
+    b = plan.block_inverse().reshape(-1, 9)
+    ob = op.blk.reshape(-1, 9)
+    assert np.max(np.abs(b - ob) / np.abs(ob).max(axis=1, keepdims=True)) < 1e-10
+    r = np.random.default_rng(3).standard_normal(plan.n_dof)
+    assert rel(plan.apply_precond(r), op.precond(r)) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_pressure_load_matches_oracle(ctx, name):
+    w = P.workload(name)
+    plan, op, _, _ = make(ctx, w, coloured=False)
+    F = plan.pressure_load(w["q0"], w["patches"])
+    Fo = op.pressure_load(w["q0"], w["patches"]).astype(np.float32)
+    np.testing.assert_allclose(F, Fo, rtol=2e-7, atol=0)
+
+
+@pytest.mark.parametrize("path", [_lib.PATH_STRUCTURED, _lib.PATH_COLOURED])
+def test_solve_c1_matches_oracles(ctx, path):
+    w = P.workload("c1")
+    plan, op, _, _ = make(ctx, w)
+    F = plan.pressure_load(w["q0"])
+    x64, x32, st = plan.solve(F, tolerance=1e-8, max_iterations=20000, path=path)
+    assert st["converged"] and not st["breakdown"]
+    xo, so = op.pcg(F.astype(np.float64), tol=1e-8)
+    assert abs(st["iterations"] - so["iterations"]) <= 3
+    assert st["trueRelResidual"] <= 1e-8
+    assert op.true_rel_residual(x64, F.astype(np.float64)) <= 1e-8
+    assert O.max_nodal_w_error(x64, xo) < 1e-6
+    xj, js = O.run_js_solver(32, F, E, NU, 0.1, [{"type": "fixed", "location": "all_edges"}])
+    assert O.compare_max_w(x32, xj) < 1e-4
+    assert O.max_nodal_w_error(x32, xj) < 1e-3
+
+
+def test_solve_c2_matches_golden(ctx):
+    """61,347 DOF: FP64 solve vs the FP64 oracle and the float32 solver.ts
+    restatement (golden vectors)."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "c2_142.npz"))
+    w = P.workload("c2")
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    F = plan.pressure_load(w["q0"], w["patches"])
+    np.testing.assert_allclose(F, g["F"], rtol=2e-7, atol=0)
+    x64, x32, st = plan.solve(g["F"], tolerance=1e-8, max_iterations=20000)
+    assert st["converged"]
+    assert st["trueRelResidual"] <= 1e-8, st
+    assert O.max_nodal_w_error(x64, g["x64"]) < 1e-6
+    assert O.compare_max_w(x32, g["x_js"]) < 1e-4
+    assert O.max_nodal_w_error(x32, g["x_js"]) < 1e-3
+    op = O.OracleProblem(w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    assert op.true_rel_residual(x64, g["F"].astype(np.float64)) <= 1e-8
+
+
+def test_dkt_triangles_coloured_path(ctx):
+    w = P.workload("c1")
+    w["mesh"] = P.rect_mesh(16)
+    w["mask"] = P.constrained_mask(w["mesh"], [{"type": "fixed", "location": "all_edges"}])
+    plan, op, mesh, col = make(ctx, w, triangles=True)
+    assert not plan.structured
+    x = np.random.default_rng(4).standard_normal(plan.n_dof)
+    assert rel(plan.apply_k(x), op.apply_k(x)) < 1e-12
+    F = plan.pressure_load(1e4)
+    np.testing.assert_allclose(F, op.pressure_load(1e4).astype(np.float32), rtol=2e-7)
+    x64, x32, st = plan.solve(F, tolerance=1e-8, max_iterations=5000)
+    xo, so = op.pcg(F.astype(np.float64), tol=1e-8)
+    assert st["converged"] and abs(st["iterations"] - so["iterations"]) <= 2
+    assert O.max_nodal_w_error(x64, xo) < 1e-6
+
+
+def test_edge_cases(ctx):
+    w = P.workload("c1")
+    plan, op, _, _ = make(ctx, w)
+    for path in (_lib.PATH_STRUCTURED, _lib.PATH_COLOURED):
+        x64, _, st = plan.solve(np.zeros(plan.n_dof, np.float32), path=path)
+        assert st["converged"] and st["iterations"] == 0 and not x64.any()
+        F = plan.pressure_load(1e4)
+        x64, _, st = plan.solve(F, max_iterations=1, path=path)
+        assert not st["converged"] and st["iterations"] == 1
+        xo, so = op.pcg(F.astype(np.float64), max_iter=1)
+        assert rel(x64, xo) < 1e-12
+    # every DOF constrained: identity system
+    mask = np.full(w["mesh"]["nodeCount"], 7, np.uint8)
+    p2 = Plan(ctx, w["mesh"], E, NU, 0.1, mask, P.colour_structured(w["mesh"]))
+    F = np.random.default_rng(5).standard_normal(p2.n_dof).astype(np.float32)
+    x64, _, st = p2.solve(F)
+    assert st["converged"] and rel(x64, F.astype(np.float64)) < 1e-12

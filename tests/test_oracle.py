@@ -1,0 +1,271 @@
+"""Pins the CPU oracles (FP64 C and float32 JS) before anything is compared
+with them: known answers, element invariants, closed-form plate solutions, an
+assembled sparse direct solve, and agreement of the two independent
+restatements.  CPU only."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle_api as O
+from platepcg import problem as P
+
+E, NU = 210e9, 0.3
+
+
+def rect_xy(a, b, x0=0.0, y0=0.0):
+    return np.array([x0, y0, x0 + a, y0, x0 + a, y0 + b, x0, y0 + b])
+
+
+# ----------------------------------------------------- known answers (pcg.ts:221-247)
+
+def test_c_oracle_tridiagonal_known_answer():
+    x, it = O.test_system()
+    assert it > 0
+    np.testing.assert_allclose(x, np.ones(10), rtol=1e-12)
+
+
+def test_js_oracle_tridiagonal_known_answer():
+    r = O.run_js("const r = O.solveTestSystem(); console.log(JSON.stringify("
+                 "{x: Array.from(r.solution), c: r.converged, it: r.iterations}));")
+    assert r["c"]
+    np.testing.assert_allclose(r["x"], np.ones(10), rtol=1e-6)
+
+
+# ----------------------------------------------------- Q4 element (element.ts:258-483)
+
+@pytest.mark.parametrize("xy", [rect_xy(0.3, 0.2), rect_xy(0.07042253, 0.07042254, 1.0, 2.0),
+                                np.array([0.0, 0.0, 1.0, 0.1, 1.2, 0.9, -0.1, 1.1])])
+def test_q4_ke_invariants(xy):
+    K = O.ke_q4(xy, E, NU, 0.1)
+    np.testing.assert_allclose(K, K.T, rtol=0, atol=1e-9 * np.abs(K).max())
+    lam = np.linalg.eigvalsh(0.5 * (K + K.T))
+    tol = 1e-10 * lam.max()
+    assert np.sum(np.abs(lam) < tol) == 5, lam      # 3 rigid + 2 spurious (1-pt shear)
+    assert np.all(lam > -tol)
+    x, y = xy[0::2], xy[1::2]
+    modes = []
+    u = np.zeros(12); u[0::3] = 1; modes.append(u)                          # w = 1
+    u = np.zeros(12); u[0::3] = y; u[1::3] = 1; modes.append(u)            # w = y, thx = 1
+    u = np.zeros(12); u[0::3] = x; u[2::3] = -1; modes.append(u)           # w = x, thy = -1
+    for u in modes:
+        assert np.linalg.norm(K @ u) < 1e-9 * np.abs(K).max() * np.linalg.norm(u)
+    # a wrong sign in the shear coupling breaks the rotation modes
+    u = np.zeros(12); u[0::3] = y; u[1::3] = -1
+    assert np.linalg.norm(K @ u) > 1e-3 * np.abs(K).max()
+
+
+def test_q4_rectangle_hourglass_in_null_space():
+    K = O.ke_q4(rect_xy(0.5, 0.25), E, NU, 0.1)
+    u = np.zeros(12); u[0::3] = [1, -1, 1, -1]
+    assert np.linalg.norm(K @ u) < 1e-10 * np.abs(K).max()
+
+
+def test_q4_thickness_scaling():
+    """Bending scales with t^3, shear with t (element.ts:205,236-237)."""
+    xy = rect_xy(0.4, 0.3)
+    K1, K2 = O.ke_q4(xy, E, NU, 0.1), O.ke_q4(xy, E, NU, 0.2)
+    # w-w block is shear only, theta blocks mix: check w-w
+    ww = np.ix_([0, 3, 6, 9], [0, 3, 6, 9])
+    np.testing.assert_allclose(K2[ww], 2 * K1[ww], rtol=1e-12)
+
+
+def test_js_and_c_ke_agree():
+    xs = [rect_xy(10 / 142, 10 / 142, 1.0, 2.0).tolist(), [0.0, 0.0, 1.0, 0.1, 1.2, 0.9, -0.1, 1.1]]
+    r = O.run_js("const out = []; for (const xy of %s) out.push(Array.from("
+                 "O.computeElementStiffness(new Float32Array(xy), {E: 210e9, nu: 0.3, t: 0.1})));"
+                 "console.log(JSON.stringify(out));" % xs)
+    for xy, kj in zip(xs, r):
+        xy32 = np.asarray(xy, np.float32).astype(np.float64)
+        Kc = O.ke_q4(xy32, E, NU, 0.1).ravel()
+        np.testing.assert_allclose(kj, Kc, rtol=0, atol=2e-6 * np.abs(Kc).max())
+
+
+# ----------------------------------------------------- "DKT" (element.ts:642-943)
+
+@pytest.mark.parametrize("xy", [[0, 0, 1, 0, 1, 1], [0, 0, 1, 1, 0, 1], [0, 0, 0.7, 0.1, 0.2, 0.9]])
+def test_dkt_fingerprints(xy):
+    """The reference triangle as written: symmetric, rank 8, and it does not
+    annihilate the rigid translation w = 1."""
+    K = O.ke_dkt(np.asarray(xy, np.float64), E, NU, 0.1)
+    np.testing.assert_allclose(K, K.T, rtol=0, atol=1e-9 * np.abs(K).max())
+    lam = np.linalg.eigvalsh(K)
+    assert np.sum(np.abs(lam) < 1e-9 * lam.max()) == 1
+    u = np.array([1, 0, 0, 1, 0, 0, 1, 0, 0.0])
+    assert np.linalg.norm(K @ u) > 1e-3 * np.abs(K).max()
+
+
+def test_dkt_degenerate_is_zero():
+    assert np.all(O.ke_dkt(np.array([0, 0, 1, 1, 2, 2.0]), E, NU, 0.1) == 0)
+
+
+def test_js_and_c_dkt_agree():
+    xs = [[0, 0, 1, 0, 1, 1], [0.0, 0.0, 0.7, 0.1, 0.2, 0.9]]
+    r = O.run_js("const out = []; for (const xy of %s) out.push(Array.from("
+                 "O.computeDKTStiffness(new Float32Array(xy), {E: 210e9, nu: 0.3, t: 0.1})));"
+                 "console.log(JSON.stringify(out));" % xs)
+    for xy, kj in zip(xs, r):
+        Kc = O.ke_dkt(np.asarray(xy, np.float32).astype(np.float64), E, NU, 0.1).ravel()
+        np.testing.assert_allclose(kj, Kc, rtol=0, atol=1e-5 * np.abs(Kc).max())
+
+
+# ----------------------------------------------------- assembly, BCs, PCG
+
+def assembled(op):
+    """Sparse K with the reference's BC rows (y[c] = x[c]) from the oracle's Ke."""
+    nd, npe = op.nd, op.npe
+    el = op.elems.reshape(-1, npe).astype(np.int64)
+    dofs = (el[:, :, None] * 3 + np.arange(3)).reshape(len(el), nd)
+    Ke = op.Ke.reshape(-1, nd, nd)
+    rows = np.repeat(dofs, nd, axis=1).ravel()
+    cols = np.tile(dofs, (1, nd)).ravel()
+    K = sp.coo_matrix((Ke.ravel(), (rows, cols)), shape=(3 * op.nn,) * 2).tocsr()
+    c = np.nonzero(op.cdof)[0]
+    K = K.tolil()
+    for i in c:
+        K.rows[i] = [i]
+        K.data[i] = [1.0]
+    return K.tocsr()
+
+
+@pytest.mark.parametrize("npe", [4, 3])
+def test_matrix_free_apply_matches_assembled(npe):
+    mesh = P.rect_mesh(6)
+    if npe == 3:
+        mesh = P.split_quads(mesh)
+    mask = P.constrained_mask(mesh, [{"type": "pinned", "location": "all_edges"}])
+    op = O.OracleProblem(mesh, E, NU, 0.05, mask)
+    K = assembled(op)
+    x = np.random.default_rng(0).standard_normal(3 * op.nn)
+    np.testing.assert_allclose(op.apply_k(x), K @ x, rtol=1e-12, atol=1e-9 * abs(K).max())
+
+
+def test_pcg_matches_sparse_direct():
+    mesh = P.rect_mesh(8)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask)
+    F = op.pressure_load(1e4)
+    x, st = op.pcg(F, tol=1e-13)
+    assert st["converged"]
+    xd = spla.spsolve(assembled(op).tocsc(), F)
+    np.testing.assert_allclose(x, xd, rtol=0, atol=1e-9 * np.abs(xd).max())
+
+
+def test_block_inverse_times_block_is_identity():
+    mesh = P.rect_mesh(6)
+    mask = P.edge_mask(6, 6, ["fixed", "pinned", "fixed", "pinned"])
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask)
+    K = assembled(op).toarray()
+    inv = op.blk.reshape(-1, 3, 3)
+    for n in range(op.nn):
+        B = K[3 * n:3 * n + 3, 3 * n:3 * n + 3].copy()
+        for d in range(3):                       # solver.ts:256-270 zeroes the column too
+            if op.cdof[3 * n + d]:
+                B[:, d] = 0
+                B[d, :] = 0
+                B[d, d] = 1
+        np.testing.assert_allclose(inv[n] @ B, np.eye(3), atol=1e-10)
+
+
+def test_zero_rhs_and_breakdown_rules():
+    mesh = P.rect_mesh(4)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask)
+    x, st = op.pcg(np.zeros(3 * op.nn))
+    assert st["converged"] and st["iterations"] == 0 and not x.any()
+    x, st = op.pcg(op.pressure_load(1e4), max_iter=1)
+    assert not st["converged"] and st["iterations"] == 1
+
+
+# ----------------------------------------------------- closed-form plates
+
+@pytest.mark.parametrize("support,coef", [("fixed", 0.00126), ("pinned", 0.00406)])
+def test_kirchhoff_uniform_load(support, coef):
+    """w_max of a thin square plate under uniform q: 0.00126 q L^4 / D clamped,
+    0.00406 q L^4 / D simply supported (Timoshenko & Woinowsky-Krieger)."""
+    n, t, q, L = 48, 0.05, 1e4, 10.0
+    mesh = P.rect_mesh(n)
+    mask = P.constrained_mask(mesh, [{"type": support, "location": "all_edges"}])
+    op = O.OracleProblem(mesh, E, NU, t, mask)
+    x, st = op.pcg(op.pressure_load(q), tol=1e-10)
+    assert st["converged"]
+    D = E * t ** 3 / (12 * (1 - NU ** 2))
+    wmax = np.abs(x[0::3]).max()
+    assert abs(wmax / (coef * q * L ** 4 / D) - 1) < 0.02
+    # the maximum sits at the centre node
+    assert np.argmax(np.abs(x[0::3])) == (n // 2) * (n + 1) + n // 2
+
+
+def test_pressure_load_totals():
+    mesh = P.rect_mesh(40)
+    op = O.OracleProblem(mesh, E, NU, 0.1, np.zeros(mesh["nodeCount"], np.uint8))
+    F = op.pressure_load(1e4)
+    assert abs(F[0::3].sum() / 1e6 - 1) < 1e-12 and not F[1::3].any() and not F[2::3].any()
+    A, s = 5e4, 1.0
+    F = op.pressure_load(0.0, np.array([[A, 5.0, 5.0, s]]))
+    assert abs(F[0::3].sum() / (2 * np.pi * s * s * A) - 1) < 1e-3
+    tri = P.split_quads(mesh)
+    opt = O.OracleProblem(tri, E, NU, 0.1, np.zeros(mesh["nodeCount"], np.uint8))
+    assert abs(opt.pressure_load(1e4)[0::3].sum() / 1e6 - 1) < 1e-12
+
+
+# ----------------------------------------------------- mesh and colouring
+
+def test_mesher_matches_js_restatement():
+    for n in (32, 142):
+        r = O.run_js("const m = O.generateRectangularMesh(10, 10, 10 / %d); console.log("
+                     "JSON.stringify({nodes: Array.from(m.nodes), el: Array.from(m.elements),"
+                     " b: Array.from(m.boundaryNodes), nx: m.gridNx}));" % n)
+        m = P.rect_mesh(n)
+        assert r["nx"] == n
+        assert np.array_equal(np.array(r["nodes"], np.float32), m["nodes"])
+        assert np.array_equal(np.array(r["el"], np.uint32), m["elements"])
+        assert np.array_equal(np.array(r["b"], np.uint32), m["boundaryNodes"])
+
+
+def colour_valid(mesh, col):
+    npe = mesh["nodesPerElement"]
+    el = mesh["elements"].reshape(-1, npe)
+    off = col["colour_offsets"]
+    for c in range(col["colour_count"]):
+        nodes = el[col["elements_by_colour"][off[c]:off[c + 1]]].ravel()
+        if len(np.unique(nodes)) != len(nodes):
+            return False
+    return sorted(col["elements_by_colour"].tolist()) == list(range(len(el)))
+
+
+def test_colourings_valid_and_match_js():
+    q = P.rect_mesh(10)
+    t = P.split_quads(q)
+    cq, cg = P.colour_structured(q), P.colour_greedy(t)
+    assert colour_valid(q, cq) and colour_valid(t, cg)
+    assert cq["colour_count"] == 4 and cg["colour_count"] == 7
+    r = O.run_js("const q = O.rectangularMeshN(10, 10, 10, 10); const t = O.splitQuadMesh(q);"
+                 "console.log(JSON.stringify({s: Array.from(O.computeElementColoring(q).elementColors),"
+                 " g: Array.from(O.computeElementColoringGreedy(t).elementColors)}));")
+    assert np.array_equal(r["s"], cq["elementColors"])
+    assert np.array_equal(r["g"], cg["elementColors"])
+
+
+# ----------------------------------------------------- the two oracles agree
+
+@pytest.mark.parametrize("tri", [False, True])
+def test_js_float32_solver_matches_fp64_oracle(tri):
+    """The float32 restatement of solver.ts and the FP64 oracle solve the same
+    problem; the C1 plate (and its triangle split)."""
+    n = 32 if not tri else 16
+    mesh = P.rect_mesh(n)
+    if tri:
+        mesh = P.split_quads(mesh)
+    sup = [{"type": "fixed", "location": "all_edges"}]
+    mask = P.constrained_mask(mesh, sup)
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask)
+    F = op.pressure_load(1e4).astype(np.float32)
+    x64, st = op.pcg(F.astype(np.float64), tol=1e-8)
+    assert st["converged"]
+    x32, js = O.run_js_solver(n, F, E, NU, 0.1, sup, triangles=tri)
+    assert js["converged"]
+    assert O.compare_max_w(x32, x64) < 1e-4
+    assert O.max_nodal_w_error(x32, x64) < 1e-3
+    if not tri:
+        assert st["iterations"] == pytest.approx(334, abs=40)
