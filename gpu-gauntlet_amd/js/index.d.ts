@@ -1,0 +1,80 @@
+// TypeScript declarations of the Node host (index.js).  The data types are
+// the reference's (src/lib/plate/types.ts:14-102) so a caller of
+// solveGPU (src/lib/plate/gpu/solver.ts:80-87) compiles unchanged.
+
+export interface PlateMaterial {
+  E: number;
+  nu: number;
+  t: number;
+  /** Extension: per-element thickness (no reference counterpart). */
+  elementT?: Float32Array | null;
+}
+
+export interface PlateMesh {
+  nodes: Float32Array;
+  elements: Uint32Array;
+  boundaryNodes: Uint32Array;
+  nodeCount: number;
+  elementCount: number;
+  nodesPerElement?: number;
+  gridNx: number;
+  gridNy: number;
+}
+
+export interface ElementColoring {
+  colors: Uint32Array[];
+  elementColors: Uint8Array;
+  colorCount: number;
+}
+
+/** GPUSolverOptions (gpu/solver.ts:35-42) plus this library's choices. */
+export interface GPUSolverOptions {
+  tolerance?: number;
+  maxIterations?: number;
+  /** Rejected: this library has no CPU solver. */
+  forceCPU?: boolean;
+  precision?: 'fp64' | 'mixed';
+  preconditioner?: 'blockJacobi' | 'learned';
+  path?: 'auto' | 'structured' | 'coloured';
+  innerTolerance?: number;
+  maxRefinements?: number;
+}
+
+/** GPUSolveResult (gpu/solver.ts:47-60) plus the FP64 fields. */
+export interface GPUSolveResult {
+  solution: Float32Array;
+  iterations: number;
+  finalResidual: number;
+  converged: boolean;
+  gpuTimeMs: number;
+  usedGPU: boolean;
+  solution64: Float64Array;
+  trueRelResidual: number;
+  deviceMs: number;
+  breakdown: boolean;
+  sweeps: number;
+}
+
+export declare function solveGPU(
+  mesh: PlateMesh,
+  material: PlateMaterial,
+  coloring: ElementColoring,
+  F: Float32Array,
+  constrainedDOFs: Set<number>,
+  options?: GPUSolverOptions
+): Promise<GPUSolveResult>;
+
+export declare function isGPUAvailable(): boolean;
+export declare function destroyContext(): void;
+
+export interface GaussianPatch { A: number; cx: number; cy: number; sigma: number; }
+
+export declare class PlateSolver {
+  constructor(mesh: PlateMesh, material: PlateMaterial, coloring: ElementColoring | null,
+              constrainedDOFs: Set<number>);
+  readonly structured: boolean;
+  solve(F: Float32Array, options?: GPUSolverOptions): Promise<Omit<GPUSolveResult, 'gpuTimeMs' | 'usedGPU'>>;
+  applyK(x: Float64Array, path?: 'auto' | 'structured' | 'coloured'): Float64Array;
+  pressureLoad(q0: number, patches?: GaussianPatch[]): Float32Array;
+  destroy(): void;
+}
