@@ -39,6 +39,11 @@ __global__ void k_f64_to_f32(int64_t n, const double* a, float* b) {
   if (i < n) b[i] = (float)a[i];
 }
 
+__global__ void k_residual(int64_t n, const double* F, const double* Kx, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = F[i] - Kx[i];
+}
+
 int nblocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 bool detect_structured(const pp_mesh* m) {
@@ -82,6 +87,13 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
+}
+
+int launch_residual(pp_plan* pl, const double* dF, const double* dKx, double* out) {
+  hipLaunchKernelGGL(k_residual, dim3(nblocks(pl->n_dof, 256)), dim3(256), 0, pl->ctx->stream,
+                     (int64_t)pl->n_dof, dF, dKx, out);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
 }
 
 int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel) {
@@ -141,14 +153,20 @@ int pp_ctx_create(int device, pp_ctx** out) {
   return PP_OK;
 }
 
-int pp_ctx_destroy(pp_ctx* c) {
-  if (!c) return PP_OK;
+// A context is freed when its owner and every plan made from it have let go.
+static void ctx_release(pp_ctx* c) {
+  if (--c->refs > 0) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int pp_ctx_destroy(pp_ctx* c) {
+  if (!c) return PP_OK;
+  ctx_release(c);
   return PP_OK;
 }
 
@@ -238,15 +256,18 @@ int pp_plan_create(pp_ctx* ctx, const pp_mesh* m, const pp_material* mat,
     // the structured path owns the block inverses when it exists
     if ((rc = struct_setup(pl))) { delete pl; return rc; }
   }
+  ctx->refs++;
   *out = pl;
   return PP_OK;
 }
 
 int pp_plan_destroy(pp_plan* pl) {
   if (!pl) return PP_OK;
-  (void)hipSetDevice(pl->ctx->device);
-  (void)hipStreamSynchronize(pl->ctx->stream);
+  pp_ctx* c = pl->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   delete pl;
+  ctx_release(c);
   return PP_OK;
 }
 
@@ -299,7 +320,9 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     return fail(PP_ERR_INVALID, "unknown precision");
   }
   if (rc) return rc;
-  if (o->flags & PP_FLAG_NO_TRUE_RESIDUAL) return PP_OK;
+  // the structured FP64 solver measures the true residual itself
+  if ((o->flags & PP_FLAG_NO_TRUE_RESIDUAL) || (structured && o->precision == PP_PRECISION_FP64))
+    return PP_OK;
   double rel = 0;
   rc = structured ? struct_true_residual(pl, pl->x.p, &rel) : general_true_residual(pl, pl->x.p, &rel);
   if (rc) return rc;

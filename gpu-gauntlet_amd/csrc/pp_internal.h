@@ -77,6 +77,7 @@ struct pp_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int cu_count = 0;
+  int refs = 1;  // the owner plus one per live plan
 };
 
 struct pp_plan {
@@ -142,5 +143,7 @@ int general_pressure(pp_plan* pl, double q0, const double* patches, int np, floa
 
 // shared helpers (pp_api.hip)
 int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel);
+// out = F - Kx (out may alias Kx)
+int launch_residual(pp_plan* pl, const double* dF, const double* dKx, double* out);
 
 }  // namespace pp

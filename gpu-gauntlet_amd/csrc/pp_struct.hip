@@ -239,6 +239,8 @@ __global__ void k_pressure(KGrid G, double q0, const double* pt, int np, float* 
 struct PcgArgs {
   KGrid G;
   const double* F;
+  const double* r0;  // restart: initial residual F - K x (x read from x); nullptr = cold start
+  double bnorm_in;   // restart: |F| for the relative stopping test
   double *x, *r, *z, *pb0, *pb1, *Ap;
   const double* minv;
   double* part;
@@ -265,26 +267,35 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
   int64_t my_n = -1;
   unsigned my_fix = 0;
 
-  // ---------------- init: r = F, z = M^-1 r, r.z, F.F
+  // ---------------- init: r = F (or r0 on a restart), z = M^-1 r, r.z, r.r
+  const bool restart = A.r0 != nullptr;
+  const double* src = restart ? A.r0 : A.F;
   double acc[2] = {0, 0};
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
     const int gi = i0 + tx, gj = j0 + ty;
     if (node_thread && gi <= G.nx && gj <= G.ny) {
       const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
-      double rv[3] = {A.F[3 * n], A.F[3 * n + 1], A.F[3 * n + 2]}, zv[3], m[6];
+      double rv[3] = {src[3 * n], src[3 * n + 1], src[3 * n + 2]}, zv[3], m[6];
       for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
       sym3_apply(m, rv, zv);
       for (int d = 0; d < 3; d++) {
         A.z[3 * n + d] = zv[d];
-        if (!RES) { A.r[3 * n + d] = rv[d]; A.x[3 * n + d] = 0.0; }
+        if (!RES) {
+          A.r[3 * n + d] = rv[d];
+          if (!restart) A.x[3 * n + d] = 0.0;
+        }
         acc[0] += rv[d] * zv[d];
         acc[1] += rv[d] * rv[d];
       }
       if (RES) {
         my_n = n;
         my_fix = G.cmask[n];
-        for (int d = 0; d < 3; d++) { rr_[d] = rv[d]; zr[d] = zv[d]; }
+        for (int d = 0; d < 3; d++) {
+          rr_[d] = rv[d];
+          zr[d] = zv[d];
+          xr[d] = restart ? A.x[3 * n + d] : 0.0;
+        }
         for (int k = 0; k < 6; k++) mi[k] = m[k];
       }
     }
@@ -292,13 +303,14 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
   block_sum<2, NT>(acc, red + 8);
   if (!grid_allreduce<2>(acc, A.part, A.gs, epoch, red)) return;
   double rz = acc[0], rr = acc[1];
-  const double bnorm = sqrt(rr);
-  if (bnorm < 1e-30) {
+  const double bnorm = restart ? A.bnorm_in : sqrt(rr);
+  const bool done0 = restart ? (sqrt(rr) < A.tol || sqrt(rr) / bnorm < A.tol) : bnorm < 1e-30;
+  if (done0) {
     if (RES && my_n >= 0)
-      for (int d = 0; d < 3; d++) A.x[3 * my_n + d] = 0.0;
+      for (int d = 0; d < 3; d++) A.x[3 * my_n + d] = xr[d];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       A.st->iterations = 0; A.st->converged = 1; A.st->breakdown = 0;
-      A.st->final_residual = 0.0; A.st->bnorm = bnorm;
+      A.st->final_residual = restart ? sqrt(rr) : 0.0; A.st->bnorm = bnorm;
     }
     return;
   }
@@ -483,55 +495,89 @@ int struct_true_residual(pp_plan* pl, const double* dx, double* rel) {
   return launch_norm_ratio(pl, pl->F.p, pl->tmp.p, rel);
 }
 
-int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+// One cooperative launch of the persistent PCG (cold start when r0 is null).
+static int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxit,
+                      DevStats* ds) {
   hipStream_t s = pl->ctx->stream;
   const int ntiles = ((pl->nx + 1 + TX - 1) / TX) * ((pl->ny + 1 + TY - 1) / TY);
   int per_cu = 0;
   PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<true>, NT, 0));
-  const int capacity = per_cu * pl->ctx->cu_count;
-  const bool res = ntiles <= capacity;
+  const bool res = ntiles <= per_cu * pl->ctx->cu_count;
   int per_cu_s = 0;
   PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_pcg<false>, NT, 0));
   const int nwg = res ? ntiles : std::min(ntiles, per_cu_s * pl->ctx->cu_count);
   if (nwg <= 0) return fail(PP_ERR_HIP, "persistent PCG kernel cannot be made resident");
   int rc;
   if (pl->partials.n < (size_t)4 * nwg && (rc = pl->partials.alloc(4 * (size_t)nwg))) return rc;
-
   PcgArgs A;
   A.G = make_kgrid(pl);
   A.F = pl->F.p;
+  A.r0 = r0;
+  A.bnorm_in = bnorm;
   A.x = pl->x.p; A.r = pl->r.p; A.z = pl->z.p; A.pb0 = pl->p0.p; A.pb1 = pl->p1.p;
   A.Ap = pl->Ap.p;
   A.minv = pl->minv.p;
   A.part = pl->partials.p;
   A.gs = pl->gsync.p;
   A.st = pl->dstats.p;
-  A.tol = o->tolerance;
-  A.maxit = o->max_iterations;
+  A.tol = tol;
+  A.maxit = maxit;
   void* args[] = {&A};
   hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, s, pl->gsync.p);
   PP_HIP(hipGetLastError());
-  PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   if (res)
     PP_HIP(hipLaunchCooperativeKernel((const void*)k_pcg<true>, dim3(nwg), dim3(NT), args, 0, s));
   else
     PP_HIP(hipLaunchCooperativeKernel((const void*)k_pcg<false>, dim3(nwg), dim3(NT), args, 0, s));
-  PP_HIP(hipEventRecord(pl->ctx->ev1, s));
-  DevStats ds;
   GridSync gsh;
-  PP_HIP(hipMemcpyAsync(&ds, pl->dstats.p, sizeof(ds), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
   PP_HIP(hipMemcpyAsync(&gsh, pl->gsync.p, sizeof(gsh), hipMemcpyDeviceToHost, s));
   PP_HIP(hipStreamSynchronize(s));
+  if (gsh.abort) return fail(PP_ERR_TIMEOUT, "grid barrier spin limit reached");
+  return PP_OK;
+}
+
+// FP64 PCG with residual replacement: after the recurrence converges the
+// true residual F - K x is recomputed in FP64; while it is above the
+// tolerance (the recurrence drifts from it by rounding) PCG restarts from x
+// with that residual, at most kMaxRestarts times.  All of it is timed.
+int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  constexpr int kMaxRestarts = 4;
+  hipStream_t s = pl->ctx->stream;
+  PP_HIP(hipEventRecord(pl->ctx->ev0, s));
+  DevStats ds;
+  int rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, &ds);
+  if (rc) return rc;
+  const double bnorm = ds.bnorm;
+  int total = ds.iterations;
+  st->sweeps = 1;
+  st->sweep_iterations[0] = ds.iterations;
+  double rel = 0;
+  if (ds.converged && bnorm > 0) {
+    for (int k = 0;; k++) {
+      if ((rc = struct_apply_k(pl, pl->x.p, pl->tmp.p))) return rc;
+      if ((rc = launch_norm_ratio(pl, pl->F.p, pl->tmp.p, &rel))) return rc;
+      st->sweep_residual[st->sweeps - 1] = rel;
+      if (rel <= o->tolerance || k == kMaxRestarts || total >= o->max_iterations) break;
+      if ((rc = launch_residual(pl, pl->F.p, pl->tmp.p, pl->tmp.p))) return rc;
+      rc = launch_pcg(pl, pl->tmp.p, bnorm, o->tolerance, o->max_iterations - total, &ds);
+      if (rc) return rc;
+      total += ds.iterations;
+      if (st->sweeps < PP_MAX_SWEEPS) st->sweep_iterations[st->sweeps] = ds.iterations;
+      st->sweeps++;
+      if (!ds.converged) break;
+    }
+  }
+  PP_HIP(hipEventRecord(pl->ctx->ev1, s));
+  PP_HIP(hipEventSynchronize(pl->ctx->ev1));
   float ms = 0;
   PP_HIP(hipEventElapsedTime(&ms, pl->ctx->ev0, pl->ctx->ev1));
-  if (gsh.abort) return fail(PP_ERR_TIMEOUT, "grid barrier spin limit reached");
-  st->iterations = ds.iterations;
+  st->iterations = total;
   st->converged = ds.converged;
   st->breakdown = ds.breakdown;
   st->final_residual = ds.final_residual;
+  st->true_rel_residual = rel;
   st->device_ms = ms;
-  st->sweeps = 1;
-  st->sweep_iterations[0] = ds.iterations;
   return PP_OK;
 }
 
