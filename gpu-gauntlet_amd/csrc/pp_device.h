@@ -49,60 +49,111 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
-// Grid-wide all-reduce + barrier.  Thread 0 of every workgroup holds its
-// block's NV (<= 2) partial sums in v.  Each workgroup publishes them into
-// slot (epoch & 1) of `part` (2 * gridDim.x doubles per slot), arrives on the
-// counter and waits until every workgroup has arrived.  Wave 0 then sums the
+// Cross-workgroup exchange inside a persistent kernel.  Every value that
+// another workgroup reads while the kernel runs is written with st_coh and
+// read with ld_coh: relaxed atomic accesses at device scope (sc1), served by
+// the device-coherent level, so no L2 write-back or invalidate fence is ever
+// issued (those fences are most of the cost of a release/acquire barrier).
+// Data a workgroup only reads back itself keeps plain loads and stores.
+// Measured on MI355X: plain stores with L1-bypassing workgroup-scope loads,
+// or a workgroup-scope arrival counter, lose updates between CUs even when
+// all workgroups sit on one XCD; device-scope relaxed accesses do not.
+__device__ __forceinline__ double ld_coh(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_coh(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until this wave's outstanding stores have completed; every thread
+// calls it before the workgroup barrier that precedes an arrival.
+__device__ __forceinline__ void publish() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Grid-wide all-reduce + barrier of a persistent kernel (all nwg workgroups
+// co-resident).  Thread 0 of every workgroup holds its block's NV (<= 2)
+// partial sums in v; it publishes them into slot (epoch & 1) of `part`
+// (2 * nwg doubles per slot), waits for its stores, arrives on the counter
+// and spins until all nwg workgroups have arrived.  Wave 0 then sums the
 // partials in a fixed order, so every workgroup gets bit-identical totals and
-// takes identical control decisions.  Two slots suffice: a workgroup cannot
-// publish epoch e+2 before all have passed epoch e+1, i.e. finished reading e.
-// Returns false if the barrier was aborted (spin budget spent somewhere).
+// takes identical control decisions.  Two slots suffice: no workgroup can
+// publish epoch e+2 before all have passed e+1, i.e. finished reading e.
+// Returns false when the barrier gave up (spin budget spent here or the
+// abort flag raised elsewhere); the kernel then exits.
+//
+// No arrival counter: the partial itself is the arrival.  Each partial is
+// stored as one 64-bit word whose 8 low mantissa bits carry the epoch tag
+// (the value keeps 44 fraction bits, a relative change below 6e-14), so a
+// single coherent store publishes both, and readers poll the slots until
+// every tag is current.  Atomic-free: device-scope atomics on one address
+// serialise across workgroups.  The slot memory must be zeroed before the
+// kernel starts (tag 0 never matches; the first epoch uses tag 1).
+__device__ __forceinline__ unsigned long long tag_word(double v, unsigned tag) {
+  return (__double_as_longlong(v) & ~0xffull) | (unsigned long long)(tag & 0xffu);
+}
+__device__ __forceinline__ double untag_word(unsigned long long w) {
+  return __longlong_as_double((long long)(w & ~0xffull));
+}
+
 template <int NV>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* part, GridSync* gs,
-                                               unsigned& epoch, double* lds) {
+                                               unsigned& epoch, double* lds, unsigned nwg,
+                                               unsigned me) {
   static_assert(NV >= 1 && NV <= 2, "two values per slot");
-  const unsigned nwg = gridDim.x;
-  double* slot = part + (size_t)(epoch & 1u) * 2 * nwg;
-  if (threadIdx.x == 0) {
+  unsigned long long* slot = (unsigned long long*)part + (size_t)(epoch & 1u) * 2 * nwg;
+  const unsigned tag = (epoch + 1u) & 0xffu ? (epoch + 1u) & 0xffu : 0x80u;
+  if (threadIdx.x < 64) {
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) slot[k * nwg + blockIdx.x] = v[k];
-    __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned target = (epoch + 1u) * nwg;
+      for (int k = 0; k < NV; k++)
+        __hip_atomic_store(&slot[k * nwg + me], tag_word(v[k], tag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
     unsigned spins = 0;
     int ok = 1;
-    while (__hip_atomic_load(&gs->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-          ++spins > kSpinLimit) {
-        __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double s[NV];
+    for (;;) {
+      int cur = 1;
+#pragma unroll
+      for (int k = 0; k < NV; k++) s[k] = 0;
+      for (unsigned i = threadIdx.x; i < nwg; i += 64)
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+          const unsigned long long w =
+              __hip_atomic_load(&slot[k * nwg + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          cur &= (unsigned)(w & 0xffu) == tag;
+          s[k] += untag_word(w);
+        }
+      if (__all(cur)) break;
+      if (++spins > kSpinLimit ||
+          __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        if (threadIdx.x == 0)
+          __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    ((int*)lds)[2 * NV] = ok;
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k] = wave_sum(s[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; k++) lds[k] = s[k];
+      ((int*)lds)[2 * NV] = ok;
+    }
   }
   epoch++;
   __syncthreads();
   const bool ok = ((int*)lds)[2 * NV] != 0;
-  if (ok && threadIdx.x < 64) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0;
-      for (unsigned i = threadIdx.x; i < nwg; i += 64) s += slot[k * nwg + i];
-      s = wave_sum(s);
-      if (threadIdx.x == 0) lds[k] = s;
-    }
-  }
-  __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = lds[k];
   __syncthreads();
-  if (!ok) return false;
-  // The data written before the barrier by other workgroups is read by all
-  // threads after it: make this thread's later loads observe it.
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return true;
+  return ok;
 }
 
 // y = Ke u for the Q4 Mindlin element of element.ts:258-483 on an
@@ -113,7 +164,7 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* part, Gr
 // {d thy/dx, -d thx/dy, d thy/dy - d thx/dx}, shear strains
 // {dw/dx + thy, dw/dy - thx}; corner order SW, SE, NE, NW.
 template <typename T>
-__device__ __forceinline__ void q4_rect_apply(T a, T b, T ia, T ib, T D11, T D12, T D33, T Ds,
+__host__ __device__ __forceinline__ void q4_rect_apply(T a, T b, T ia, T ib, T D11, T D12, T D33, T Ds,
                                               const T (&u)[12], T (&y)[12]) {
   const T g = (T)0.5773502691896257;
   const T fb = (T)0.25 * a * b;
@@ -159,6 +210,54 @@ __device__ __forceinline__ void q4_rect_apply(T a, T b, T ia, T ib, T D11, T D12
       y[3 * i + 1] += (T)(-0.25) * qy;
       y[3 * i + 2] += (T)0.25 * qx;
     }
+  }
+}
+
+// The same y = Ke u as q4_rect_apply with the Gauss sums done in closed
+// form.  On a rectangle dN_i/dx = xi_i (1 + eta_i eta) / (2a) and
+// dN_i/dy = eta_i (1 + xi_i xi) / (2b) (corner signs xi_i, eta_i = -+1), so
+// d/dx of a rotation field depends on eta only and d/dy on xi only.  The
+// curvatures at the 2x2 points therefore reduce to four edge differences per
+// rotation, and the back-projection B^T m needs only the moment sums
+// sum_q m_q, sum_q xi_q m_q, sum_q eta_q m_q (g^2 = 1/3).  About 100 flops
+// instead of ~360, and far fewer live registers.
+template <typename T>
+__host__ __device__ __forceinline__ void q4_rect_apply_fast(T a, T b, T ia, T ib, T D11, T D12,
+                                                           T D33, T Ds, const T (&u)[12],
+                                                           T (&y)[12]) {
+  const T g = (T)0.5773502691896257;
+  const T ga = (T)0.5 * ia, gb = (T)0.5 * ib;  // 1/(2a), 1/(2b)
+  const T F = a * b;                           // 4 points x |J| (= ab/4)
+  // edge differences of thx (index 1) and thy (index 2); corners SW SE NE NW
+  const T Sx = u[4] - u[1], Nx = u[7] - u[10], Wx = u[10] - u[1], Ex = u[7] - u[4];
+  const T Sy = u[5] - u[2], Ny = u[8] - u[11], Wy = u[11] - u[2], Ey = u[8] - u[5];
+  // kx(eta) = ga[(Sy+Ny) + eta(Ny-Sy)], ky(xi) = -gb[(Wx+Ex) + xi(Ex-Wx)],
+  // kxy = T(xi) - S(eta), T = gb[(Wy+Ey) + xi(Ey-Wy)], S = ga[(Sx+Nx) + eta(Nx-Sx)].
+  // Over the 4 points: sum kx = 4 KXp, sum eta kx = 4 g KXm, sum xi kx = 0, etc.
+  const T KXp = ga * (Sy + Ny), KXm = g * ga * (Ny - Sy);
+  const T KYp = -gb * (Wx + Ex), KYm = -g * gb * (Ex - Wx);
+  const T Tp = gb * (Wy + Ey), Tm = g * gb * (Ey - Wy);
+  const T Sp = ga * (Sx + Nx), Sm = g * ga * (Nx - Sx);
+  // moment sums over the points (times |J|): m = D kappa
+  const T SA = F * (D11 * KXp + D12 * KYp);  // sum mx
+  const T EA = g * F * D11 * KXm;            // sum eta mx
+  const T SB = F * (D12 * KXp + D11 * KYp);  // sum my
+  const T XB = g * F * D11 * KYm;            // sum xi my
+  const T SC = F * D33 * (Tp - Sp);          // sum mxy
+  const T XC = g * F * D33 * Tm;             // sum xi mxy
+  const T EC = -g * F * D33 * Sm;            // sum eta mxy
+  // shear at the centre (weight 4, |J| = ab/4)
+  const T gxz = ga * (u[3] + u[6] - u[0] - u[9]) + (T)0.25 * (u[2] + u[5] + u[8] + u[11]);
+  const T gyz = gb * (u[6] + u[9] - u[0] - u[3]) - (T)0.25 * (u[1] + u[4] + u[7] + u[10]);
+  const T fs = a * b * Ds;
+  const T qx = fs * gxz, qy = fs * gyz;
+  const T xs[4] = {-1, 1, 1, -1}, es[4] = {-1, -1, 1, 1};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const T xi = xs[i], et = es[i];
+    y[3 * i] = ga * xi * qx + gb * et * qy;
+    y[3 * i + 1] = -gb * et * (SB + xi * XB) - ga * xi * (SC + et * EC) - (T)0.25 * qy;
+    y[3 * i + 2] = ga * xi * (SA + et * EA) + gb * et * (SC + xi * XC) + (T)0.25 * qx;
   }
 }
 

@@ -68,6 +68,10 @@ struct DevStats {
   int pad;
   double final_residual;
   double bnorm;
+  // wall-clock ticks (100 MHz) spent by workgroup 0 in each part of an
+  // iteration, summed over iterations: stage, elements, node sums,
+  // all-reduce A, phase B, all-reduce B
+  unsigned long long ticks[6];
 };
 
 }  // namespace pp

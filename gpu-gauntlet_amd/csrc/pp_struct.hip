@@ -1,133 +1,72 @@
-// Structured-grid (tensor mesh) kernels: element operator, block-Jacobi
-// set-up and apply, pressure loads, and the persistent FP64 PCG.
+// Structured-grid (tensor mesh) kernels: K.x, block-Jacobi set-up and
+// apply, pressure loads, and the persistent FP64 PCG.
 //
 // Layout in HBM: every DOF vector is node-major [w, thx, thy] doubles (the
 // reference's numbering, so nothing is permuted at the boundary); the
 // preconditioner is 6 doubles per node (upper triangle of the symmetric 3x3
-// inverse); the geometry is two small arrays of column widths / row heights
-// (a, 1/a) and (b, 1/b) because every element of a tensor grid built from
-// float32 coordinates is an exact axis-aligned rectangle.
+// inverse); the geometry is two small arrays of column widths / row heights.
 //
-// K.p is computed per tile of TX x TY nodes owned by one workgroup: the tile
-// plus a one-node halo of p is staged in LDS, every element touching the tile
-// ((TX+1) x (TY+1)) is applied matrix-free (B^T D B p) into LDS, and each node
-// then sums its four elements' contributions in a fixed order.  No atomics,
-// no colour passes, no races: each node is written by its owning thread only.
+// One persistent PCG kernel body, three launch shapes:
+//  - up to 64 tiles of 31x31 nodes (e.g. the 61k-DOF plate): one
+//    1024-thread workgroup per tile, per-node state in registers;
+//  - 15x15-node tiles, one 256-thread workgroup each, state in registers;
+//  - 15x15-node tiles strided over a resident grid, state in HBM.
 
 #include <math.h>
 
-#include "pp_device.h"
+#include <algorithm>
+
+#include "pp_grid.h"
 
 namespace pp {
 
+using CfgS = Tile<15, 15, 256>;   // device-wide kernel and plain K.x
+using CfgX = Tile<31, 31, 1024>;  // large-tile persistent kernel
+
+KGrid make_kgrid(const pp_plan* pl) {
+  KGrid G;
+  G.nx = pl->nx;
+  G.ny = pl->ny;
+  G.xc = pl->xc.p;
+  G.yc = pl->yc.p;
+  G.ax = pl->xc.p + (pl->nx + 1);  // widths / heights are stored after the coordinates
+  G.by = pl->yc.p + (pl->ny + 1);
+  G.elem_t = pl->var_t ? pl->elem_t.p : nullptr;
+  G.t0 = pl->t;
+  G.Dfac = pl->E / (12.0 * (1.0 - pl->nu * pl->nu));
+  G.Gk = (5.0 / 6.0) * pl->E / (2.0 * (1.0 + pl->nu));
+  G.nu = pl->nu;
+  G.cmask = pl->cmask.p;
+  return G;
+}
+
 namespace {
 
-constexpr int TX = 15, TY = 15, NT = 256;  // tile of 15x15 nodes, 16x16 elements
-constexpr int PW = TX + 2, PH = TY + 2, NP = PW * PH;
-constexpr int EW = TX + 1, EH = TY + 1, NE = EW * EH;
-static_assert(NE <= NT, "one element per thread");
-
-struct KGrid {
-  int nx, ny;
-  const double* ax;    // 2 per column: a, 1/a
-  const double* by;    // 2 per row: b, 1/b
-  const double* xc;    // node x coordinates (nx+1)
-  const double* yc;    // node y coordinates (ny+1)
-  const float* elem_t; // nullptr = uniform t0
-  double t0, Dfac, Gk, nu;
-  const uint8_t* cmask;
-};
-
-__device__ __forceinline__ int tiles_x(const KGrid& G) { return (G.nx + 1 + TX - 1) / TX; }
-__device__ __forceinline__ int tiles_total(const KGrid& G) {
-  return tiles_x(G) * ((G.ny + 1 + TY - 1) / TY);
-}
-
-// Stage p = z + beta * pold (or p = src when pold == nullptr) for the tile
-// plus halo into LDS (SoA: p_s[c * NP + idx]); outside the grid p = 0.
-template <typename T, typename S>
-__device__ __forceinline__ void stage_tile(const KGrid& G, int i0, int j0, const S* src,
-                                           const S* pold, T beta, T* p_s) {
-  for (int idx = threadIdx.x; idx < NP; idx += NT) {
-    const int lx = idx % PW, ly = idx / PW;
-    const int gi = i0 - 1 + lx, gj = j0 - 1 + ly;
-    T v0 = 0, v1 = 0, v2 = 0;
-    if (gi >= 0 && gi <= G.nx && gj >= 0 && gj <= G.ny) {
-      const size_t n = (size_t)gj * (G.nx + 1) + gi;
-      v0 = (T)src[3 * n];
-      v1 = (T)src[3 * n + 1];
-      v2 = (T)src[3 * n + 2];
-      if (pold) {
-        v0 += beta * (T)pold[3 * n];
-        v1 += beta * (T)pold[3 * n + 1];
-        v2 += beta * (T)pold[3 * n + 2];
-      }
-    }
-    p_s[idx] = v0;
-    p_s[NP + idx] = v1;
-    p_s[2 * NP + idx] = v2;
-  }
-}
-
-// Apply every element touching the tile; results to ye_s[k * NE + le].
-template <typename T>
-__device__ __forceinline__ void tile_elements(const KGrid& G, int i0, int j0, const T* p_s,
-                                              T* ye_s) {
-  const int le = threadIdx.x;
-  if (le >= NE) return;
-  const int ex = le % EW, ey = le / EW;
-  const int gi = i0 - 1 + ex, gj = j0 - 1 + ey;
-  T y[12];
-  if (gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny) {
-    const int c0 = ey * PW + ex, c1 = c0 + 1, c2 = c0 + PW + 1, c3 = c0 + PW;
-    const T u[12] = {p_s[c0], p_s[NP + c0], p_s[2 * NP + c0], p_s[c1], p_s[NP + c1],
-                     p_s[2 * NP + c1], p_s[c2], p_s[NP + c2], p_s[2 * NP + c2],
-                     p_s[c3], p_s[NP + c3], p_s[2 * NP + c3]};
-    const double t = G.elem_t ? (double)G.elem_t[(size_t)gj * G.nx + gi] : G.t0;
-    const double D = G.Dfac * t * t * t;
-    const T D11 = (T)D, D12 = (T)(D * G.nu), D33 = (T)(D * (1.0 - G.nu) / 2.0);
-    const T Ds = (T)(G.Gk * t);
-    q4_rect_apply<T>((T)G.ax[2 * gi], (T)G.by[2 * gj], (T)G.ax[2 * gi + 1], (T)G.by[2 * gj + 1],
-                     D11, D12, D33, Ds, u, y);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 12; k++) y[k] = 0;
-  }
-#pragma unroll
-  for (int k = 0; k < 12; k++) ye_s[k * NE + le] = y[k];
-}
-
-// Sum of the four elements around node (tx, ty) of the tile, fixed order.
-template <typename T>
-__device__ __forceinline__ void node_sum(int tx, int ty, const T* ye_s, T (&y)[3]) {
-  const int sw = ty * EW + tx, se = sw + 1, ne = sw + EW + 1, nw = sw + EW;
-#pragma unroll
-  for (int d = 0; d < 3; d++)
-    y[d] = ye_s[(6 + d) * NE + sw] + ye_s[(9 + d) * NE + se] + ye_s[d * NE + ne] +
-           ye_s[(3 + d) * NE + nw];
-}
+int grid_blocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------ K . x
 
-__global__ __launch_bounds__(NT) void k_apply(KGrid G, const double* x, double* y) {
-  __shared__ double p_s[3 * NP];
-  __shared__ double ye_s[12 * NE];
-  const int tix = blockIdx.x % tiles_x(G), tiy = blockIdx.x / tiles_x(G);
-  const int i0 = tix * TX, j0 = tiy * TY;
-  stage_tile<double, double>(G, i0, j0, x, nullptr, 0.0, p_s);
+// y = K x with the reference's BC rows (solver.ts:479-482: y[c] = x[c]).
+__global__ __launch_bounds__(CfgS::NT) void k_apply(KGrid G, const double* x, double* y) {
+  using C = CfgS;
+  __shared__ double p_s[3 * C::NP];
+  __shared__ double ye_s[12 * C::NE];
+  const int tix = blockIdx.x % tiles_x<C>(G), tiy = blockIdx.x / tiles_x<C>(G);
+  const int i0 = tix * C::TX, j0 = tiy * C::TY;
+  stage_tile<C, double, double>(G, i0, j0, x, nullptr, 0.0, p_s);
   __syncthreads();
-  tile_elements<double>(G, i0, j0, p_s, ye_s);
+  tile_elements<C, double>(G, i0, j0, p_s, ye_s);
   __syncthreads();
-  if (threadIdx.x < TX * TY) {
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  if (threadIdx.x < C::TX * C::TY) {
+    const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
     const int gi = i0 + tx, gj = j0 + ty;
     if (gi <= G.nx && gj <= G.ny) {
       double v[3];
-      node_sum<double>(tx, ty, ye_s, v);
+      node_sum<C, double>(tx, ty, ye_s, v);
       const size_t n = (size_t)gj * (G.nx + 1) + gi;
       const unsigned fix = G.cmask[n];
-      const int c = (ty + 1) * PW + tx + 1;
-      for (int d = 0; d < 3; d++) y[3 * n + d] = (fix >> d) & 1u ? p_s[d * NP + c] : v[d];
+      const int c = (ty + 1) * C::PW + tx + 1;
+      for (int d = 0; d < 3; d++) y[3 * n + d] = (fix >> d) & 1u ? p_s[d * C::NP + c] : v[d];
     }
   }
 }
@@ -150,13 +89,13 @@ __global__ void k_block_inverse(KGrid G, double* minv) {
   for (int k = 0; k < 4; k++) {
     const int gi = ei[k], gj = ej[k];
     if (gi < 0 || gi >= G.nx || gj < 0 || gj >= G.ny) continue;
-    const double t = G.elem_t ? (double)G.elem_t[(size_t)gj * G.nx + gi] : G.t0;
-    const double D = G.Dfac * t * t * t;
+    double D, Ds;
+    elem_material(G, gi, gj, D, Ds);
     for (int c = 0; c < 3; c++) {
       double u[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, y[12];
       u[3 * corner[k] + c] = 1.0;
       q4_rect_apply<double>(G.ax[2 * gi], G.by[2 * gj], G.ax[2 * gi + 1], G.by[2 * gj + 1], D,
-                            D * G.nu, D * (1.0 - G.nu) / 2.0, G.Gk * t, u, y);
+                            D * G.nu, D * (1.0 - G.nu) / 2.0, Ds, u, y);
       for (int r = 0; r < 3; r++) A[r * 3 + c] += y[3 * corner[k] + r];
     }
   }
@@ -225,22 +164,23 @@ __global__ void k_pressure(KGrid G, double q0, const double* pt, int np, float* 
 
 // -------------------------------------------------- persistent FP64 PCG
 
-// One workgroup per tile when RES (all per-node state kept in registers
-// across iterations; only p and z go through memory for the halos), or
-// tiles strided over the grid when !RES (state in HBM).
-//
-// Iteration k (two grid all-reduces):
-//   A: p = z + beta p_old staged with halo (p double-buffered), Ap = K p in
-//      LDS, BC rows Ap = p, partial p.Ap           -> all-reduce -> alpha
-//   B: x += alpha p, r -= alpha Ap, z = M^-1 r, partial r.z, r.r
-//                                                  -> all-reduce -> beta, |r|
-// which is pcg.ts:146-206 with the p update moved to the start of the next
-// iteration.  Stopping rule |r| < tol or |r|/|F| < tol (pcg.ts:174).
+// Iteration k (two all-reduces):
+//   A: p = z + beta p_old staged with halo (p double-buffered so no
+//      workgroup overwrites what a neighbour still reads), Ap = K p in LDS,
+//      BC rows Ap = p, partial p.Ap                  -> all-reduce -> alpha
+//   B: x += alpha p, r -= alpha Ap, z = M^-1 r, partials r.z, r.r
+//                                                    -> all-reduce -> beta, |r|
+// i.e. pcg.ts:146-206 with the p update moved to the start of the next
+// iteration.  Stopping rule |r| < tol or |r|/|F| < tol (pcg.ts:174), pAp
+// breakdown (pcg.ts:154), zero right-hand side (pcg.ts:118).  A restart
+// (r0 != nullptr) starts from the x in memory with residual r0 and measures
+// the relative test against the given |F|.  Only z and p are read by other
+// workgroups; they go through st_coh / ld_coh.
 struct PcgArgs {
   KGrid G;
   const double* F;
-  const double* r0;  // restart: initial residual F - K x (x read from x); nullptr = cold start
-  double bnorm_in;   // restart: |F| for the relative stopping test
+  const double* r0;
+  double bnorm_in;
   double *x, *r, *z, *pb0, *pb1, *Ap;
   const double* minv;
   double* part;
@@ -250,20 +190,28 @@ struct PcgArgs {
   int maxit;
 };
 
-template <bool RES>
-__global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
-  __shared__ double p_s[3 * NP];
-  __shared__ double ye_s[12 * NE];
-  __shared__ double red[64];
+// RES: one tile per workgroup, per-node x, r, z in registers (p stays in
+// LDS, Ap in ap_s between the phases).  !RES: tiles strided over the
+// workgroups, state in HBM.
+template <class C, bool RES>
+__global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
+  __shared__ double p_s[3 * C::NP];
+  __shared__ double ye_s[12 * C::NE];
+  __shared__ double red[80];
+  __shared__ double ap_s[RES ? 3 * C::NT : 1];
+  const unsigned me = blockIdx.x, nwg = gridDim.x;
   const KGrid& G = A.G;
-  const int ntx = tiles_x(G), ntiles = tiles_total(G);
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const bool node_thread = threadIdx.x < TX * TY;
+  const int ntx = tiles_x<C>(G), ntiles = tiles_total<C>(G);
+  const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
+  const bool node_thread = threadIdx.x < C::TX * C::TY;
   unsigned epoch = 0;
+  auto allreduce = [&](auto& v) -> bool {
+    publish();  // every thread's stores complete before the arrival
+    block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
+    return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
+  };
 
-  // per-thread state of the owned node (RES only)
-  double xr[3] = {0, 0, 0}, rr_[3] = {0, 0, 0}, zr[3] = {0, 0, 0}, pr[3] = {0, 0, 0};
-  double apr[3] = {0, 0, 0}, mi[6] = {0, 0, 0, 0, 0, 0};
+  double xr[3] = {0, 0, 0}, rr_[3] = {0, 0, 0}, zr[3] = {0, 0, 0};
   int64_t my_n = -1;
   unsigned my_fix = 0;
 
@@ -271,8 +219,8 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
   const bool restart = A.r0 != nullptr;
   const double* src = restart ? A.r0 : A.F;
   double acc[2] = {0, 0};
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
+  for (int tile = me; tile < ntiles; tile += nwg) {
+    const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
     const int gi = i0 + tx, gj = j0 + ty;
     if (node_thread && gi <= G.nx && gj <= G.ny) {
       const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
@@ -280,7 +228,7 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
       for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
       sym3_apply(m, rv, zv);
       for (int d = 0; d < 3; d++) {
-        A.z[3 * n + d] = zv[d];
+        st_coh(&A.z[3 * n + d], zv[d]);
         if (!RES) {
           A.r[3 * n + d] = rv[d];
           if (!restart) A.x[3 * n + d] = 0.0;
@@ -296,19 +244,17 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
           zr[d] = zv[d];
           xr[d] = restart ? A.x[3 * n + d] : 0.0;
         }
-        for (int k = 0; k < 6; k++) mi[k] = m[k];
       }
     }
   }
-  block_sum<2, NT>(acc, red + 8);
-  if (!grid_allreduce<2>(acc, A.part, A.gs, epoch, red)) return;
+  if (!allreduce(acc)) return;
   double rz = acc[0], rr = acc[1];
   const double bnorm = restart ? A.bnorm_in : sqrt(rr);
   const bool done0 = restart ? (sqrt(rr) < A.tol || sqrt(rr) / bnorm < A.tol) : bnorm < 1e-30;
   if (done0) {
     if (RES && my_n >= 0)
       for (int d = 0; d < 3; d++) A.x[3 * my_n + d] = xr[d];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (me == 0 && threadIdx.x == 0) {
       A.st->iterations = 0; A.st->converged = 1; A.st->breakdown = 0;
       A.st->final_residual = restart ? sqrt(rr) : 0.0; A.st->bnorm = bnorm;
     }
@@ -322,32 +268,31 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
     double* pnew = (it & 1) ? A.pb0 : A.pb1;
     // ---------------- phase A: p, Ap = K p, p.Ap
     double pap[1] = {0};
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-      const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
-      stage_tile<double, double>(G, i0, j0, A.z, it == 0 ? nullptr : pold, beta, p_s);
+    for (int tile = me; tile < ntiles; tile += nwg) {
+      const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
+      stage_tile<C, double, double, true>(G, i0, j0, A.z, it == 0 ? nullptr : pold, beta, p_s);
       __syncthreads();
-      tile_elements<double>(G, i0, j0, p_s, ye_s);
+      tile_elements<C, double>(G, i0, j0, p_s, ye_s);
       __syncthreads();
       const int gi = i0 + tx, gj = j0 + ty;
       if (node_thread && gi <= G.nx && gj <= G.ny) {
         const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
         const unsigned fix = RES ? my_fix : G.cmask[n];
         double v[3];
-        node_sum<double>(tx, ty, ye_s, v);
-        const int c = (ty + 1) * PW + tx + 1;
+        node_sum<C, double>(tx, ty, ye_s, v);
+        const int c = (ty + 1) * C::PW + tx + 1;
         for (int d = 0; d < 3; d++) {
-          const double pd = p_s[d * NP + c];
+          const double pd = p_s[d * C::NP + c];
           const double ad = ((fix >> d) & 1u) ? pd : v[d];
-          pnew[3 * n + d] = pd;
+          st_coh(&pnew[3 * n + d], pd);
           pap[0] += pd * ad;
-          if (RES) { pr[d] = pd; apr[d] = ad; }
+          if (RES) ap_s[d * C::NT + threadIdx.x] = ad;
           else A.Ap[3 * n + d] = ad;
         }
       }
       __syncthreads();
     }
-    block_sum<1, NT>(pap, red + 8);
-    if (!grid_allreduce<1>(pap, A.part, A.gs, epoch, red)) return;
+    if (!allreduce(pap)) return;
     if (fabs(pap[0]) < 1e-30) { breakdown = 1; it++; break; }
     const double alpha = rz / pap[0];
 
@@ -355,41 +300,41 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
     acc[0] = 0; acc[1] = 0;
     if (RES) {
       if (my_n >= 0) {
+        const int c = (ty + 1) * C::PW + tx + 1;
         for (int d = 0; d < 3; d++) {
-          xr[d] += alpha * pr[d];
-          rr_[d] -= alpha * apr[d];
+          xr[d] += alpha * p_s[d * C::NP + c];
+          rr_[d] -= alpha * ap_s[d * C::NT + threadIdx.x];
         }
-        sym3_apply(mi, rr_, zr);
+        sym3_apply(A.minv + 6 * my_n, rr_, zr);
         for (int d = 0; d < 3; d++) {
-          A.z[3 * my_n + d] = zr[d];
+          st_coh(&A.z[3 * my_n + d], zr[d]);
           acc[0] += rr_[d] * zr[d];
           acc[1] += rr_[d] * rr_[d];
         }
       }
     } else {
-      for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
+      for (int tile = me; tile < ntiles; tile += nwg) {
+        const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
         const int gi = i0 + tx, gj = j0 + ty;
         if (node_thread && gi <= G.nx && gj <= G.ny) {
           const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
           double rv[3], zv[3], m[6];
           for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
           for (int d = 0; d < 3; d++) {
-            A.x[3 * n + d] += alpha * pnew[3 * n + d];
+            A.x[3 * n + d] += alpha * ld_coh(&pnew[3 * n + d]);
             rv[d] = A.r[3 * n + d] - alpha * A.Ap[3 * n + d];
             A.r[3 * n + d] = rv[d];
           }
           sym3_apply(m, rv, zv);
           for (int d = 0; d < 3; d++) {
-            A.z[3 * n + d] = zv[d];
+            st_coh(&A.z[3 * n + d], zv[d]);
             acc[0] += rv[d] * zv[d];
             acc[1] += rv[d] * rv[d];
           }
         }
       }
     }
-    block_sum<2, NT>(acc, red + 8);
-    if (!grid_allreduce<2>(acc, A.part, A.gs, epoch, red)) return;
+    if (!allreduce(acc)) return;
     rr = acc[1];
     const double rn = sqrt(rr);
     if (rn < A.tol || rn / bnorm < A.tol) { converged = 1; it++; break; }
@@ -398,12 +343,185 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgArgs A) {
   }
   if (RES && my_n >= 0)
     for (int d = 0; d < 3; d++) A.x[3 * my_n + d] = xr[d];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (me == 0 && threadIdx.x == 0) {
     A.st->iterations = it;
     A.st->converged = converged;
     A.st->breakdown = breakdown;
     A.st->final_residual = sqrt(rr);
     A.st->bnorm = bnorm;
+  }
+}
+
+// Resident variant: one tile per workgroup and every per-node vector in
+// registers of the node's thread.  Only tile-edge nodes are ever read by
+// another workgroup (as its halo), so only they publish z and p (st_coh),
+// and each phase A fetches just the one-node halo ring (ld_coh): the tile
+// interior never goes through memory.  x is written once at the end.
+template <class C>
+__device__ __forceinline__ void ring_pos(int k, int& lx, int& ly) {
+  if (k < C::PW) { lx = k; ly = 0; }
+  else if (k < 2 * C::PW) { lx = k - C::PW; ly = C::PH - 1; }
+  else if (k < 2 * C::PW + C::TY) { lx = 0; ly = 1 + (k - 2 * C::PW); }
+  else { lx = C::PW - 1; ly = 1 + (k - 2 * C::PW - C::TY); }
+}
+
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_pcg_res(PcgArgs A) {
+  constexpr int NRING = 2 * C::PW + 2 * C::TY;
+  static_assert(NRING <= C::NT, "one ring node per thread");
+  __shared__ double p_s[3 * C::NP];
+  __shared__ double ye_s[12 * C::NE];
+  __shared__ double red[80];
+  const unsigned me = blockIdx.x, nwg = gridDim.x;
+  const KGrid& G = A.G;
+  const int ntx = tiles_x<C>(G);
+  const int i0 = (me % ntx) * C::TX, j0 = (me / ntx) * C::TY;
+  const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
+  const int gi = i0 + tx, gj = j0 + ty;
+  const bool mine = threadIdx.x < C::TX * C::TY && gi <= G.nx && gj <= G.ny;
+  const int64_t n = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
+  // tile-edge nodes are some neighbour's halo
+  const bool edge = mine && (tx == 0 || ty == 0 || tx == C::TX - 1 || ty == C::TY - 1 ||
+                             gi == G.nx || gj == G.ny);
+  const int c = (ty + 1) * C::PW + tx + 1;  // own node in p_s
+  // this thread's halo ring entry, if any
+  int rn_ = -1;
+  int64_t rnode = -1;
+  if (threadIdx.x < NRING) {
+    int lx, ly;
+    ring_pos<C>(threadIdx.x, lx, ly);
+    rn_ = ly * C::PW + lx;
+    const int hi = i0 - 1 + lx, hj = j0 - 1 + ly;
+    if (hi >= 0 && hi <= G.nx && hj >= 0 && hj <= G.ny) rnode = (int64_t)hj * (G.nx + 1) + hi;
+  }
+  unsigned epoch = 0;
+  const bool timing = me == 0 && threadIdx.x == 0;
+  unsigned long long tk[6] = {0, 0, 0, 0, 0, 0};
+  auto allreduce = [&](auto& v) -> bool {
+    publish();
+    block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
+    return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
+  };
+
+  // x, r, z in registers; p lives in the node's own p_s slot from one
+  // iteration to the next and Ap in ap_s between the two phases
+  __shared__ double ap_s[3 * C::NT];
+  double x[3] = {0, 0, 0}, r[3] = {0, 0, 0}, z[3] = {0, 0, 0};
+  const unsigned fix = mine ? G.cmask[n] : 0u;
+  const bool restart = A.r0 != nullptr;
+  double acc[2] = {0, 0};
+  if (mine) {
+    const double* src = restart ? A.r0 : A.F;
+    for (int d = 0; d < 3; d++) {
+      r[d] = src[3 * n + d];
+      x[d] = restart ? A.x[3 * n + d] : 0.0;
+    }
+    sym3_apply(A.minv + 6 * n, r, z);
+    for (int d = 0; d < 3; d++) {
+      if (edge) st_coh(&A.z[3 * n + d], z[d]);
+      acc[0] += r[d] * z[d];
+      acc[1] += r[d] * r[d];
+    }
+  }
+  if (!allreduce(acc)) return;
+  double rz = acc[0], rr = acc[1];
+  const double bnorm = restart ? A.bnorm_in : sqrt(rr);
+  const bool done0 = restart ? (sqrt(rr) < A.tol || sqrt(rr) / bnorm < A.tol) : bnorm < 1e-30;
+  if (done0) {
+    if (mine)
+      for (int d = 0; d < 3; d++) A.x[3 * n + d] = x[d];
+    if (me == 0 && threadIdx.x == 0) {
+      A.st->iterations = 0; A.st->converged = 1; A.st->breakdown = 0;
+      A.st->final_residual = restart ? sqrt(rr) : 0.0; A.st->bnorm = bnorm;
+    }
+    return;
+  }
+
+  double beta = 0;
+  int it = 0, converged = 0, breakdown = 0;
+  for (; it < A.maxit; it++) {
+    const double* pold = (it & 1) ? A.pb1 : A.pb0;
+    double* pnew = (it & 1) ? A.pb0 : A.pb1;
+    unsigned long long t0 = timing ? wall_clock64() : 0;
+    // ---------------- phase A: p = z + beta p, halo ring, Ap = K p, p.Ap
+    // every staged position is written once: own nodes, the ring (zero
+    // outside the grid), and zero for tile slots past the grid's edge
+    if (mine) {
+      for (int d = 0; d < 3; d++) {
+        const double pd = it == 0 ? z[d] : z[d] + beta * p_s[d * C::NP + c];
+        p_s[d * C::NP + c] = pd;
+        if (edge) st_coh(&pnew[3 * n + d], pd);
+      }
+    } else if (threadIdx.x < C::TX * C::TY) {
+      for (int d = 0; d < 3; d++) p_s[d * C::NP + c] = 0.0;
+    }
+    if (rn_ >= 0) {
+      for (int d = 0; d < 3; d++) {
+        double v = 0.0;
+        if (rnode >= 0) {
+          v = ld_coh(&A.z[3 * rnode + d]);
+          if (it > 0) v += beta * ld_coh(&pold[3 * rnode + d]);
+        }
+        p_s[d * C::NP + rn_] = v;
+      }
+    }
+    __syncthreads();
+    unsigned long long t1 = timing ? wall_clock64() : 0;
+    tile_elements<C, double>(G, i0, j0, p_s, ye_s);
+    __syncthreads();
+    unsigned long long t2 = timing ? wall_clock64() : 0;
+    double pap[1] = {0};
+    if (mine) {
+      double v[3];
+      node_sum<C, double>(tx, ty, ye_s, v);
+      for (int d = 0; d < 3; d++) {
+        const double pd = p_s[d * C::NP + c];
+        const double ad = ((fix >> d) & 1u) ? pd : v[d];
+        ap_s[d * C::NT + threadIdx.x] = ad;
+        pap[0] += pd * ad;
+      }
+    }
+    unsigned long long t3 = timing ? wall_clock64() : 0;
+    if (!allreduce(pap)) return;
+    unsigned long long t4 = timing ? wall_clock64() : 0;
+    if (fabs(pap[0]) < 1e-30) { breakdown = 1; it++; break; }
+    const double alpha = rz / pap[0];
+    // ---------------- phase B: x, r, z, r.z, r.r
+    acc[0] = 0; acc[1] = 0;
+    if (mine) {
+      for (int d = 0; d < 3; d++) {
+        x[d] += alpha * p_s[d * C::NP + c];
+        r[d] -= alpha * ap_s[d * C::NT + threadIdx.x];
+      }
+      sym3_apply(A.minv + 6 * n, r, z);
+      for (int d = 0; d < 3; d++) {
+        if (edge) st_coh(&A.z[3 * n + d], z[d]);
+        acc[0] += r[d] * z[d];
+        acc[1] += r[d] * r[d];
+      }
+    }
+    unsigned long long t5 = timing ? wall_clock64() : 0;
+    if (!allreduce(acc)) return;
+    if (timing) {
+      const unsigned long long t6 = wall_clock64();
+      tk[0] += t1 - t0; tk[1] += t2 - t1; tk[2] += t3 - t2;
+      tk[3] += t4 - t3; tk[4] += t5 - t4; tk[5] += t6 - t5;
+    }
+    rr = acc[1];
+    const double rnorm = sqrt(rr);
+    if (rnorm < A.tol || rnorm / bnorm < A.tol) { converged = 1; it++; break; }
+    beta = acc[0] / rz;
+    rz = acc[0];
+  }
+  if (mine)
+    for (int d = 0; d < 3; d++) A.x[3 * n + d] = x[d];
+  if (me == 0 && threadIdx.x == 0) {
+    A.st->iterations = it;
+    A.st->converged = converged;
+    A.st->breakdown = breakdown;
+    A.st->final_residual = sqrt(rr);
+    A.st->bnorm = bnorm;
+    for (int k = 0; k < 6; k++) A.st->ticks[k] = tk[k];
   }
 }
 
@@ -414,25 +532,86 @@ __global__ void k_zero_sync(GridSync* gs) {
   }
 }
 
-KGrid make_kgrid(const pp_plan* pl) {
-  KGrid G;
-  G.nx = pl->nx;
-  G.ny = pl->ny;
-  G.xc = pl->xc.p;
-  G.yc = pl->yc.p;
-  G.elem_t = pl->var_t ? pl->elem_t.p : nullptr;
-  G.t0 = pl->t;
-  G.Dfac = pl->E / (12.0 * (1.0 - pl->nu * pl->nu));
-  G.Gk = (5.0 / 6.0) * pl->E / (2.0 * (1.0 + pl->nu));
-  G.nu = pl->nu;
-  G.cmask = pl->cmask.p;
-  // column widths and row heights are stored after the coordinates
-  G.ax = pl->xc.p + (pl->nx + 1);
-  G.by = pl->yc.p + (pl->ny + 1);
-  return G;
+// Launch shape of the persistent PCG for this plan.
+struct PcgShape {
+  int kind;  // 0 = 31x31 tiles resident, 1 = 15x15 resident, 2 = 15x15 strided
+  int grid;
+};
+
+// Large tiles (fewer workgroups, cheaper barriers) while they are few;
+// smaller resident tiles next; tiles strided over a resident grid last.
+constexpr int kMaxLargeTiles = 64;
+
+int pcg_shape(pp_plan* pl, bool small_tiles, PcgShape* sh) {
+  const int nx = pl->nx, ny = pl->ny;
+  const int nl = ((nx + 1 + CfgX::TX - 1) / CfgX::TX) * ((ny + 1 + CfgX::TY - 1) / CfgX::TY);
+  const int ns = ((nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((ny + 1 + CfgS::TY - 1) / CfgS::TY);
+  int per_cu = 0;
+  if (!small_tiles && nl <= kMaxLargeTiles) {
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_res<CfgX>, CfgX::NT, 0));
+    if (nl <= per_cu * pl->ctx->cu_count) {
+      sh->kind = 0;
+      sh->grid = nl;
+      return PP_OK;
+    }
+  }
+  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_res<CfgS>, CfgS::NT, 0));
+  if (ns <= per_cu * pl->ctx->cu_count) {
+    sh->kind = 1;
+    sh->grid = ns;
+    return PP_OK;
+  }
+  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgS, false>, CfgS::NT, 0));
+  sh->kind = 2;
+  sh->grid = std::min(ns, per_cu * pl->ctx->cu_count);
+  if (sh->grid <= 0) return fail(PP_ERR_HIP, "persistent PCG kernel cannot be made resident");
+  return PP_OK;
 }
 
-int grid_blocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+// One launch of the persistent PCG (cold start when r0 is null).  A plain
+// launch sized to what is co-resident: the workgroups then all run at once
+// on an otherwise idle device; if they cannot (another process holds CUs)
+// the barrier spin budget runs out and the call fails with PP_ERR_TIMEOUT
+// instead of hanging.  (A cooperative launch would refuse to run at all
+// while another process has the device open.)
+int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxit,
+               bool small_tiles, DevStats* ds) {
+  hipStream_t s = pl->ctx->stream;
+  PcgShape sh;
+  int rc = pcg_shape(pl, small_tiles, &sh);
+  if (rc) return rc;
+  if (pl->partials.n < (size_t)4 * sh.grid && (rc = pl->partials.alloc(4 * (size_t)sh.grid)))
+    return rc;
+  PcgArgs A;
+  A.G = make_kgrid(pl);
+  A.F = pl->F.p;
+  A.r0 = r0;
+  A.bnorm_in = bnorm;
+  A.x = pl->x.p; A.r = pl->r.p; A.z = pl->z.p; A.pb0 = pl->p0.p; A.pb1 = pl->p1.p;
+  A.Ap = pl->Ap.p;
+  A.minv = pl->minv.p;
+  A.part = pl->partials.p;
+  A.gs = pl->gsync.p;
+  A.st = pl->dstats.p;
+  A.tol = tol;
+  A.maxit = maxit;
+  hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, s, pl->gsync.p);
+  PP_HIP(hipMemsetAsync(pl->dstats.p, 0, sizeof(DevStats), s));
+  PP_HIP(hipMemsetAsync(pl->partials.p, 0, 4 * (size_t)sh.grid * sizeof(double), s));
+  if (sh.kind == 0)
+    hipLaunchKernelGGL((k_pcg_res<CfgX>), dim3(sh.grid), dim3(CfgX::NT), 0, s, A);
+  else if (sh.kind == 1)
+    hipLaunchKernelGGL((k_pcg_res<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
+  else
+    hipLaunchKernelGGL((k_pcg<CfgS, false>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
+  PP_HIP(hipGetLastError());
+  GridSync gsh;
+  PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipMemcpyAsync(&gsh, pl->gsync.p, sizeof(gsh), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipStreamSynchronize(s));
+  if (gsh.abort) return fail(PP_ERR_TIMEOUT, "grid barrier spin limit reached");
+  return PP_OK;
+}
 
 }  // namespace
 
@@ -468,8 +647,9 @@ int struct_setup(pp_plan* pl) {
 
 int struct_apply_k(pp_plan* pl, const double* dx, double* dy) {
   KGrid G = make_kgrid(pl);
-  const int ntiles = ((pl->nx + 1 + TX - 1) / TX) * ((pl->ny + 1 + TY - 1) / TY);
-  hipLaunchKernelGGL(k_apply, dim3(ntiles), dim3(NT), 0, pl->ctx->stream, G, dx, dy);
+  const int ntiles =
+      ((pl->nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((pl->ny + 1 + CfgS::TY - 1) / CfgS::TY);
+  hipLaunchKernelGGL(k_apply, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, dx, dy);
   PP_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -495,48 +675,6 @@ int struct_true_residual(pp_plan* pl, const double* dx, double* rel) {
   return launch_norm_ratio(pl, pl->F.p, pl->tmp.p, rel);
 }
 
-// One cooperative launch of the persistent PCG (cold start when r0 is null).
-static int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxit,
-                      DevStats* ds) {
-  hipStream_t s = pl->ctx->stream;
-  const int ntiles = ((pl->nx + 1 + TX - 1) / TX) * ((pl->ny + 1 + TY - 1) / TY);
-  int per_cu = 0;
-  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<true>, NT, 0));
-  const bool res = ntiles <= per_cu * pl->ctx->cu_count;
-  int per_cu_s = 0;
-  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_pcg<false>, NT, 0));
-  const int nwg = res ? ntiles : std::min(ntiles, per_cu_s * pl->ctx->cu_count);
-  if (nwg <= 0) return fail(PP_ERR_HIP, "persistent PCG kernel cannot be made resident");
-  int rc;
-  if (pl->partials.n < (size_t)4 * nwg && (rc = pl->partials.alloc(4 * (size_t)nwg))) return rc;
-  PcgArgs A;
-  A.G = make_kgrid(pl);
-  A.F = pl->F.p;
-  A.r0 = r0;
-  A.bnorm_in = bnorm;
-  A.x = pl->x.p; A.r = pl->r.p; A.z = pl->z.p; A.pb0 = pl->p0.p; A.pb1 = pl->p1.p;
-  A.Ap = pl->Ap.p;
-  A.minv = pl->minv.p;
-  A.part = pl->partials.p;
-  A.gs = pl->gsync.p;
-  A.st = pl->dstats.p;
-  A.tol = tol;
-  A.maxit = maxit;
-  void* args[] = {&A};
-  hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, s, pl->gsync.p);
-  PP_HIP(hipGetLastError());
-  if (res)
-    PP_HIP(hipLaunchCooperativeKernel((const void*)k_pcg<true>, dim3(nwg), dim3(NT), args, 0, s));
-  else
-    PP_HIP(hipLaunchCooperativeKernel((const void*)k_pcg<false>, dim3(nwg), dim3(NT), args, 0, s));
-  GridSync gsh;
-  PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
-  PP_HIP(hipMemcpyAsync(&gsh, pl->gsync.p, sizeof(gsh), hipMemcpyDeviceToHost, s));
-  PP_HIP(hipStreamSynchronize(s));
-  if (gsh.abort) return fail(PP_ERR_TIMEOUT, "grid barrier spin limit reached");
-  return PP_OK;
-}
-
 // FP64 PCG with residual replacement: after the recurrence converges the
 // true residual F - K x is recomputed in FP64; while it is above the
 // tolerance (the recurrence drifts from it by rounding) PCG restarts from x
@@ -546,10 +684,12 @@ int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   hipStream_t s = pl->ctx->stream;
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   DevStats ds;
-  int rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, &ds);
+  const bool dw = (o->flags & PP_FLAG_SMALL_TILES) != 0;
+  int rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, dw, &ds);
   if (rc) return rc;
   const double bnorm = ds.bnorm;
   int total = ds.iterations;
+  for (int k = 0; k < 6; k++) st->phase_us[k] = 0.01 * (double)ds.ticks[k];
   st->sweeps = 1;
   st->sweep_iterations[0] = ds.iterations;
   double rel = 0;
@@ -557,12 +697,13 @@ int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     for (int k = 0;; k++) {
       if ((rc = struct_apply_k(pl, pl->x.p, pl->tmp.p))) return rc;
       if ((rc = launch_norm_ratio(pl, pl->F.p, pl->tmp.p, &rel))) return rc;
-      st->sweep_residual[st->sweeps - 1] = rel;
+      st->sweep_residual[std::min(st->sweeps, PP_MAX_SWEEPS) - 1] = rel;
       if (rel <= o->tolerance || k == kMaxRestarts || total >= o->max_iterations) break;
       if ((rc = launch_residual(pl, pl->F.p, pl->tmp.p, pl->tmp.p))) return rc;
-      rc = launch_pcg(pl, pl->tmp.p, bnorm, o->tolerance, o->max_iterations - total, &ds);
+      rc = launch_pcg(pl, pl->tmp.p, bnorm, o->tolerance, o->max_iterations - total, dw, &ds);
       if (rc) return rc;
       total += ds.iterations;
+      for (int k = 0; k < 6; k++) st->phase_us[k] += 0.01 * (double)ds.ticks[k];
       if (st->sweeps < PP_MAX_SWEEPS) st->sweep_iterations[st->sweeps] = ds.iterations;
       st->sweeps++;
       if (!ds.converged) break;

@@ -24,6 +24,7 @@ PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1
 PATH_AUTO, PATH_STRUCTURED, PATH_COLOURED = 0, 1, 2
 MAX_SWEEPS = 32
 FLAG_NO_TRUE_RESIDUAL = 1
+FLAG_SMALL_TILES = 2
 
 
 class Mesh(C.Structure):
@@ -54,7 +55,7 @@ class SolveStats(C.Structure):
                 ("sweeps", C.c_int32), ("final_residual", C.c_double),
                 ("true_rel_residual", C.c_double), ("device_ms", C.c_double),
                 ("sweep_iterations", C.c_int32 * MAX_SWEEPS),
-                ("sweep_residual", C.c_double * MAX_SWEEPS)]
+                ("sweep_residual", C.c_double * MAX_SWEEPS), ("phase_us", C.c_double * 6)]
 
 
 # name -> (restype, argtypes)

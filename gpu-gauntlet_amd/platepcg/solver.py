@@ -149,6 +149,7 @@ def stats_dict(st):
         "deviceMs": st.device_ms,
         "sweepIterations": [st.sweep_iterations[i] for i in range(min(n, _lib.MAX_SWEEPS))],
         "sweepResidual": [st.sweep_residual[i] for i in range(min(n, _lib.MAX_SWEEPS))],
+        "phaseUs": [st.phase_us[i] for i in range(6)],
     }
 
 

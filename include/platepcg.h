@@ -89,6 +89,9 @@ typedef struct {
 /* Skip the FP64 |F - K x| / |F| recomputation after the solve
  * (stats.true_rel_residual is then 0). */
 #define PP_FLAG_NO_TRUE_RESIDUAL 1
+/* Structured path: use the 15x15-node tiles even where the plate fits the
+ * large-tile persistent kernel (for comparison and testing). */
+#define PP_FLAG_SMALL_TILES 2
 
 #define PP_MAX_SWEEPS 32
 
@@ -104,6 +107,11 @@ typedef struct {
   double device_ms;           /* HIP-event time of the resident solve */
   int32_t sweep_iterations[PP_MAX_SWEEPS];
   double sweep_residual[PP_MAX_SWEEPS]; /* true FP64 rel. residual after each sweep */
+  /* Wall time (us) workgroup 0 of the resident persistent kernel spent in
+   * each part of an iteration, summed over iterations: halo/stage, element
+   * apply, node sums, all-reduce of p.Ap, vector updates, all-reduce of
+   * r.z and r.r.  Zero where not measured. */
+  double phase_us[6];
 } pp_solve_stats;
 
 typedef struct pp_ctx pp_ctx;

@@ -153,3 +153,19 @@ def test_edge_cases(ctx):
     F = np.random.default_rng(5).standard_normal(p2.n_dof).astype(np.float32)
     x64, _, st = p2.solve(F)
     assert st["converged"] and rel(x64, F.astype(np.float64)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [200, 600])
+def test_large_grids_match_oracle_iterates(ctx, n):
+    """Plates too big for one XCD run the device-wide kernels (resident at
+    200 x 200, strided over tiles at 600 x 600): their first 30 PCG iterates
+    equal the FP64 oracle's."""
+    mesh = P.rect_mesh(n)
+    mask = P.constrained_mask(mesh, [{"type": "pinned", "location": "all_edges"}])
+    plan = Plan(ctx, mesh, E, NU, 0.05, mask)
+    op = O.OracleProblem(mesh, E, NU, 0.05, mask)
+    F = plan.pressure_load(1e4, np.array([[3e4, 3.0, 6.0, 1.5]]))
+    x64, _, st = plan.solve(F, max_iterations=30)
+    assert st["iterations"] == 30 and not st["converged"]
+    xo, so = op.pcg(F.astype(np.float64), max_iter=30)
+    assert rel(x64, xo) < 1e-9
