@@ -76,15 +76,17 @@ __device__ __forceinline__ void st_coh(float* p, float v) {
 __device__ __forceinline__ void publish() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Grid-wide all-reduce + barrier of a persistent kernel (all nwg workgroups
-// co-resident).  Thread 0 of every workgroup holds its block's NV (<= 2)
-// partial sums in v; it publishes them into slot (epoch & 1) of `part`
-// (2 * nwg doubles per slot), waits for its stores, arrives on the counter
-// and spins until all nwg workgroups have arrived.  Wave 0 then sums the
-// partials in a fixed order, so every workgroup gets bit-identical totals and
-// takes identical control decisions.  Two slots suffice: no workgroup can
-// publish epoch e+2 before all have passed e+1, i.e. finished reading e.
-// Returns false when the barrier gave up (spin budget spent here or the
-// abort flag raised elsewhere); the kernel then exits.
+// co-resident).  Thread 0 of every workgroup holds its block's NV (<= 4)
+// partial sums in v and publishes them into slot (epoch & 1) of `part`
+// (4 * nwg words per slot, 8 * nwg in all); wave 0 of every workgroup then
+// waits until all nwg partials of this epoch are there and sums them in a
+// fixed order, so every workgroup gets bit-identical totals and takes
+// identical control decisions.  Two slots suffice: no workgroup can publish
+// epoch e+2 before all have passed e+1, i.e. finished reading e.  The
+// caller makes every thread's earlier stores complete (publish() before the
+// workgroup barrier), so data written before the call is visible to every
+// workgroup after it.  Returns false when the barrier gave up (spin budget
+// spent here or the abort flag raised elsewhere); the kernel then exits.
 //
 // No arrival counter: the partial itself is the arrival.  Each partial is
 // stored as one 64-bit word whose 8 low mantissa bits carry the epoch tag
@@ -104,8 +106,8 @@ template <int NV>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* part, GridSync* gs,
                                                unsigned& epoch, double* lds, unsigned nwg,
                                                unsigned me) {
-  static_assert(NV >= 1 && NV <= 2, "two values per slot");
-  unsigned long long* slot = (unsigned long long*)part + (size_t)(epoch & 1u) * 2 * nwg;
+  static_assert(NV >= 1 && NV <= 4, "four values per slot");
+  unsigned long long* slot = (unsigned long long*)part + (size_t)(epoch & 1u) * 4 * nwg;
   const unsigned tag = (epoch + 1u) & 0xffu ? (epoch + 1u) & 0xffu : 0x80u;
   if (threadIdx.x < 64) {
     if (threadIdx.x == 0) {

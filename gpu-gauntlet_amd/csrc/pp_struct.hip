@@ -525,6 +525,191 @@ __global__ __launch_bounds__(C::NT) void k_pcg_res(PcgArgs A) {
   }
 }
 
+// ------------------------------ single-reduction (Chronopoulos-Gear) PCG
+
+// Same iterates as pcg.ts:98-215 in exact arithmetic, one all-reduce per
+// iteration (Chronopoulos & Gear 1989):
+//   x += a p;  r -= a s;  u = M^-1 r;  w = K u
+//   g' = r.u, d' = w.u, rho' = r.r                  (one all-reduce)
+//   b = g'/g;  pAp = d' - b g'/a;  a' = g'/pAp;  p = u + b p;  s = w + b s
+// (s = K p is carried instead of recomputed.)  K u needs u on the one-node
+// halo ring: every workgroup keeps r and s of its ring nodes itself, updated
+// with the owners' arithmetic (cg_step) from the w values the owners publish
+// before the all-reduce, so the ring copies stay bit-identical and no second
+// barrier is needed for the halo.  Stopping rule, breakdown and zero
+// right-hand side as pcg.ts:118-181; |r_{k+1}| arrives with the all-reduce
+// that follows the matvec of u_{k+1}.
+__device__ __forceinline__ void cg_rs(double a, const double (&s)[3], double (&r)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) r[d] = fma(-a, s[d], r[d]);
+}
+__device__ __forceinline__ void cg_sw(double b, const double (&w)[3], double (&s)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) s[d] = fma(b, s[d], w[d]);
+}
+__device__ __forceinline__ void precond_fma(const double* m, const double (&r)[3], double (&u)[3]) {
+  const double m00 = m[0], m01 = m[1], m02 = m[2], m11 = m[3], m12 = m[4], m22 = m[5];
+  u[0] = fma(m00, r[0], fma(m01, r[1], m02 * r[2]));
+  u[1] = fma(m01, r[0], fma(m11, r[1], m12 * r[2]));
+  u[2] = fma(m02, r[0], fma(m12, r[1], m22 * r[2]));
+}
+
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_pcg_cg(PcgArgs A) {
+  constexpr int NRING = 2 * C::PW + 2 * C::TY;
+  static_assert(NRING <= C::NT, "one ring node per thread");
+  __shared__ double u_s[3 * C::NP];
+  __shared__ double ye_s[12 * C::NE];
+  __shared__ double red[80];
+  const unsigned me = blockIdx.x, nwg = gridDim.x;
+  const KGrid& G = A.G;
+  const int ntx = tiles_x<C>(G);
+  const int i0 = (me % ntx) * C::TX, j0 = (me / ntx) * C::TY;
+  const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
+  const int gi = i0 + tx, gj = j0 + ty;
+  const bool mine = threadIdx.x < C::TX * C::TY && gi <= G.nx && gj <= G.ny;
+  const int64_t n = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
+  const bool edge = mine && (tx == 0 || ty == 0 || tx == C::TX - 1 || ty == C::TY - 1 ||
+                             gi == G.nx || gj == G.ny);
+  const int c = (ty + 1) * C::PW + tx + 1;
+  int rpos = -1;
+  int64_t rnode = -1;
+  if (threadIdx.x < NRING) {
+    int lx, ly;
+    ring_pos<C>(threadIdx.x, lx, ly);
+    rpos = ly * C::PW + lx;
+    const int hi = i0 - 1 + lx, hj = j0 - 1 + ly;
+    if (hi >= 0 && hi <= G.nx && hj >= 0 && hj <= G.ny) rnode = (int64_t)hj * (G.nx + 1) + hi;
+  }
+  unsigned epoch = 0;
+  const bool timing = me == 0 && threadIdx.x == 0;
+  unsigned long long tk[6] = {0, 0, 0, 0, 0, 0};
+
+  // own state
+  double x[3] = {0, 0, 0}, r[3] = {0, 0, 0}, u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+  double p[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+  // ring copies (r, s) and the ring node's M^-1
+  double rr_[3] = {0, 0, 0}, rs_[3] = {0, 0, 0}, rm[6] = {0, 0, 0, 0, 0, 0};
+  const unsigned fix = mine ? G.cmask[n] : 0u;
+  const bool restart = A.r0 != nullptr;
+  const double* src = restart ? A.r0 : A.F;
+  const double* mown = A.minv + 6 * n;
+  if (mine)
+    for (int d = 0; d < 3; d++) {
+      r[d] = src[3 * n + d];
+      x[d] = restart ? A.x[3 * n + d] : 0.0;
+    }
+  if (rnode >= 0) {
+    for (int d = 0; d < 3; d++) rr_[d] = src[3 * rnode + d];
+    for (int k = 0; k < 6; k++) rm[k] = A.minv[6 * rnode + k];
+  }
+
+  // u = M^-1 r on own nodes and ring, staged; w = K u; edge w published;
+  // partials r.u, w.u, r.r
+  auto matvec = [&](double* wpub, double (&acc)[3]) {
+    if (mine) {
+      precond_fma(mown, r, u);
+      for (int d = 0; d < 3; d++) u_s[d * C::NP + c] = u[d];
+    } else if (threadIdx.x < C::TX * C::TY) {
+      for (int d = 0; d < 3; d++) u_s[d * C::NP + c] = 0.0;
+    }
+    if (rpos >= 0) {
+      double ru[3] = {0, 0, 0};
+      if (rnode >= 0) precond_fma(rm, rr_, ru);
+      for (int d = 0; d < 3; d++) u_s[d * C::NP + rpos] = ru[d];
+    }
+    __syncthreads();
+    tile_elements<C, double>(G, i0, j0, u_s, ye_s);
+    __syncthreads();
+    acc[0] = acc[1] = acc[2] = 0;
+    if (mine) {
+      double v[3];
+      node_sum<C, double>(tx, ty, ye_s, v);
+      for (int d = 0; d < 3; d++) {
+        w[d] = ((fix >> d) & 1u) ? u[d] : v[d];
+        if (edge) st_coh(&wpub[3 * n + d], w[d]);
+        acc[0] += r[d] * u[d];
+        acc[1] += w[d] * u[d];
+        acc[2] += r[d] * r[d];
+      }
+    }
+  };
+  auto allreduce = [&](double (&v)[3]) -> bool {
+    publish();
+    block_sum<3, C::NT>(v, red + 16);
+    return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
+  };
+  // ring w published by the owners into wpub before the last all-reduce
+  auto ring_w = [&](const double* wpub, double (&wv)[3]) {
+    for (int d = 0; d < 3; d++) wv[d] = rnode >= 0 ? ld_coh(&wpub[3 * rnode + d]) : 0.0;
+  };
+
+  // ---------------- init: u0, w0 = K u0, g0 = r.u, d0 = w.u
+  double acc[3];
+  matvec(A.pb0, acc);
+  if (!allreduce(acc)) return;
+  double g = acc[0], pap = acc[1];
+  const double bnorm = restart ? A.bnorm_in : sqrt(acc[2]);
+  double rho = acc[2];
+  int it = 0, converged = 0, breakdown = 0;
+  const bool done0 = restart ? (sqrt(rho) < A.tol || sqrt(rho) / bnorm < A.tol) : bnorm < 1e-30;
+  if (done0) {
+    converged = 1;
+  } else if (A.maxit > 0 && fabs(pap) < 1e-30) {
+    breakdown = 1;
+    it = 1;
+  } else {
+    double a = g / pap;
+    for (int d = 0; d < 3; d++) { p[d] = u[d]; s[d] = w[d]; }
+    if (rpos >= 0) ring_w(A.pb0, rs_);
+    for (; it < A.maxit;) {
+      unsigned long long t0 = timing ? wall_clock64() : 0;
+      double* wpub = (it & 1) ? A.pb0 : A.pb1;
+      // x += a p, r -= a s (own); r -= a s (ring copies)
+      if (mine)
+        for (int d = 0; d < 3; d++) x[d] = fma(a, p[d], x[d]);
+      cg_rs(a, s, r);
+      if (rnode >= 0) cg_rs(a, rs_, rr_);
+      unsigned long long t1 = timing ? wall_clock64() : 0;
+      matvec(wpub, acc);
+      unsigned long long t2 = timing ? wall_clock64() : 0;
+      if (!allreduce(acc)) return;
+      unsigned long long t3 = timing ? wall_clock64() : 0;
+      it++;
+      const double g1 = acc[0], d1 = acc[1];
+      rho = acc[2];
+      const double rn = sqrt(rho);
+      if (rn < A.tol || rn / bnorm < A.tol) { converged = 1; break; }
+      const double b = g1 / g;
+      pap = d1 - b * g1 / a;
+      if (fabs(pap) < 1e-30) { breakdown = 1; it++; break; }
+      a = g1 / pap;
+      g = g1;
+      for (int d = 0; d < 3; d++) p[d] = fma(b, p[d], u[d]);
+      cg_sw(b, w, s);
+      if (rpos >= 0) {
+        double wv[3];
+        ring_w(wpub, wv);
+        if (rnode >= 0) cg_sw(b, wv, rs_);
+      }
+      if (timing) {
+        const unsigned long long t4 = wall_clock64();
+        tk[0] += t1 - t0; tk[1] += t2 - t1; tk[3] += t3 - t2; tk[4] += t4 - t3;
+      }
+    }
+  }
+  if (mine)
+    for (int d = 0; d < 3; d++) A.x[3 * n + d] = x[d];
+  if (me == 0 && threadIdx.x == 0) {
+    A.st->iterations = it;
+    A.st->converged = converged;
+    A.st->breakdown = breakdown;
+    A.st->final_residual = sqrt(rho);
+    A.st->bnorm = bnorm;
+    for (int k = 0; k < 6; k++) A.st->ticks[k] = tk[k];
+  }
+}
+
 __global__ void k_zero_sync(GridSync* gs) {
   if (threadIdx.x < 64) {
     gs->count = 0;
@@ -542,11 +727,20 @@ struct PcgShape {
 // smaller resident tiles next; tiles strided over a resident grid last.
 constexpr int kMaxLargeTiles = 64;
 
-int pcg_shape(pp_plan* pl, bool small_tiles, PcgShape* sh) {
+int pcg_shape(pp_plan* pl, int variant, PcgShape* sh) {
   const int nx = pl->nx, ny = pl->ny;
   const int nl = ((nx + 1 + CfgX::TX - 1) / CfgX::TX) * ((ny + 1 + CfgX::TY - 1) / CfgX::TY);
   const int ns = ((nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((ny + 1 + CfgS::TY - 1) / CfgS::TY);
   int per_cu = 0;
+  const bool small_tiles = variant == PP_FLAG_SMALL_TILES;
+  if (variant == 0) {  // default: single-reduction kernel while it is resident
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_cg<CfgS>, CfgS::NT, 0));
+    if (ns <= per_cu * pl->ctx->cu_count) {
+      sh->kind = 3;
+      sh->grid = ns;
+      return PP_OK;
+    }
+  }
   if (!small_tiles && nl <= kMaxLargeTiles) {
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_res<CfgX>, CfgX::NT, 0));
     if (nl <= per_cu * pl->ctx->cu_count) {
@@ -575,12 +769,12 @@ int pcg_shape(pp_plan* pl, bool small_tiles, PcgShape* sh) {
 // instead of hanging.  (A cooperative launch would refuse to run at all
 // while another process has the device open.)
 int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxit,
-               bool small_tiles, DevStats* ds) {
+               int variant, DevStats* ds) {
   hipStream_t s = pl->ctx->stream;
   PcgShape sh;
-  int rc = pcg_shape(pl, small_tiles, &sh);
+  int rc = pcg_shape(pl, variant, &sh);
   if (rc) return rc;
-  if (pl->partials.n < (size_t)4 * sh.grid && (rc = pl->partials.alloc(4 * (size_t)sh.grid)))
+  if (pl->partials.n < (size_t)8 * sh.grid && (rc = pl->partials.alloc(8 * (size_t)sh.grid)))
     return rc;
   PcgArgs A;
   A.G = make_kgrid(pl);
@@ -597,13 +791,15 @@ int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxi
   A.maxit = maxit;
   hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, s, pl->gsync.p);
   PP_HIP(hipMemsetAsync(pl->dstats.p, 0, sizeof(DevStats), s));
-  PP_HIP(hipMemsetAsync(pl->partials.p, 0, 4 * (size_t)sh.grid * sizeof(double), s));
+  PP_HIP(hipMemsetAsync(pl->partials.p, 0, 8 * (size_t)sh.grid * sizeof(double), s));
   if (sh.kind == 0)
     hipLaunchKernelGGL((k_pcg_res<CfgX>), dim3(sh.grid), dim3(CfgX::NT), 0, s, A);
   else if (sh.kind == 1)
     hipLaunchKernelGGL((k_pcg_res<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
-  else
+  else if (sh.kind == 2)
     hipLaunchKernelGGL((k_pcg<CfgS, false>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
+  else
+    hipLaunchKernelGGL((k_pcg_cg<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
   PP_HIP(hipGetLastError());
   GridSync gsh;
   PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
@@ -684,7 +880,8 @@ int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   hipStream_t s = pl->ctx->stream;
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   DevStats ds;
-  const bool dw = (o->flags & PP_FLAG_SMALL_TILES) != 0;
+  // 0: single-reduction kernel; 1: two-reduction kernel; 2: two-reduction on small tiles
+  const int dw = (o->flags & PP_FLAG_SMALL_TILES) ? 2 : (o->flags & PP_FLAG_CLASSIC_PCG) ? 1 : 0;
   int rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, dw, &ds);
   if (rc) return rc;
   const double bnorm = ds.bnorm;

@@ -92,6 +92,9 @@ typedef struct {
 /* Structured path: use the 15x15-node tiles even where the plate fits the
  * large-tile persistent kernel (for comparison and testing). */
 #define PP_FLAG_SMALL_TILES 2
+/* Structured path: the two-reduction PCG kernel (the recurrence order of
+ * pcg.ts) instead of the default single-reduction kernel. */
+#define PP_FLAG_CLASSIC_PCG 4
 
 #define PP_MAX_SWEEPS 32
 
