@@ -33,7 +33,21 @@ def parse():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-sample-iters", type=int, default=1200,
                     help="iterations of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (profiler calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy of known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 5}))
 
 
 def cpu_baseline(w, F, full_iters, sample_iters):
@@ -62,6 +76,9 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     import numpy as np
 
     from platepcg import Context, Plan, _lib
@@ -91,10 +108,12 @@ def main():
     bytes_per_launch = BYTES_PER_DOF_ITER_FP64 * n_dof * it
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     traffic = None
-    tr_path = os.path.join(ROOT, "profiles", f"pcg_fp64_{a.workload}_traffic.json")
-    if os.path.exists(tr_path):
+    tr_path = os.path.join(ROOT, "profiles", "pcg_fp64_142x142_traffic.md")
+    if a.workload == "c2" and os.path.exists(tr_path):
         with open(tr_path) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    traffic = int(line.split(":")[1])
     out = {
         "metric": "ms/solve to 1e-8 rel. residual (61k DOF, FP64 block-Jacobi PCG)",
         "value": ms_per_step, "unit": "ms/solve", "n_gpus": 1, "steps": a.steps,
@@ -108,7 +127,7 @@ def main():
         "true_rel_residual": chk.true_rel_residual, "converged": bool(chk.converged),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pcg (persistent FP64 PCG)",
+                     "kernel": "k_pcg_cg (persistent single-reduction FP64 PCG)",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if a.cpu_sample_iters > 0:
