@@ -23,6 +23,8 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per DOF per FP64 PCG iteration: read p, x, r and the
 # 3x3 block inverse (3 words per DOF), write p, x, r = 9 words of 8 bytes.
 BYTES_PER_DOF_ITER_FP64 = 72
+# FP32 vectors (6 words of 4 bytes) with the FP64 block inverse (3 x 8 bytes).
+BYTES_PER_DOF_ITER_MIXED = 48
 
 
 def parse():
@@ -30,9 +32,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2")
-    ap.add_argument("--cpu-sample-iters", type=int, default=1200,
-                    help="iterations of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"],
+                    help="c2: 142 x 142 FP64 (default); c3: 700 x 700 mixed precision")
+    ap.add_argument("--cpu-sample-iters", type=int, default=-1,
+                    help="iterations of the CPU baseline sample (0 = skip; default 1200, "
+                         "150 for c3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (profiler calibration)")
     return ap.parse_args()
@@ -50,9 +54,10 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 5}))
 
 
-def cpu_baseline(w, F, full_iters, sample_iters):
+def cpu_baseline(w, F, full_iters, sample_iters, basis="iterations it needs to reach 1e-8"):
     """Float32 restatement of solver.ts (oracle/solver_f32.js) in Node, one
-    thread, on the first `sample_iters` iterations of the same solve."""
+    thread, on the first `sample_iters` iterations of the same solve,
+    extrapolated to `full_iters` iterations."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as O
 
@@ -65,8 +70,7 @@ def cpu_baseline(w, F, full_iters, sample_iters):
         "value": ms_per_iter * full_iters, "unit": "ms/solve", "cores": 1, "kind": "port",
         "sample": (f"float32 solver.ts restatement in Node {st['node']} on "
                    f"{st['cpuModel']}: {st['iterations']} PCG iterations timed "
-                   f"({st['solveTimeMs']:.0f} ms), x {full_iters} iterations it needs to "
-                   f"reach 1e-8"),
+                   f"({st['solveTimeMs']:.0f} ms), extrapolated x {full_iters} {basis}"),
         "ms_per_iteration": ms_per_iter,
     }
 
@@ -85,11 +89,15 @@ def main():
     from platepcg import problem as P
 
     w = P.workload(a.workload)
+    mixed = a.workload == "c3"
+    if a.cpu_sample_iters < 0:
+        a.cpu_sample_iters = 150 if mixed else 1200
     ctx = Context(0)
     plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
     F = plan.pressure_load(w["q0"], w["patches"])
     plan.set_rhs(F)
-    opts = plan.options(tolerance=1e-8, max_iterations=100000,
+    prec = _lib.PRECISION_MIXED if mixed else _lib.PRECISION_FP64
+    opts = plan.options(tolerance=1e-8, max_iterations=400000, precision=prec,
                         flags=_lib.FLAG_NO_TRUE_RESIDUAL)
     for _ in range(a.warmup):
         plan.solve_resident(opts)
@@ -100,12 +108,13 @@ def main():
         dev_ms.append(st.device_ms)
         iters.append(st.iterations)
     wall = (time.perf_counter() - t0) * 1e3
-    chk = plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=100000))
+    chk = plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=400000, precision=prec))
     ms_per_step = wall / a.steps
     n_dof = plan.n_dof
     it = int(np.median(iters))
     kern_ms = float(np.mean(dev_ms))
-    bytes_per_launch = BYTES_PER_DOF_ITER_FP64 * n_dof * it
+    bpd = BYTES_PER_DOF_ITER_MIXED if mixed else BYTES_PER_DOF_ITER_FP64
+    bytes_per_launch = bpd * n_dof * it
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "pcg_fp64_142x142_traffic.md")
@@ -114,27 +123,48 @@ def main():
             for line in f:
                 if line.startswith("hbm_bytes_per_launch:"):
                     traffic = int(line.split(":")[1])
+    if mixed:
+        metric = "ms/solve to 1e-8 rel. residual (1.47M DOF, FP32-inner / FP64-refined PCG)"
+        kernel = "k_pcg32 (persistent FP32 PCG, MFMA element apply) + FP64 refinement"
+    else:
+        metric = "ms/solve to 1e-8 rel. residual (61k DOF, FP64 block-Jacobi PCG)"
+        kernel = "k_pcg_cg (persistent single-reduction FP64 PCG)"
     out = {
-        "metric": "ms/solve to 1e-8 rel. residual (61k DOF, FP64 block-Jacobi PCG)",
+        "metric": metric,
         "value": ms_per_step, "unit": "ms/solve", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": False,
-        "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64",
+        "data": "synthetic",
         "config": {"workload": f"{a.workload}: {w['mesh']['gridNx']}x{w['mesh']['gridNy']} Q4 "
-                   f"Mindlin plate, {n_dof} DOF, clamped, 10 kPa + Gaussian patch",
+                   f"Mindlin plate, t={w['t']} m, {n_dof} DOF, clamped, 10 kPa"
+                   + (" + Gaussian patch" if len(w["patches"]) else ""),
                    "dof": n_dof, "iterations": it, "precond": "block-jacobi-3x3",
-                   "precision": "fp64"},
+                   "precision": "mixed" if mixed else "fp64"},
         "device_ms_mean": kern_ms, "device_ms_min": float(np.min(dev_ms)),
         "true_rel_residual": chk.true_rel_residual, "converged": bool(chk.converged),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pcg_cg (persistent single-reduction FP64 PCG)",
+                     "kernel": kernel,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    if mixed:
+        out["sweeps"] = chk.sweeps
+        n = min(chk.sweeps, _lib.MAX_SWEEPS)
+        out["sweep_iterations"] = [chk.sweep_iterations[i] for i in range(n)]
+        out["sweep_true_rel_residual"] = [chk.sweep_residual[i] for i in range(n)]
+    basis = "iterations it needs to reach 1e-8"
+    full = it
+    if mixed:
+        st64 = plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=400000))
+        out["fp64_same_plate"] = {"device_ms": st64.device_ms, "iterations": st64.iterations,
+                                  "true_rel_residual": st64.true_rel_residual}
+        full = st64.iterations
+        basis = "iterations of FP64 block-Jacobi PCG to 1e-8 (extrapolation)"
+    elif a.workload == "c2":
+        full = int(np.load(os.path.join(ROOT, "tests", "golden", "c2_142.npz"))["it_js"])
     if a.cpu_sample_iters > 0:
-        g = np.load(os.path.join(ROOT, "tests", "golden", "c2_142.npz")) if a.workload == "c2" else None
-        full = int(g["it_js"]) if g is not None else it
         try:
-            out["cpu_baseline"] = cpu_baseline(w, F, full, a.cpu_sample_iters)
+            out["cpu_baseline"] = cpu_baseline(w, F, full, a.cpu_sample_iters, basis)
         except (OSError, subprocess.CalledProcessError) as e:
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -97,22 +97,30 @@ int launch_residual(pp_plan* pl, const double* dF, const double* dKx, double* ou
 }
 
 int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel) {
-  hipStream_t s = pl->ctx->stream;
-  DevBuf<double> part;
-  int rc = part.alloc(2 * kNormBlocks);
+  double a = 0, b = 0;
+  int rc = launch_norms(pl, dF, dKx, &a, &b);
   if (rc) return rc;
+  *rel = b > 0 ? a / b : 0.0;
+  return PP_OK;
+}
+
+int launch_norms(pp_plan* pl, const double* dF, const double* dKx, double* res, double* fnorm) {
+  hipStream_t s = pl->ctx->stream;
+  int rc;
+  if (pl->normp.n < 2 * kNormBlocks && (rc = pl->normp.alloc(2 * kNormBlocks))) return rc;
   hipLaunchKernelGGL(k_resid_parts, dim3(kNormBlocks), dim3(256), 0, s, (int64_t)pl->n_dof, dF, dKx,
-                     part.p);
+                     pl->normp.p);
   PP_HIP(hipGetLastError());
-  std::vector<double> h(2 * kNormBlocks);
-  PP_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * 8, hipMemcpyDeviceToHost, s));
+  double h[2 * kNormBlocks];
+  PP_HIP(hipMemcpyAsync(h, pl->normp.p, sizeof(h), hipMemcpyDeviceToHost, s));
   PP_HIP(hipStreamSynchronize(s));
   double a = 0, b = 0;
   for (int i = 0; i < kNormBlocks; i++) {
     a += h[i];
     b += h[kNormBlocks + i];
   }
-  *rel = b > 0 ? sqrt(a) / sqrt(b) : 0.0;
+  *res = sqrt(a);
+  *fnorm = sqrt(b);
   return PP_OK;
 }
 
@@ -320,9 +328,8 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     return fail(PP_ERR_INVALID, "unknown precision");
   }
   if (rc) return rc;
-  // the structured FP64 solver measures the true residual itself
-  if ((o->flags & PP_FLAG_NO_TRUE_RESIDUAL) || (structured && o->precision == PP_PRECISION_FP64))
-    return PP_OK;
+  // the structured solvers measure the true residual themselves
+  if ((o->flags & PP_FLAG_NO_TRUE_RESIDUAL) || structured) return PP_OK;
   double rel = 0;
   rc = structured ? struct_true_residual(pl, pl->x.p, &rel) : general_true_residual(pl, pl->x.p, &rel);
   if (rc) return rc;

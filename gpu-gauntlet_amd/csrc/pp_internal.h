@@ -65,9 +65,10 @@ struct DevStats {
   int iterations;
   int converged;
   int breakdown;
-  int pad;
+  int updated;  // mixed inner solve: stopped because |r| fell by the update factor
   double final_residual;
   double bnorm;
+  double rz_prev;  // mixed inner solve: r.z of the iteration before the exit
   // wall-clock ticks (100 MHz) spent by workgroup 0 in each part of an
   // iteration, summed over iterations: stage, elements, node sums,
   // all-reduce A, phase B, all-reduce B
@@ -117,10 +118,11 @@ struct pp_plan {
   pp::DevBuf<pp::GridSync> gsync;
   pp::DevBuf<pp::DevStats> dstats;
   pp::DevBuf<double> scal;            // coloured path device scalars
+  pp::DevBuf<double> normp;           // norm-reduction partials
 
   // mixed precision state (FP32 inner solve)
   pp::DevBuf<float> xs, rs, zs, ps0, ps1, Aps;
-  pp::DevBuf<double> xacc;
+  pp::DevBuf<float> kmix;             // shared element matrix: bending | shear, 16x16 each
 
   bool rhs_set = false;
   double rhs_norm = 0;
@@ -147,6 +149,8 @@ int general_pressure(pp_plan* pl, double q0, const double* patches, int np, floa
 
 // shared helpers (pp_api.hip)
 int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel);
+// |F - Kx| and |F| (FP64, fixed-order reduction)
+int launch_norms(pp_plan* pl, const double* dF, const double* dKx, double* res, double* fnorm);
 // out = F - Kx (out may alias Kx)
 int launch_residual(pp_plan* pl, const double* dF, const double* dKx, double* out);
 

@@ -36,7 +36,9 @@ export interface GPUSolverOptions {
   precision?: 'fp64' | 'mixed';
   preconditioner?: 'blockJacobi' | 'learned';
   path?: 'auto' | 'structured' | 'coloured';
+  /** mixed: a refinement sweep ends when the FP32 residual has fallen by this factor (default 1e-3) */
   innerTolerance?: number;
+  /** mixed: most FP64 refinement sweeps (default 200) */
   maxRefinements?: number;
 }
 

@@ -110,8 +110,8 @@ class PlateSolver {
     return this.lib.solve(this.handle, F, {
       tolerance: o.tolerance, maxIterations: o.maxIterations,
       precision: PRECISION[o.precision || 'fp64'], precond: PRECOND[o.preconditioner || 'blockJacobi'],
-      path: PATH[o.path || 'auto'], innerTolerance: o.innerTolerance || 1e-4,
-      maxRefinements: o.maxRefinements || 12,
+      path: PATH[o.path || 'auto'], innerTolerance: o.innerTolerance || 0,
+      maxRefinements: o.maxRefinements || 0,
     });
   }
 

@@ -294,8 +294,8 @@ napi_value Solve(napi_env env, napi_callback_info info) {
   w->opts.precision = (int32_t)prop(env, o, "precision", PP_PRECISION_FP64);
   w->opts.precond = (int32_t)prop(env, o, "precond", PP_PRECOND_BLOCK_JACOBI);
   w->opts.path = (int32_t)prop(env, o, "path", PP_PATH_AUTO);
-  w->opts.inner_tolerance = prop(env, o, "innerTolerance", 1e-4);
-  w->opts.max_refinements = (int32_t)prop(env, o, "maxRefinements", 12);
+  w->opts.inner_tolerance = prop(env, o, "innerTolerance", 0.0);
+  w->opts.max_refinements = (int32_t)prop(env, o, "maxRefinements", 0);
   w->opts.flags = (int32_t)prop(env, o, "flags", 0);
   w->x32.resize(w->n);
   w->x64.resize(w->n);

@@ -25,6 +25,8 @@ PATH_AUTO, PATH_STRUCTURED, PATH_COLOURED = 0, 1, 2
 MAX_SWEEPS = 32
 FLAG_NO_TRUE_RESIDUAL = 1
 FLAG_SMALL_TILES = 2
+FLAG_CLASSIC_PCG = 4
+FLAG_RELIABLE_UPDATE = 8
 
 
 class Mesh(C.Structure):

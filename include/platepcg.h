@@ -81,8 +81,9 @@ typedef struct {
   int32_t precision;     /* PP_PRECISION_* */
   int32_t precond;       /* PP_PRECOND_* */
   int32_t path;          /* PP_PATH_* */
-  double inner_tolerance;  /* mixed: relative tolerance of each inner solve */
-  int32_t max_refinements; /* mixed: outer sweeps */
+  double inner_tolerance;  /* mixed: a sweep ends when the FP32 residual has
+                              fallen by this factor (<= 0: 1e-3) */
+  int32_t max_refinements; /* mixed: most FP64 refinement sweeps (<= 0: 200) */
   int32_t flags;           /* PP_FLAG_* */
 } pp_solve_opts;
 
@@ -95,6 +96,10 @@ typedef struct {
 /* Structured path: the two-reduction PCG kernel (the recurrence order of
  * pcg.ts) instead of the default single-reduction kernel. */
 #define PP_FLAG_CLASSIC_PCG 4
+/* Mixed precision: keep the inner search direction across the FP64
+ * residual replacement ("reliable update") instead of restarting the inner
+ * PCG from p = z at every refinement sweep. */
+#define PP_FLAG_RELIABLE_UPDATE 8
 
 #define PP_MAX_SWEEPS 32
 
