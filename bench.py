@@ -109,6 +109,13 @@ def main():
         iters.append(st.iterations)
     wall = (time.perf_counter() - t0) * 1e3
     chk = plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=400000, precision=prec))
+    # PCIe-inclusive: host F in, solve, host x (FP32 and FP64) out
+    t1 = time.perf_counter()
+    n_e2e = 1 if mixed else 3
+    for _ in range(n_e2e):
+        plan.solve(F, tolerance=1e-8, max_iterations=400000, precision=prec,
+                   flags=_lib.FLAG_NO_TRUE_RESIDUAL)
+    e2e_ms = (time.perf_counter() - t1) * 1e3 / n_e2e
     ms_per_step = wall / a.steps
     n_dof = plan.n_dof
     it = int(np.median(iters))
@@ -141,6 +148,7 @@ def main():
                    "dof": n_dof, "iterations": it, "precond": "block-jacobi-3x3",
                    "precision": "mixed" if mixed else "fp64"},
         "device_ms_mean": kern_ms, "device_ms_min": float(np.min(dev_ms)),
+        "pcie_inclusive_ms": e2e_ms,
         "true_rel_residual": chk.true_rel_residual, "converged": bool(chk.converged),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
