@@ -5,7 +5,7 @@
 #include <algorithm>
 #include <string>
 
-#include "pp_device.h"
+#include "pp_cnn.h"
 
 namespace pp {
 
@@ -124,6 +124,27 @@ int launch_norms(pp_plan* pl, const double* dF, const double* dKx, double* res, 
   return PP_OK;
 }
 
+// A context is freed when its owner and every plan made from it have let go.
+void ctx_release(pp_ctx* c) {
+  if (--c->refs > 0) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// A plan is freed when its owner and every network made on it have let go.
+void plan_release(pp_plan* pl) {
+  if (--pl->refs > 0) return;
+  pp_ctx* c = pl->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete pl;
+  ctx_release(c);
+}
+
 }  // namespace pp
 
 using namespace pp;
@@ -159,17 +180,6 @@ int pp_ctx_create(int device, pp_ctx** out) {
   c->cu_count = prop.multiProcessorCount;
   *out = c;
   return PP_OK;
-}
-
-// A context is freed when its owner and every plan made from it have let go.
-static void ctx_release(pp_ctx* c) {
-  if (--c->refs > 0) return;
-  (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
 }
 
 int pp_ctx_destroy(pp_ctx* c) {
@@ -271,11 +281,7 @@ int pp_plan_create(pp_ctx* ctx, const pp_mesh* m, const pp_material* mat,
 
 int pp_plan_destroy(pp_plan* pl) {
   if (!pl) return PP_OK;
-  pp_ctx* c = pl->ctx;
-  (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  delete pl;
-  ctx_release(c);
+  plan_release(pl);
   return PP_OK;
 }
 
@@ -312,14 +318,20 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   if (!pl || !o || !st) return fail(PP_ERR_INVALID, "NULL argument");
   if (!pl->rhs_set) return fail(PP_ERR_INVALID, "no right-hand side set");
   if (!(o->tolerance > 0) || o->max_iterations < 0) return fail(PP_ERR_INVALID, "bad tolerance/max_iterations");
-  if (o->precond != PP_PRECOND_BLOCK_JACOBI)
-    return fail(PP_ERR_UNSUPPORTED, "preconditioner not available in this build");
+  if (o->precond != PP_PRECOND_BLOCK_JACOBI && o->precond != PP_PRECOND_LEARNED)
+    return fail(PP_ERR_INVALID, "unknown preconditioner");
   memset(st, 0, sizeof(*st));
   PP_HIP(hipSetDevice(pl->ctx->device));
   bool structured = false;
   int rc = pick_path(pl, o->path, &structured);
   if (rc) return rc;
-  if (o->precision == PP_PRECISION_FP64) {
+  if (o->precond == PP_PRECOND_LEARNED) {
+    if (!structured) return fail(PP_ERR_UNSUPPORTED, "the learned preconditioner needs the structured path");
+    if (!pl->learned) return fail(PP_ERR_INVALID, "no network attached (pp_plan_set_learned)");
+    if (o->precision != PP_PRECISION_FP64)
+      return fail(PP_ERR_UNSUPPORTED, "the learned preconditioner runs with FP64 flexible PCG");
+    rc = learned_solve(pl, o, st);
+  } else if (o->precision == PP_PRECISION_FP64) {
     rc = structured ? struct_solve_fp64(pl, o, st) : general_solve_fp64(pl, o, st);
   } else if (o->precision == PP_PRECISION_MIXED) {
     if (!structured) return fail(PP_ERR_UNSUPPORTED, "mixed precision needs the structured path");

@@ -1,0 +1,72 @@
+// Learned preconditioner: network state shared by pp_cnn.hip (kernels,
+// training) and pp_fcg.hip (flexible PCG).
+#pragma once
+
+#include <vector>
+
+#include "pp_grid.h"
+
+namespace pp {
+
+// channels-last BF16 tensor view: (pixel, c) at p[pixel * cs + c0 + c]
+struct TV {
+  uint16_t* p;
+  int cs, c0;
+};
+
+struct Layer {
+  int cin, cout, relu, dil;
+  int coutP, coutA, cinP, kp, kpT;  // padded sizes (16 / 4 / 16 / K of forward / K of transpose)
+  int64_t w_off, b_off;             // FP32 master weights [cout][9][cin], bias [cout]
+  int64_t wf_off, wt_off;           // packed BF16 forward / backward-data weights
+};
+
+struct Cnn {
+  pp_plan* pl = nullptr;
+  hipStream_t stream = nullptr;
+  int cu = 0, nx = 0, ny = 0;
+  int H[3] = {0, 0, 0}, W[3] = {0, 0, 0};
+  double sw = 1.0, st = 0.3;  // input scales of the w and rotation channels
+  double omega = 0.5;         // block-Jacobi smoothing weight around the network
+  int cat0C = 0, cat1C = 0;
+  std::vector<Layer> layers;
+  int64_t n_params = 0;
+  int adam_t = 0;
+  double last_grad_norm = 0.0;
+  int linear = 1;       // no ReLU, no biases
+  int line_search = 1;  // energy-optimal step on the network's correction
+
+  DevBuf<float> params, grads, adam_m, adam_v;
+  DevBuf<uint16_t> wf, wt;
+  DevBuf<uint16_t> xin, a0, cat0, r1, a1, cat1, r2, a2, bb, c1b, d1, c0, c0b;
+  DevBuf<uint16_t> g_c0b, g_c0, g_cat0, g_d1, g_c1, g_cat1, g_bb, g_a2, g_r2, g_a1, g_r1, g_a0,
+      g_out;
+  DevBuf<float> out32, wpart, bpart, logt;
+  DevBuf<double> zbj, mout, dvec, kvec, evec, rvec, scal, dotp, bumps, z0v, z1v, r1v;
+
+  const TV* mk(const TV& v) const { return linear ? nullptr : &v; }  // ReLU mask or none
+  int setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed);
+  int pack_weights();
+  int apply(const double* r, double* out);         // network alone, device vectors
+  int precondition(const double* r, double* out, bool line_search);  // smoothing + network + smoothing
+  int estimate_omega();
+  int backward();
+  int loss_grad(const double* e, double* loss_dev);  // e on the device
+  int gen_error(uint64_t seed, double* e);
+  int adam_step(float lr, float max_norm);
+  int conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int out32_cs, int lev,
+               bool backward, const TV* mask = nullptr, int mask_from = 0);
+  int conv_wgrad(const Layer& L, const TV& in, const TV& dout, int lev);
+  int resample(bool restrict_op, const TV& in, int lin, const TV& out, int lout, int C,
+               float scale, const TV* mask, bool accumulate);
+  ~Cnn();
+};
+
+// flexible PCG with the network as preconditioner (pp_fcg.hip)
+int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+
+}  // namespace pp
+
+struct pp_cnn {
+  pp::Cnn net;
+};

@@ -1,0 +1,1081 @@
+// Learned preconditioner: a 3-level U-net on the nodal grid (BF16 MFMA,
+// FP32 accumulate), trained on the device by this file's own forward and
+// backward kernels, and applied inside flexible PCG.
+//
+// Tensors are channels-last BF16: element (pixel, c) of a view at
+// p[pixel * cs + c0 + c], pixel = y * W + x on the (padded) node grid.
+// Channel strides and offsets are multiples of 4 so a lane's four channels
+// are one 8-byte load.  Level 0 is the node grid padded to 4k+1 on each side;
+// levels 1 and 2 are the vertex-centred coarsenings (H_{l+1} = (H_l + 1) / 2)
+// with full-weighting restriction R and bilinear prolongation P = 4 R^T.
+//
+//   x_in(4) -e0a-> a0(16) -e0b-> s0(16) -R-> r1(16) -e1a-> a1(32) -e1b-> s1(32)
+//   -R-> r2(32) -ba-> a2(64) -bb-> b(64) -P-> [P(b) | s1](96) -d1a-> c1(32)
+//   -d1b-> d1(32) -P-> [P(d1) | s0](48) -d0a-> c0(16) -d0b-> c0b(16) -head-> o(3)
+//
+// every conv 3x3; with ReLU and biases (all but the head) or, by default,
+// without either, so the network is a fixed linear operator.  Inputs: the
+// block-Jacobi preconditioned residual z = M^-1 r (3 channels, scaled per
+// channel and by its RMS s) and log(t / t0) times the first of them.  Output: M(r) = z + s * o / cin_scale
+// (the network learns the correction beyond block Jacobi).
+//
+// Training pairs: e from a seed (a windowed sum of Gaussian bumps in w with
+// Kirchhoff-consistent rotations, zero on constrained DOFs), r = K e with the
+// solver's own FP64 operator.  Loss: the energy norm of the error,
+// (M(r) - e)^T K (M(r) - e) / e^T K e, whose gradient in M(r) is one more
+// K apply.  Adam on FP32 master weights.
+//
+// Convolution as implicit GEMM on v_mfma_f32_16x16x16bf16_1k: M = 16 pixels,
+// N = 16 output channels, K = (tap, input channel) in chunks of 16.  The
+// backward-data pass is the same kernel with the flipped, transposed weights;
+// the weight gradient is a GEMM over pixels with LDS-staged transposed tiles
+// and deterministic partial sums.
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "pp_cnn.h"
+
+namespace pp {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+int nblk(int64_t n, int bs = 256) { return (int)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------ convolution
+
+struct ConvArgs {
+  const uint16_t* in;
+  int in_cs, in_c0, cin;  // cin % 4 == 0
+  uint16_t* out;          // bf16 output (or out32)
+  int out_cs, out_c0, cout;
+  float* out32;
+  int out32_cs;
+  const uint16_t* w;  // [NB*16][kp] bf16, k = tap * cin + ci
+  int kp;             // multiple of 16, >= 9 cin
+  const float* bias;  // [cout] or null
+  int H, W, dil, relu;
+  const uint16_t* mask;  // out[c] *= (mask[pixel][c] > 0) for c >= mask_from
+  int mask_cs, mask_c0, mask_from;
+};
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
+  extern __shared__ uint16_t w_s[];
+  const int nw = NB * 16 * A.kp;
+  for (int i = threadIdx.x * 4; i < nw; i += 1024) *(uint2*)&w_s[i] = *(const uint2*)&A.w[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int npix = A.H * A.W, nmb = (npix + 15) / 16;
+  const int m = lane & 15, kq = lane >> 4;
+  const int k9 = 9 * A.cin;
+  for (int mb = blockIdx.x * 4 + wave; mb < nmb; mb += gridDim.x * 4) {
+    const int pix = mb * 16 + m;
+    const bool pv = pix < npix;
+    const int px = pv ? pix % A.W : 0, py = pv ? pix / A.W : 0;
+    f4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < A.kp; k0 += 16) {
+      const int kb = k0 + 4 * kq;
+      s4 a = {0, 0, 0, 0};
+      if (pv && kb < k9) {
+        const int tap = kb / A.cin, ci = kb - tap * A.cin;
+        const int qx = px + (tap % 3 - 1) * A.dil, qy = py + (tap / 3 - 1) * A.dil;
+        if (qx >= 0 && qx < A.W && qy >= 0 && qy < A.H)
+          a = *(const s4*)&A.in[((size_t)qy * A.W + qx) * A.in_cs + A.in_c0 + ci];
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        const s4 b = *(const s4*)&w_s[(nb * 16 + m) * A.kp + kb];
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+      }
+    }
+    // D[row = pixel 4 kq + i][col = channel m] of each N block
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+      const int co = nb * 16 + m;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int op = mb * 16 + 4 * kq + i;
+        if (op >= npix || co >= A.cout) continue;
+        float v = acc[nb][i] + (A.bias ? A.bias[co] : 0.0f);
+        if (A.relu) v = fmaxf(v, 0.0f);
+        if (A.mask && co >= A.mask_from &&
+            !(bf2f(A.mask[(size_t)op * A.mask_cs + A.mask_c0 + co]) > 0.0f))
+          v = 0.0f;
+        if (A.out32) A.out32[(size_t)op * A.out32_cs + co] = v;
+        else A.out[(size_t)op * A.out_cs + A.out_c0 + co] = f2bf(v);
+      }
+    }
+  }
+}
+
+// Weight gradient dW[co][tap*cin + ci] = sum_p dout[p][co] in[p + off(tap)][ci]:
+// a GEMM over pixels (K), M = output channels, N = (tap, ci).  One row
+// segment of 64 pixels at a time is staged transposed in LDS (dout as
+// [co][64], the input rows y - dil, y, y + dil as [ci][3][64 + 2 dil]); each
+// wave owns 4 (M block, N block) tiles; partial sums per row chunk.
+struct WgArgs {
+  const uint16_t* in;
+  int in_cs, in_c0, cin;
+  const uint16_t* dout;
+  int dout_cs, dout_c0, cout;
+  int H, W, dil;
+  int kpad, coutP, rows;
+  float* part;   // [nchunks][coutP][kpad]
+  float* bpart;  // [nchunks][coutP] bias partials (written by blockIdx.y == 0)
+};
+
+__global__ __launch_bounds__(256) void k_wgrad(WgArgs A) {
+  extern __shared__ uint16_t sm[];
+  const int TW = 64 + 2 * A.dil;
+  uint16_t* in_s = sm;                    // [cin][3][TW]
+  uint16_t* do_s = sm + A.cin * 3 * TW;   // [coutP][64]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nkb = A.kpad / 16, nmb = A.coutP / 16, npairs = nmb * nkb;
+  const int k9 = 9 * A.cin;
+  int pm[4], pn[4];
+  bool pv[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int q = blockIdx.y * 16 + wave * 4 + j;
+    pv[j] = q < npairs;
+    pm[j] = pv[j] ? q / nkb : 0;
+    pn[j] = pv[j] ? q % nkb : 0;
+  }
+  f4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) acc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bsum = 0.0f;  // bias partial of channel threadIdx.x (y == 0 only)
+  const int y0 = blockIdx.x * A.rows, y1 = min(A.H, y0 + A.rows);
+  for (int y = y0; y < y1; y++) {
+    for (int x0 = 0; x0 < A.W; x0 += 64) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < A.coutP * 64; i += 256) {
+        const int co = i % A.coutP, px = i / A.coutP;
+        uint16_t v = 0;
+        if (co < A.cout && x0 + px < A.W)
+          v = A.dout[((size_t)y * A.W + x0 + px) * A.dout_cs + A.dout_c0 + co];
+        do_s[co * 64 + px] = v;
+      }
+      for (int i = threadIdx.x; i < A.cin * 3 * TW; i += 256) {
+        const int ci = i % A.cin, rest = i / A.cin;
+        const int xx = rest % TW, r = rest / TW;
+        const int gx = x0 - A.dil + xx, gy = y + (r - 1) * A.dil;
+        uint16_t v = 0;
+        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+          v = A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + ci];
+        in_s[(ci * 3 + r) * TW + xx] = v;
+      }
+      __syncthreads();
+      if (blockIdx.y == 0 && threadIdx.x < A.cout)
+        for (int px = 0; px < 64; px++) bsum += bf2f(do_s[threadIdx.x * 64 + px]);
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) {
+        const int pxb = kb * 16 + 4 * (lane >> 4);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (!pv[j]) continue;  // uniform over the wave
+          const int co = pm[j] * 16 + (lane & 15);
+          const s4 a = *(const s4*)&do_s[co * 64 + pxb];
+          const int n = pn[j] * 16 + (lane & 15);
+          s4 b = {0, 0, 0, 0};
+          if (n < k9) {
+            const int tap = n / A.cin, ci = n - tap * A.cin;
+            const int base = (ci * 3 + tap / 3) * TW + pxb + (tap % 3) * A.dil;
+            b[0] = (short)in_s[base];
+            b[1] = (short)in_s[base + 1];
+            b[2] = (short)in_s[base + 2];
+            b[3] = (short)in_s[base + 3];
+          }
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float* P = A.part + (size_t)blockIdx.x * A.coutP * A.kpad;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (!pv[j]) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int co = pm[j] * 16 + 4 * (lane >> 4) + i;
+      const int n = pn[j] * 16 + (lane & 15);
+      P[(size_t)co * A.kpad + n] = acc[j][i];
+    }
+  }
+  if (blockIdx.y == 0 && threadIdx.x < A.coutP)
+    A.bpart[(size_t)blockIdx.x * A.coutP + threadIdx.x] = threadIdx.x < A.cout ? bsum : 0.0f;
+}
+
+// grad[co][n] (n < 9 cin) = sum over chunks, fixed order; bias likewise.
+__global__ void k_wgrad_reduce(const float* part, const float* bpart, int nchunks, int coutP,
+                               int kpad, int cout, int k9, float* gw, float* gb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cout * k9) {
+    const int co = i / k9, n = i % k9;
+    float s = 0.0f;
+    for (int c = 0; c < nchunks; c++) s += part[((size_t)c * coutP + co) * kpad + n];
+    gw[i] = s;
+  }
+  if (i < cout) {
+    float s = 0.0f;
+    for (int c = 0; c < nchunks; c++) s += bpart[(size_t)c * coutP + i];
+    gb[i] = s;
+  }
+}
+
+// Pack FP32 master weights W[co][tap][ci] into the forward layout
+// wf[co][tap*cin + ci] (rows to coutP, columns to kp, zero padded) and the
+// backward-data layout wt[ci][tap'*coutA + co] = W[co][8 - tap'][ci].
+__global__ void k_pack_w(const float* W, int cout, int cin, int coutP, int kp, int coutA,
+                         int cinP, int kpT, uint16_t* wf, uint16_t* wt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < coutP * kp) {
+    const int co = i / kp, k = i % kp;
+    float v = 0.0f;
+    if (co < cout && k < 9 * cin) v = W[(size_t)co * 9 * cin + k];
+    wf[i] = f2bf(v);
+  }
+  if (i < cinP * kpT) {
+    const int ci = i / kpT, k = i % kpT;
+    float v = 0.0f;
+    if (ci < cin && k < 9 * coutA) {
+      const int tp = k / coutA, co = k % coutA;
+      if (co < cout) v = W[((size_t)co * 9 + (8 - tp)) * cin + ci];
+    }
+    wt[i] = f2bf(v);
+  }
+}
+
+// ------------------------------------------------------- grid transfer
+
+// out[I,J][c] = scale * sum w(a) w(b) in[2I+a, 2J+b][c] / 16 (w = 1 2 1),
+// then *= (mask > 0) and += previous out when accumulate.
+__global__ void k_restrict(const uint16_t* in, int in_cs, int in_c0, int Hf, int Wf,
+                           uint16_t* out, int out_cs, int out_c0, int Hc, int Wc, int C,
+                           float scale, const uint16_t* mask, int mask_cs, int mask_c0,
+                           int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)Hc * Wc * C) return;
+  const int c = (int)(i % C);
+  const int64_t pix = i / C;
+  const int X = (int)(pix % Wc), Y = (int)(pix / Wc);
+  float s = 0.0f;
+  for (int b = -1; b <= 1; b++) {
+    const int fy = 2 * Y + b;
+    if (fy < 0 || fy >= Hf) continue;
+    const float wy = b == 0 ? 2.0f : 1.0f;
+    for (int a = -1; a <= 1; a++) {
+      const int fx = 2 * X + a;
+      if (fx < 0 || fx >= Wf) continue;
+      const float wx = a == 0 ? 2.0f : 1.0f;
+      s += wx * wy * bf2f(in[((size_t)fy * Wf + fx) * in_cs + in_c0 + c]);
+    }
+  }
+  float v = scale * s * (1.0f / 16.0f);
+  uint16_t* o = &out[(size_t)pix * out_cs + out_c0 + c];
+  if (accumulate) v += bf2f(*o);
+  if (mask && !(bf2f(mask[(size_t)pix * mask_cs + mask_c0 + c]) > 0.0f)) v = 0.0f;
+  *o = f2bf(v);
+}
+
+// out[x,y][c] = scale * bilinear(in) (vertex-centred, Hf = 2 Hc - 1).
+__global__ void k_prolong(const uint16_t* in, int in_cs, int in_c0, int Hc, int Wc, uint16_t* out,
+                          int out_cs, int out_c0, int Hf, int Wf, int C, float scale,
+                          const uint16_t* mask, int mask_cs, int mask_c0, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)Hf * Wf * C) return;
+  const int c = (int)(i % C);
+  const int64_t pix = i / C;
+  const int x = (int)(pix % Wf), y = (int)(pix / Wf);
+  const int x0 = x >> 1, y0 = y >> 1;
+  const int x1 = (x & 1) ? x0 + 1 : x0, y1 = (y & 1) ? y0 + 1 : y0;
+  const float wx = (x & 1) ? 0.5f : 1.0f, wy = (y & 1) ? 0.5f : 1.0f;
+  auto at = [&](int X, int Y) { return bf2f(in[((size_t)Y * Wc + X) * in_cs + in_c0 + c]); };
+  float s = wx * wy * at(x0, y0);
+  if (x & 1) s += wx * wy * at(x1, y0);
+  if (y & 1) s += wx * wy * at(x0, y1);
+  if ((x & 1) && (y & 1)) s += wx * wy * at(x1, y1);
+  float v = scale * s;
+  uint16_t* o = &out[(size_t)pix * out_cs + out_c0 + c];
+  if (accumulate) v += bf2f(*o);
+  if (mask && !(bf2f(mask[(size_t)pix * mask_cs + mask_c0 + c]) > 0.0f)) v = 0.0f;
+  *o = f2bf(v);
+}
+
+// ------------------------------------------------------ input / output
+
+// z = M^-1 r (FP64, per node), partial sums of (scale_c z_c)^2 per block.
+__global__ void k_bj_norm(int64_t nn, const double* minv, const double* r, double* z, double sw,
+                          double st, double* part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nn;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    double rv[3] = {r[3 * n], r[3 * n + 1], r[3 * n + 2]}, zv[3];
+    sym3_apply(minv + 6 * n, rv, zv);
+    z[3 * n] = zv[0];
+    z[3 * n + 1] = zv[1];
+    z[3 * n + 2] = zv[2];
+    acc += sw * sw * zv[0] * zv[0] + st * st * (zv[1] * zv[1] + zv[2] * zv[2]);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Fixed-order sum of np (<= 256) partials at stride `stride` by one
+// 256-thread block; thread 0 gets the result.
+__device__ double block_total(const double* part, int np, int stride) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  sh[t] = t < np ? part[(size_t)t * stride] : 0.0;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) sh[t] += sh[t + k];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// scal[0] = s = sqrt(sum / count) (1 when zero); fixed order.
+__global__ void k_scale(const double* part, int np, double count, double* scal) {
+  const double a = block_total(part, np, 1);
+  if (threadIdx.x == 0) {
+    const double s = sqrt(a / count);
+    scal[0] = s > 0.0 ? s : 1.0;
+  }
+}
+
+// x_in[pixel] = (sw z_w, st z_thx, st z_thy) / s and log(t / t0) times the
+// first channel (so the input, and a network without activation and biases,
+// stay linear in r); pixels outside the plate are zero.
+__global__ void k_pack_in(int nx, int ny, int Wp, int Hp, const double* z, const double* scal,
+                          double sw, double st, const float* logt, uint16_t* xin) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)Wp * Hp) return;
+  const int x = (int)(p % Wp), y = (int)(p / Wp);
+  uint16_t v[4] = {0, 0, 0, 0};
+  if (x <= nx && y <= ny) {
+    const int64_t n = (int64_t)y * (nx + 1) + x;
+    const double is = 1.0 / scal[0];
+    const float x0 = (float)(sw * z[3 * n] * is);
+    v[0] = f2bf(x0);
+    v[1] = f2bf((float)(st * z[3 * n + 1] * is));
+    v[2] = f2bf((float)(st * z[3 * n + 2] * is));
+    v[3] = f2bf(logt[n] * x0);
+  }
+  *(uint2*)&xin[4 * p] = make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
+}
+
+// M(r) = z + s * o / scale_c on free DOFs; z on constrained DOFs.
+__global__ void k_unpack_out(int nx, int ny, int Wp, const double* z, const float* o,
+                             const double* scal, double sw, double st, const uint8_t* cmask,
+                             double* out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
+  const int x = (int)(n % (nx + 1)), y = (int)(n / (nx + 1));
+  const int64_t p = (int64_t)y * Wp + x;
+  const unsigned fix = cmask[n];
+  const double s = scal[0];
+  const double inv[3] = {1.0 / sw, 1.0 / st, 1.0 / st};
+  for (int d = 0; d < 3; d++) {
+    double v = z[3 * n + d];
+    if (!((fix >> d) & 1u)) v += s * inv[d] * (double)o[4 * p + d];
+    out[3 * n + d] = v;
+  }
+}
+
+// dL/do[pixel][c] = factor * s * g[3n + c] / scale_c on free DOFs (bf16, 4 channels).
+__global__ void k_pack_dout(int nx, int ny, int Wp, int Hp, const double* g, const double* scal,
+                            const double* fac, double sw, double st, const uint8_t* cmask,
+                            uint16_t* dout) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)Wp * Hp) return;
+  const int x = (int)(p % Wp), y = (int)(p / Wp);
+  uint16_t v[4] = {0, 0, 0, 0};
+  if (x <= nx && y <= ny) {
+    const int64_t n = (int64_t)y * (nx + 1) + x;
+    const unsigned fix = cmask[n];
+    const double f = fac[0] * scal[0];
+    const double inv[3] = {1.0 / sw, 1.0 / st, 1.0 / st};
+    for (int d = 0; d < 3; d++)
+      if (!((fix >> d) & 1u)) v[d] = f2bf((float)(f * inv[d] * g[3 * n + d]));
+  }
+  *(uint2*)&dout[4 * p] = make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
+}
+
+// Two dot products a.b and c.d, per-block partials (fixed order).
+__global__ void k_dot2(int64_t n, const double* a, const double* b, const double* c,
+                       const double* d, double* part) {
+  __shared__ double r0[256], r1[256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    s0 += a[i] * b[i];
+    if (c) s1 += c[i] * d[i];
+  }
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0[0];
+    part[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
+// out[0] = sum a.b, out[1] = sum c.d over the partials.
+__global__ void k_dot2_finish(const double* part, int np, double* out) {
+  const double a = block_total(part, np, 2);
+  __syncthreads();
+  const double b = block_total(part + 1, np, 2);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+// d = m - e
+__global__ void k_sub(int64_t n, const double* m, const double* e, double* d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = m[i] - e[i];
+}
+
+// loss = dKd / eKe, gradient factor 2 / eKe: io = {dKd, eKe} -> out
+__global__ void k_loss(const double* io, double* loss, double* fac) {
+  if (threadIdx.x == 0) {
+    const double eke = io[1] > 0.0 ? io[1] : 1.0;
+    loss[0] = io[0] / eke;
+    fac[0] = 2.0 / eke;
+  }
+}
+
+// Adam on the flat FP32 parameters (bias-corrected).
+__global__ void k_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr, float b1,
+                       float b2, float eps, float c1, float c2, float clip) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] * clip;
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// sum of squares of the flat gradient, per-block partials
+__global__ void k_sumsq(int64_t n, const float* g, double* part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)g[i] * g[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Training error field of one seed: w = phi(x, y) sum_k a_k exp(-|x - c_k|^2 / (2 s_k^2)),
+// phi = sin^2(pi u) sin^2(pi v) on the plate's unit square (u, v); thx = dw/dy,
+// thy = -dw/dx (the rigid-mode signs of the Q4 element); zero on constrained DOFs.
+__global__ void k_gen_error(KGrid G, const double* bumps, int nb, double* e) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)(G.nx + 1) * (G.ny + 1)) return;
+  const int i = (int)(n % (G.nx + 1)), j = (int)(n / (G.nx + 1));
+  const double x = G.xc[i], y = G.yc[j];
+  const double x0 = G.xc[0], y0 = G.yc[0];
+  const double Lx = G.xc[G.nx] - x0, Ly = G.yc[G.ny] - y0;
+  double f = 0.0, fx = 0.0, fy = 0.0;
+  for (int k = 0; k < nb; k++) {
+    const double a = bumps[4 * k], cx = bumps[4 * k + 1], cy = bumps[4 * k + 2],
+                 s = bumps[4 * k + 3];
+    const double dx = x - cx, dy = y - cy;
+    const double g = a * exp(-(dx * dx + dy * dy) / (2.0 * s * s));
+    f += g;
+    fx -= g * dx / (s * s);
+    fy -= g * dy / (s * s);
+  }
+  const double pi = 3.141592653589793;
+  const double su = sin(pi * (x - x0) / Lx), sv = sin(pi * (y - y0) / Ly);
+  const double cu = cos(pi * (x - x0) / Lx), cv = cos(pi * (y - y0) / Ly);
+  const double ph = su * su * sv * sv;
+  const double phx = 2.0 * su * cu * (pi / Lx) * sv * sv;
+  const double phy = su * su * 2.0 * sv * cv * (pi / Ly);
+  const double w = ph * f, wx = phx * f + ph * fx, wy = phy * f + ph * fy;
+  const unsigned fix = G.cmask[n];
+  e[3 * n] = (fix & 1u) ? 0.0 : w;
+  e[3 * n + 1] = (fix & 2u) ? 0.0 : wy;
+  e[3 * n + 2] = (fix & 4u) ? 0.0 : -wx;
+}
+
+// node log(t / t0): mean of the adjacent elements' log thickness
+__global__ void k_node_logt(KGrid G, float* logt) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)(G.nx + 1) * (G.ny + 1)) return;
+  const int i = (int)(n % (G.nx + 1)), j = (int)(n / (G.nx + 1));
+  double s = 0.0;
+  int c = 0;
+  for (int b = j - 1; b <= j; b++)
+    for (int a = i - 1; a <= i; a++)
+      if (a >= 0 && a < G.nx && b >= 0 && b < G.ny) {
+        const double t = G.elem_t ? (double)G.elem_t[(size_t)b * G.nx + a] : G.t0;
+        s += log(t / G.t0);
+        c++;
+      }
+  logt[n] = (float)(c ? s / c : 0.0);
+}
+
+// ---------------------------------------------------------- host utils
+
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double uni(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+double gauss(uint64_t& s) {
+  const double u1 = std::max(uni(s), 1e-300), u2 = uni(s);
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+}
+
+// out = base + omega B in (B the 3x3 nodal block-Jacobi inverse; base may be null)
+__global__ void k_bjw(int64_t nn, const double* minv, const double* in, double omega,
+                      const double* base, double* out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double rv[3] = {in[3 * n], in[3 * n + 1], in[3 * n + 2]}, zv[3];
+  sym3_apply(minv + 6 * n, rv, zv);
+  for (int d = 0; d < 3; d++) out[3 * n + d] = (base ? base[3 * n + d] : 0.0) + omega * zv[d];
+}
+
+// out = a + alpha b
+__global__ void k_lin(int64_t n, const double* a, double alpha, const double* b, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + alpha * b[i];
+}
+
+// out = a + sign * g[0] * b (g on the device)
+__global__ void k_axpy_dev(int64_t n, const double* a, const double* g, double sign,
+                           const double* b, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + sign * g[0] * b[i];
+}
+
+// g = u.r1 / u.K u over the partials (0 when u.K u <= 0): the energy-optimal
+// length of the network's correction
+__global__ void k_gamma(const double* part, int np, double* g) {
+  const double a = block_total(part, np, 2);
+  __syncthreads();
+  const double b = block_total(part + 1, np, 2);
+  if (threadIdx.x == 0) g[0] = b > 0.0 ? a / b : 0.0;
+}
+
+// v /= |v| (host-provided norm), for the power iteration
+__global__ void k_scale_vec(int64_t n, double* v, double f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= f;
+}
+
+constexpr int kDotBlocks = 256;
+constexpr int kRowsPerChunk = 8;
+
+}  // namespace
+
+// ------------------------------------------------------------- network
+
+int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int out32_cs,
+                  int lev, bool backward, const TV* mask, int mask_from) {
+  ConvArgs A;
+  memset(&A, 0, sizeof(A));
+  const int cin = backward ? L.coutA : L.cin;
+  const int cout = backward ? L.cin : L.cout;
+  const int nb = backward ? L.cinP / 16 : L.coutP / 16;
+  A.in = in.p; A.in_cs = in.cs; A.in_c0 = in.c0; A.cin = cin;
+  A.out = out.p; A.out_cs = out.cs; A.out_c0 = out.c0; A.cout = cout;
+  A.out32 = out32; A.out32_cs = out32_cs;
+  A.w = backward ? wt.p + L.wt_off : wf.p + L.wf_off;
+  A.kp = backward ? L.kpT : L.kp;
+  A.bias = (backward || linear) ? nullptr : params.p + L.b_off;
+  A.H = H[lev]; A.W = W[lev]; A.dil = L.dil;
+  A.relu = backward ? 0 : L.relu;
+  if (mask) {
+    A.mask = mask->p; A.mask_cs = mask->cs; A.mask_c0 = mask->c0; A.mask_from = mask_from;
+  }
+  const int npix = H[lev] * W[lev], nmb = (npix + 15) / 16;
+  const int grid = std::max(1, std::min((nmb + 15) / 16, 4 * cu));
+  const size_t lds = (size_t)nb * 16 * A.kp * 2;
+  hipStream_t s = stream;
+  switch (nb) {
+    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(grid), dim3(256), lds, s, A); break;
+    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(grid), dim3(256), lds, s, A); break;
+    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(grid), dim3(256), lds, s, A); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(grid), dim3(256), lds, s, A); break;
+    case 6: hipLaunchKernelGGL(k_conv<6>, dim3(grid), dim3(256), lds, s, A); break;
+    default: return fail(PP_ERR_INVALID, "conv: unsupported channel count");
+  }
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int Cnn::conv_wgrad(const Layer& L, const TV& in, const TV& dout, int lev) {
+  WgArgs A;
+  A.in = in.p; A.in_cs = in.cs; A.in_c0 = in.c0; A.cin = L.cin;
+  A.dout = dout.p; A.dout_cs = dout.cs; A.dout_c0 = dout.c0; A.cout = L.cout;
+  A.H = H[lev]; A.W = W[lev]; A.dil = L.dil;
+  A.kpad = L.kp; A.coutP = L.coutP; A.rows = kRowsPerChunk;
+  A.part = wpart.p; A.bpart = bpart.p;
+  const int nchunks = (H[lev] + kRowsPerChunk - 1) / kRowsPerChunk;
+  const int npairs = (L.coutP / 16) * (L.kp / 16);
+  const size_t lds = ((size_t)L.cin * 3 * (64 + 2 * L.dil) + (size_t)L.coutP * 64) * 2;
+  hipLaunchKernelGGL(k_wgrad, dim3(nchunks, (npairs + 15) / 16), dim3(256), lds, stream, A);
+  PP_HIP(hipGetLastError());
+  const int k9 = 9 * L.cin;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblk((int64_t)L.cout * k9)), dim3(256), 0, stream,
+                     (const float*)wpart.p, (const float*)bpart.p, nchunks, L.coutP, L.kp, L.cout,
+                     k9, grads.p + L.w_off, grads.p + L.b_off);
+  PP_HIP(hipGetLastError());
+  if (linear) PP_HIP(hipMemsetAsync(grads.p + L.b_off, 0, L.cout * 4, stream));
+  return PP_OK;
+}
+
+int Cnn::resample(bool restrict_op, const TV& in, int lin, const TV& out, int lout, int C,
+                  float scale, const TV* mask, bool accumulate) {
+  const int64_t n = (int64_t)H[lout] * W[lout] * C;
+  if (restrict_op)
+    hipLaunchKernelGGL(k_restrict, dim3(nblk(n)), dim3(256), 0, stream, (const uint16_t*)in.p,
+                       in.cs, in.c0, H[lin], W[lin], out.p, out.cs, out.c0, H[lout], W[lout], C,
+                       scale, mask ? (const uint16_t*)mask->p : nullptr, mask ? mask->cs : 0,
+                       mask ? mask->c0 : 0, accumulate ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_prolong, dim3(nblk(n)), dim3(256), 0, stream, (const uint16_t*)in.p,
+                       in.cs, in.c0, H[lin], W[lin], out.p, out.cs, out.c0, H[lout], W[lout], C,
+                       scale, mask ? (const uint16_t*)mask->p : nullptr, mask ? mask->cs : 0,
+                       mask ? mask->c0 : 0, accumulate ? 1 : 0);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
+  pl = plan;
+  stream = plan->ctx->stream;
+  cu = plan->ctx->cu_count;
+  nx = plan->nx;
+  ny = plan->ny;
+  const int c1 = cfg && cfg->channels[0] > 0 ? cfg->channels[0] : 16;
+  const int c2 = cfg && cfg->channels[1] > 0 ? cfg->channels[1] : 32;
+  const int c3 = cfg && cfg->channels[2] > 0 ? cfg->channels[2] : 64;
+  const int bdil = cfg && cfg->bottleneck_dilation > 0 ? cfg->bottleneck_dilation : 1;
+  if (c1 % 16 || c2 % 16 || c3 % 16 || c1 > 64 || c2 > 64 || c3 > 64)
+    return fail(PP_ERR_INVALID, "channels must be multiples of 16, at most 64");
+  sw = 1.0;
+  st = cfg && cfg->rotation_scale > 0 ? cfg->rotation_scale : 0.3;
+  linear = cfg && cfg->relu ? 0 : 1;
+  line_search = cfg && cfg->fixed_step ? 0 : 1;
+  // padded level-0 grid: 4k + 1 nodes per side
+  W[0] = ((nx + 3) / 4) * 4 + 1;
+  H[0] = ((ny + 3) / 4) * 4 + 1;
+  for (int l = 1; l < 3; l++) {
+    W[l] = (W[l - 1] + 1) / 2;
+    H[l] = (H[l - 1] + 1) / 2;
+  }
+  const int C0 = c1, C1 = c2, C2 = c3;
+  auto mk = [&](int cin, int cout, int relu, int dil) {
+    Layer L;
+    L.cin = cin; L.cout = cout; L.relu = linear ? 0 : relu; L.dil = dil;
+    L.coutP = (cout + 15) / 16 * 16;
+    L.coutA = (cout + 3) / 4 * 4;
+    L.cinP = (cin + 15) / 16 * 16;
+    L.kp = (9 * cin + 15) / 16 * 16;
+    L.kpT = (9 * L.coutA + 15) / 16 * 16;
+    return L;
+  };
+  layers.clear();
+  layers.push_back(mk(4, C0, 1, 1));          // e0a
+  layers.push_back(mk(C0, C0, 1, 1));         // e0b
+  layers.push_back(mk(C0, C1, 1, 1));         // e1a
+  layers.push_back(mk(C1, C1, 1, 1));         // e1b
+  layers.push_back(mk(C1, C2, 1, 1));         // ba
+  layers.push_back(mk(C2, C2, 1, bdil));      // bb
+  layers.push_back(mk(C2 + C1, C1, 1, 1));    // d1a
+  layers.push_back(mk(C1, C1, 1, 1));         // d1b
+  layers.push_back(mk(C1 + C0, C0, 1, 1));    // d0a
+  layers.push_back(mk(C0, C0, 1, 1));         // d0b
+  layers.push_back(mk(C0, 3, 0, 1));          // head
+  int64_t np = 0, nwf = 0, nwt = 0;
+  for (auto& L : layers) {
+    L.w_off = np;
+    np += (int64_t)L.cout * 9 * L.cin;
+    L.b_off = np;
+    np += L.cout;
+    L.wf_off = nwf;
+    nwf += (int64_t)L.coutP * L.kp;
+    L.wt_off = nwt;
+    nwt += (int64_t)L.cinP * L.kpT;
+  }
+  n_params = np;
+  int rc;
+  if ((rc = params.alloc(np)) || (rc = grads.alloc(np)) || (rc = adam_m.alloc(np)) ||
+      (rc = adam_v.alloc(np)) || (rc = wf.alloc(nwf)) || (rc = wt.alloc(nwt)))
+    return rc;
+  // He initialisation (head x 0.1), zero biases
+  std::vector<float> h(np, 0.0f);
+  uint64_t s = seed;
+  for (size_t li = 0; li < layers.size(); li++) {
+    const Layer& L = layers[li];
+    const double sd = sqrt(2.0 / (9.0 * L.cin)) * (li + 1 == layers.size() ? 0.1 : 1.0);
+    for (int64_t k = 0; k < (int64_t)L.cout * 9 * L.cin; k++) h[L.w_off + k] = (float)(sd * gauss(s));
+  }
+  PP_HIP(hipMemcpyAsync(params.p, h.data(), np * 4, hipMemcpyHostToDevice, stream));
+  PP_HIP(hipMemsetAsync(adam_m.p, 0, np * 4, stream));
+  PP_HIP(hipMemsetAsync(adam_v.p, 0, np * 4, stream));
+  PP_HIP(hipMemsetAsync(grads.p, 0, np * 4, stream));
+  adam_t = 0;
+  // activations (forward) and their gradients
+  const int64_t P0 = (int64_t)H[0] * W[0], P1 = (int64_t)H[1] * W[1], P2 = (int64_t)H[2] * W[2];
+  cat0C = C1 + C0;
+  cat1C = C2 + C1;
+  struct B { DevBuf<uint16_t>* b; int64_t n; };
+  B bufs[] = {{&xin, P0 * 4},     {&a0, P0 * C0},       {&cat0, P0 * cat0C},  {&r1, P1 * C0},
+              {&a1, P1 * C1},     {&cat1, P1 * cat1C},  {&r2, P2 * C1},       {&a2, P2 * C2},
+              {&bb, P2 * C2},     {&c1b, P1 * C1},      {&d1, P1 * C1},       {&c0, P0 * C0},
+              {&c0b, P0 * C0},    {&g_c0b, P0 * C0},    {&g_c0, P0 * C0},     {&g_cat0, P0 * cat0C},
+              {&g_d1, P1 * C1},   {&g_c1, P1 * C1},     {&g_cat1, P1 * cat1C}, {&g_bb, P2 * C2},
+              {&g_a2, P2 * C2},   {&g_r2, P2 * C1},     {&g_a1, P1 * C1},     {&g_r1, P1 * C0},
+              {&g_a0, P0 * C0},   {&g_out, P0 * 4}};
+  for (auto& b : bufs)
+    if ((rc = b.b->alloc(b.n))) return rc;
+  for (auto& b : bufs) PP_HIP(hipMemsetAsync(b.b->p, 0, b.n * 2, stream));
+  if ((rc = z0v.alloc(pl->n_dof)) || (rc = z1v.alloc(pl->n_dof)) || (rc = r1v.alloc(pl->n_dof)))
+    return rc;
+  if ((rc = out32.alloc(P0 * 4)) || (rc = zbj.alloc(pl->n_dof)) || (rc = mout.alloc(pl->n_dof)) ||
+      (rc = dvec.alloc(pl->n_dof)) || (rc = kvec.alloc(pl->n_dof)) ||
+      (rc = evec.alloc(pl->n_dof)) || (rc = rvec.alloc(pl->n_dof)) ||
+      (rc = logt.alloc(pl->n_nodes)) || (rc = scal.alloc(16)) ||
+      (rc = dotp.alloc(2 * kDotBlocks)) || (rc = bumps.alloc(4 * 64)))
+    return rc;
+  // weight-gradient partials: the largest layer x its chunk count
+  int64_t wp = 0, bp = 0;
+  for (auto& L : layers)
+    for (int l = 0; l < 3; l++) {
+      const int64_t ch = (H[l] + kRowsPerChunk - 1) / kRowsPerChunk;
+      wp = std::max(wp, ch * L.coutP * L.kp);
+      bp = std::max(bp, ch * L.coutP);
+    }
+  if ((rc = wpart.alloc(wp)) || (rc = bpart.alloc(bp))) return rc;
+  KGrid G = make_kgrid(pl);
+  hipLaunchKernelGGL(k_node_logt, dim3(nblk(pl->n_nodes)), dim3(256), 0, stream, G, logt.p);
+  PP_HIP(hipGetLastError());
+  // dynamic LDS above 64 KB for the widest layers
+  size_t mx = 0;
+  for (auto& L : layers) mx = std::max({mx, (size_t)L.coutP * L.kp * 2, (size_t)L.cinP * L.kpT * 2});
+  const void* ks[] = {(const void*)k_conv<1>, (const void*)k_conv<2>, (const void*)k_conv<3>,
+                      (const void*)k_conv<4>, (const void*)k_conv<6>};
+  for (auto k : ks) PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx));
+  size_t mw = 0;
+  for (auto& L : layers)
+    mw = std::max(mw, ((size_t)L.cin * 3 * (64 + 2 * L.dil) + (size_t)L.coutP * 64) * 2);
+  PP_HIP(hipFuncSetAttribute((const void*)k_wgrad, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mw));
+  if ((rc = pack_weights())) return rc;
+  PP_HIP(hipMemsetAsync(scal.p, 0, 16 * 8, stream));
+  {
+    const double one = 1.0;  // scal[13] = 1, scal[14] = scal[15] = 0: constants
+    PP_HIP(hipMemcpyAsync(scal.p + 13, &one, 8, hipMemcpyHostToDevice, stream));
+  }
+  if (cfg && cfg->smoother_weight > 0) omega = cfg->smoother_weight;
+  else if ((rc = estimate_omega())) return rc;
+  PP_HIP(hipStreamSynchronize(stream));
+  return PP_OK;
+}
+
+int Cnn::pack_weights() {
+  for (auto& L : layers) {
+    const int64_t n = std::max((int64_t)L.coutP * L.kp, (int64_t)L.cinP * L.kpT);
+    hipLaunchKernelGGL(k_pack_w, dim3(nblk(n)), dim3(256), 0, stream, (const float*)params.p + L.w_off,
+                       L.cout, L.cin, L.coutP, L.kp, L.coutA, L.cinP, L.kpT, wf.p + L.wf_off,
+                       wt.p + L.wt_off);
+    PP_HIP(hipGetLastError());
+  }
+  return PP_OK;
+}
+
+// M(r) into out (FP64 DOF vector); r on the device.
+int Cnn::apply(const double* r, double* out) {
+  int rc;
+  hipLaunchKernelGGL(k_bj_norm, dim3(kDotBlocks), dim3(256), 0, stream, (int64_t)pl->n_nodes,
+                     (const double*)pl->minv.p, r, zbj.p, sw, st, dotp.p);
+  hipLaunchKernelGGL(k_scale, dim3(1), dim3(256), 0, stream, (const double*)dotp.p, kDotBlocks,
+                     (double)pl->n_nodes, scal.p);
+  hipLaunchKernelGGL(k_pack_in, dim3(nblk((int64_t)H[0] * W[0])), dim3(256), 0, stream, nx, ny,
+                     W[0], H[0], (const double*)zbj.p, (const double*)scal.p, sw, st,
+                     (const float*)logt.p, xin.p);
+  PP_HIP(hipGetLastError());
+  const Layer* L = layers.data();
+  const int C0 = L[0].cout, C1 = L[2].cout, C2 = L[4].cout;
+  TV vx{xin.p, 4, 0}, va0{a0.p, C0, 0}, vs0{cat0.p, cat0C, C1}, vr1{r1.p, C0, 0},
+      va1{a1.p, C1, 0}, vs1{cat1.p, cat1C, C2}, vr2{r2.p, C1, 0}, va2{a2.p, C2, 0},
+      vb{bb.p, C2, 0}, vpb{cat1.p, cat1C, 0}, vcat1{cat1.p, cat1C, 0}, vc1{c1b.p, C1, 0},
+      vd1{d1.p, C1, 0}, vpd1{cat0.p, cat0C, 0}, vcat0{cat0.p, cat0C, 0}, vc0{c0.p, C0, 0},
+      vc0b{c0b.p, C0, 0};
+  if ((rc = conv_fwd(L[0], vx, va0, nullptr, 0, 0, false))) return rc;
+  if ((rc = conv_fwd(L[1], va0, vs0, nullptr, 0, 0, false))) return rc;
+  if ((rc = resample(true, vs0, 0, vr1, 1, C0, 1.0f, nullptr, false))) return rc;
+  if ((rc = conv_fwd(L[2], vr1, va1, nullptr, 0, 1, false))) return rc;
+  if ((rc = conv_fwd(L[3], va1, vs1, nullptr, 0, 1, false))) return rc;
+  if ((rc = resample(true, vs1, 1, vr2, 2, C1, 1.0f, nullptr, false))) return rc;
+  if ((rc = conv_fwd(L[4], vr2, va2, nullptr, 0, 2, false))) return rc;
+  if ((rc = conv_fwd(L[5], va2, vb, nullptr, 0, 2, false))) return rc;
+  if ((rc = resample(false, vb, 2, vpb, 1, C2, 1.0f, nullptr, false))) return rc;
+  if ((rc = conv_fwd(L[6], vcat1, vc1, nullptr, 0, 1, false))) return rc;
+  if ((rc = conv_fwd(L[7], vc1, vd1, nullptr, 0, 1, false))) return rc;
+  if ((rc = resample(false, vd1, 1, vpd1, 0, C1, 1.0f, nullptr, false))) return rc;
+  if ((rc = conv_fwd(L[8], vcat0, vc0, nullptr, 0, 0, false))) return rc;
+  if ((rc = conv_fwd(L[9], vc0, vc0b, nullptr, 0, 0, false))) return rc;
+  if ((rc = conv_fwd(L[10], vc0b, TV{nullptr, 0, 0}, out32.p, 4, 0, false))) return rc;
+  hipLaunchKernelGGL(k_unpack_out, dim3(nblk(pl->n_nodes)), dim3(256), 0, stream, nx, ny, W[0],
+                     (const double*)zbj.p, (const float*)out32.p, (const double*)scal.p, sw, st,
+                     (const uint8_t*)pl->cmask.p, out);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+// Backward pass from g_out (bf16 dL/do); weight and bias gradients into grads.
+int Cnn::backward() {
+  int rc;
+  const Layer* L = layers.data();
+  const int C0 = L[0].cout, C1 = L[2].cout, C2 = L[4].cout;
+  TV vx{xin.p, 4, 0}, va0{a0.p, C0, 0}, vs0{cat0.p, cat0C, C1}, vr1{r1.p, C0, 0},
+      va1{a1.p, C1, 0}, vs1{cat1.p, cat1C, C2}, vr2{r2.p, C1, 0}, va2{a2.p, C2, 0},
+      vb{bb.p, C2, 0}, vcat1{cat1.p, cat1C, 0}, vc1{c1b.p, C1, 0}, vd1{d1.p, C1, 0},
+      vcat0{cat0.p, cat0C, 0}, vc0{c0.p, C0, 0}, vc0b{c0b.p, C0, 0};
+  TV gout{g_out.p, 4, 0}, gc0b{g_c0b.p, C0, 0}, gc0{g_c0.p, C0, 0}, gcat0{g_cat0.p, cat0C, 0},
+      gpd1{g_cat0.p, cat0C, 0}, gs0{g_cat0.p, cat0C, C1}, gd1{g_d1.p, C1, 0}, gc1{g_c1.p, C1, 0},
+      gcat1{g_cat1.p, cat1C, 0}, gpb{g_cat1.p, cat1C, 0}, gs1{g_cat1.p, cat1C, C2},
+      gbb{g_bb.p, C2, 0}, ga2{g_a2.p, C2, 0}, gr2{g_r2.p, C1, 0}, ga1{g_a1.p, C1, 0},
+      gr1{g_r1.p, C0, 0}, ga0{g_a0.p, C0, 0};
+  // head (no ReLU): dW; d c0b = W^T g_out masked by c0b > 0
+  if ((rc = conv_wgrad(L[10], vc0b, gout, 0))) return rc;
+  if ((rc = conv_fwd(L[10], gout, gc0b, nullptr, 0, 0, true, mk(vc0b), 0))) return rc;
+  if ((rc = conv_wgrad(L[9], vc0, gc0b, 0))) return rc;
+  if ((rc = conv_fwd(L[9], gc0b, gc0, nullptr, 0, 0, true, mk(vc0), 0))) return rc;
+  if ((rc = conv_wgrad(L[8], vcat0, gc0, 0))) return rc;
+  // d cat0: channels >= C1 are s0 (ReLU output), below are P(d1) (linear)
+  if ((rc = conv_fwd(L[8], gc0, gcat0, nullptr, 0, 0, true, mk(vcat0), C1))) return rc;
+  // P^T = 4 R, masked by d1 > 0
+  if ((rc = resample(true, gpd1, 0, gd1, 1, C1, 4.0f, mk(vd1), false))) return rc;
+  if ((rc = conv_wgrad(L[7], vc1, gd1, 1))) return rc;
+  if ((rc = conv_fwd(L[7], gd1, gc1, nullptr, 0, 1, true, mk(vc1), 0))) return rc;
+  if ((rc = conv_wgrad(L[6], vcat1, gc1, 1))) return rc;
+  if ((rc = conv_fwd(L[6], gc1, gcat1, nullptr, 0, 1, true, mk(vcat1), C2))) return rc;
+  if ((rc = resample(true, gpb, 1, gbb, 2, C2, 4.0f, mk(vb), false))) return rc;
+  if ((rc = conv_wgrad(L[5], va2, gbb, 2))) return rc;
+  if ((rc = conv_fwd(L[5], gbb, ga2, nullptr, 0, 2, true, mk(va2), 0))) return rc;
+  if ((rc = conv_wgrad(L[4], vr2, ga2, 2))) return rc;
+  if ((rc = conv_fwd(L[4], ga2, gr2, nullptr, 0, 2, true, nullptr, 0))) return rc;
+  // R^T = P / 4, accumulated into d s1 (skip path), masked by s1 > 0
+  if ((rc = resample(false, gr2, 2, gs1, 1, C1, 0.25f, mk(vs1), true))) return rc;
+  if ((rc = conv_wgrad(L[3], va1, gs1, 1))) return rc;
+  if ((rc = conv_fwd(L[3], gs1, ga1, nullptr, 0, 1, true, mk(va1), 0))) return rc;
+  if ((rc = conv_wgrad(L[2], vr1, ga1, 1))) return rc;
+  if ((rc = conv_fwd(L[2], ga1, gr1, nullptr, 0, 1, true, nullptr, 0))) return rc;
+  if ((rc = resample(false, gr1, 1, gs0, 0, C0, 0.25f, mk(vs0), true))) return rc;
+  if ((rc = conv_wgrad(L[1], va0, gs0, 0))) return rc;
+  if ((rc = conv_fwd(L[1], gs0, ga0, nullptr, 0, 0, true, mk(va0), 0))) return rc;
+  if ((rc = conv_wgrad(L[0], vx, ga0, 0))) return rc;
+  return PP_OK;
+}
+
+// The preconditioner: damped block-Jacobi smoothing around the network,
+//   z0 = w B r;  r1 = r - K z0;  u = N(r1);  z1 = z0 + g u;  z = z1 + w B (r1 - g K u)
+// (the network takes the smooth error the smoother leaves).  With
+// line_search the step g = u.r1 / u.K u minimises the energy error of z1
+// along u, so an application is never worse than the smoothing alone; in
+// training g = 1 (the loss sees the network's own output).
+int Cnn::precondition(const double* r, double* out, bool line_search) {
+  int rc;
+  const int64_t nd = pl->n_dof, nn = pl->n_nodes;
+  const double* B = pl->minv.p;
+  hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, B, r, omega,
+                     (const double*)nullptr, z0v.p);
+  if ((rc = struct_apply_k(pl, z0v.p, kvec.p))) return rc;
+  hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, r, -1.0,
+                     (const double*)kvec.p, r1v.p);
+  if ((rc = apply(r1v.p, mout.p))) return rc;
+  if ((rc = struct_apply_k(pl, mout.p, kvec.p))) return rc;  // K u
+  double* g = scal.p + 10;
+  if (line_search) {
+    hipLaunchKernelGGL(k_dot2, dim3(kDotBlocks), dim3(256), 0, stream, nd, (const double*)mout.p,
+                       (const double*)r1v.p, (const double*)mout.p, (const double*)kvec.p,
+                       dotp.p);
+    hipLaunchKernelGGL(k_gamma, dim3(1), dim3(256), 0, stream, (const double*)dotp.p, kDotBlocks, g);
+  } else {
+    hipLaunchKernelGGL(k_lin, dim3(1), dim3(1), 0, stream, (int64_t)1, (const double*)scal.p + 14,
+                       1.0, (const double*)scal.p + 13, g);  // g = 0 + 1
+  }
+  hipLaunchKernelGGL(k_axpy_dev, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)z0v.p,
+                     (const double*)g, 1.0, (const double*)mout.p, z1v.p);
+  hipLaunchKernelGGL(k_axpy_dev, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)r1v.p,
+                     (const double*)g, -1.0, (const double*)kvec.p, r1v.p);
+  hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, B, (const double*)r1v.p,
+                     omega, (const double*)z1v.p, out);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+// One pair: e (device, FP64), r = K e, z = precondition(r), e_f = e - z.
+// Loss e_f^T K e_f / e^T K e.  With S = I - w B K, e_f = S (e - z0 - u) for
+// the network output u, so dL/du = -2 S^T K e_f / e^T K e and
+// S^T K e_f = q1 - w K B q1 with q1 = K e_f.
+int Cnn::loss_grad(const double* e, double* loss_dev) {
+  int rc;
+  const int64_t nd = pl->n_dof, nn = pl->n_nodes;
+  if ((rc = struct_apply_k(pl, e, rvec.p))) return rc;
+  if ((rc = precondition(rvec.p, dvec.p, false))) return rc;  // z (scal[0] = s of the network input)
+  hipLaunchKernelGGL(k_sub, dim3(nblk(nd)), dim3(256), 0, stream, nd, e, (const double*)dvec.p,
+                     dvec.p);  // e_f
+  if ((rc = struct_apply_k(pl, dvec.p, kvec.p))) return rc;  // q1
+  hipLaunchKernelGGL(k_dot2, dim3(kDotBlocks), dim3(256), 0, stream, nd, (const double*)dvec.p,
+                     (const double*)kvec.p, e, (const double*)rvec.p, dotp.p);
+  hipLaunchKernelGGL(k_dot2_finish, dim3(1), dim3(256), 0, stream, (const double*)dotp.p,
+                     kDotBlocks, scal.p + 2);
+  hipLaunchKernelGGL(k_loss, dim3(1), dim3(64), 0, stream, (const double*)scal.p + 2, loss_dev,
+                     scal.p + 4);  // scal[4] = 2 / eKe
+  hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, (const double*)pl->minv.p,
+                     (const double*)kvec.p, 1.0, (const double*)nullptr, z0v.p);  // B q1
+  if ((rc = struct_apply_k(pl, z0v.p, z1v.p))) return rc;                          // K B q1
+  hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)kvec.p,
+                     -omega, (const double*)z1v.p, r1v.p);  // S^T K e_f
+  // dL/do = -(2 / eKe) S^T K e_f * s / scale_c
+  hipLaunchKernelGGL(k_lin, dim3(1), dim3(1), 0, stream, (int64_t)1, (const double*)scal.p + 15,
+                     -1.0, (const double*)scal.p + 4, scal.p + 5);  // scal[5] = 0 - scal[4]
+  hipLaunchKernelGGL(k_pack_dout, dim3(nblk((int64_t)H[0] * W[0])), dim3(256), 0, stream, nx, ny,
+                     W[0], H[0], (const double*)r1v.p, (const double*)scal.p,
+                     (const double*)scal.p + 5, sw, st, (const uint8_t*)pl->cmask.p, g_out.p);
+  PP_HIP(hipGetLastError());
+  return backward();
+}
+
+// omega = 4 / (3 lambda_max(B K)), lambda_max by 40 power iterations on B K
+// from a fixed pseudo-random start (x 1.05 against underestimation).
+int Cnn::estimate_omega() {
+  const int64_t nd = pl->n_dof, nn = pl->n_nodes;
+  std::vector<double> h(nd);
+  uint64_t s = 12345;
+  for (int64_t i = 0; i < nd; i++) h[i] = pl->h_cmask[i / 3] >> (i % 3) & 1 ? 0.0 : uni(s) - 0.5;
+  PP_HIP(hipMemcpyAsync(z0v.p, h.data(), nd * 8, hipMemcpyHostToDevice, stream));
+  double lam = 0.0;
+  for (int it = 0; it < 40; it++) {
+    int rc;
+    if ((rc = struct_apply_k(pl, z0v.p, kvec.p))) return rc;
+    hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, (const double*)pl->minv.p,
+                       (const double*)kvec.p, 1.0, (const double*)nullptr, z1v.p);
+    hipLaunchKernelGGL(k_dot2, dim3(kDotBlocks), dim3(256), 0, stream, nd, (const double*)z1v.p,
+                       (const double*)z1v.p, (const double*)z0v.p, (const double*)z0v.p, dotp.p);
+    double hp[2 * kDotBlocks];
+    PP_HIP(hipMemcpyAsync(hp, dotp.p, sizeof(hp), hipMemcpyDeviceToHost, stream));
+    PP_HIP(hipStreamSynchronize(stream));
+    double a = 0, b = 0;
+    for (int k = 0; k < kDotBlocks; k++) {
+      a += hp[2 * k];
+      b += hp[2 * k + 1];
+    }
+    if (!(a > 0) || !(b > 0)) break;
+    lam = sqrt(a / b);
+    hipLaunchKernelGGL(k_scale_vec, dim3(nblk(nd)), dim3(256), 0, stream, nd, z1v.p, 1.0 / sqrt(a));
+    std::swap(z0v.p, z1v.p);
+  }
+  PP_HIP(hipGetLastError());
+  omega = lam > 0 ? 4.0 / (3.0 * 1.05 * lam) : 0.5;
+  return PP_OK;
+}
+
+int Cnn::gen_error(uint64_t seed, double* e) {
+  // 32 bumps: amplitude ~ N(0,1) * sigma, sigma log-uniform in [2h, L/2]
+  const int nb = 32;
+  const double x0 = pl->h_nodes[0], x1 = pl->h_nodes[2 * nx];
+  const double y0 = pl->h_nodes[1], y1 = pl->h_nodes[2 * ((size_t)ny * (nx + 1)) + 1];
+  const double L = std::max(x1 - x0, y1 - y0), h = std::min((x1 - x0) / nx, (y1 - y0) / ny);
+  std::vector<double> b(4 * nb);
+  uint64_t s = seed;
+  for (int k = 0; k < nb; k++) {
+    const double sig = exp(log(2.0 * h) + uni(s) * (log(0.5 * L) - log(2.0 * h)));
+    b[4 * k] = gauss(s) * sig;
+    b[4 * k + 1] = x0 + uni(s) * (x1 - x0);
+    b[4 * k + 2] = y0 + uni(s) * (y1 - y0);
+    b[4 * k + 3] = sig;
+  }
+  PP_HIP(hipMemcpyAsync(bumps.p, b.data(), b.size() * 8, hipMemcpyHostToDevice, stream));
+  KGrid G = make_kgrid(pl);
+  hipLaunchKernelGGL(k_gen_error, dim3(nblk(pl->n_nodes)), dim3(256), 0, stream, G,
+                     (const double*)bumps.p, nb, e);
+  PP_HIP(hipGetLastError());
+  // the host vector b must outlive the copy
+  PP_HIP(hipStreamSynchronize(stream));
+  return PP_OK;
+}
+
+int Cnn::adam_step(float lr, float max_norm) {
+  adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - powf(b1, (float)adam_t), c2 = 1.0f - powf(b2, (float)adam_t);
+  float clip = 1.0f;
+  if (max_norm > 0) {
+    hipLaunchKernelGGL(k_sumsq, dim3(kDotBlocks), dim3(256), 0, stream, n_params,
+                       (const float*)grads.p, dotp.p);
+    double h[kDotBlocks];
+    PP_HIP(hipMemcpyAsync(h, dotp.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    PP_HIP(hipStreamSynchronize(stream));
+    double ss = 0;
+    for (int i = 0; i < kDotBlocks; i++) ss += h[i];
+    const double gn = sqrt(ss);
+    last_grad_norm = gn;
+    if (gn > max_norm) clip = (float)(max_norm / gn);
+  }
+  hipLaunchKernelGGL(k_adam, dim3(nblk(n_params)), dim3(256), 0, stream, n_params, params.p,
+                     (const float*)grads.p, adam_m.p, adam_v.p, lr, b1, b2, 1e-8f, c1, c2, clip);
+  PP_HIP(hipGetLastError());
+  return pack_weights();
+}
+
+Cnn::~Cnn() {
+  if (!pl) return;
+  if (pl->learned == this) pl->learned = nullptr;
+  (void)hipStreamSynchronize(stream);
+  pp_plan* p = pl;
+  pl = nullptr;
+  plan_release(p);
+}
+
+}  // namespace pp

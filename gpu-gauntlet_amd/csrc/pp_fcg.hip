@@ -1,0 +1,481 @@
+// Flexible PCG with the learned preconditioner, the training loop, and the
+// C ABI of networks.
+//
+// FCG(1) (Notay's flexible CG truncated to one previous direction):
+//   r = F, z = M(r), p = z
+//   loop: q = K p; alpha = p.r / p.q; x += alpha p; r -= alpha q
+//         stop when |r| < tol or |r| / |F| < tol (the rule of pcg.ts:174)
+//         z = M(r); beta = -(z.q) / (p.q); p = z + beta p
+// The K-orthogonalisation of z against the last direction keeps the method
+// convergent for a preconditioner that is not a fixed SPD matrix (the
+// network is nonlinear through its ReLUs and its input scaling).  Scalars
+// stay on the device; the host reads the convergence flag every few
+// iterations, and the update kernels stop changing x and r once it is set.
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "pp_cnn.h"
+
+namespace pp {
+
+namespace {
+
+struct FcgState {
+  double pq, pr, alpha, rr, zq, beta, fnorm, tol;
+  int it, done, breakdown, converged;
+};
+
+constexpr int kBlocks = 256;
+constexpr int kCheckEvery = 4;
+
+int nb(int64_t n) { return (int)((n + 255) / 256); }
+
+// part[2b] = a.b, part[2b+1] = c.d per block (fixed order)
+__global__ void k_dots(int64_t n, const double* a, const double* b, const double* c,
+                       const double* d, double* part) {
+  __shared__ double r0[256], r1[256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    s0 += a[i] * b[i];
+    if (c) s1 += c[i] * d[i];
+  }
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0[0];
+    part[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
+// Fixed-order sum of the kBlocks pairs of partials by one 256-thread block
+// (every thread gets the result).
+__device__ void sum2(const double* part, double& a, double& b) {
+  __shared__ double sa[kBlocks], sb[kBlocks];
+  const int t = threadIdx.x;
+  sa[t] = part[2 * t];
+  sb[t] = part[2 * t + 1];
+  __syncthreads();
+  for (int k = kBlocks / 2; k > 0; k >>= 1) {
+    if (t < k) {
+      sa[t] += sa[t + k];
+      sb[t] += sb[t + k];
+    }
+    __syncthreads();
+  }
+  a = sa[0];
+  b = sb[0];
+}
+
+// after q = K p: pq, pr -> alpha (breakdown when |p.q| < 1e-30, pcg.ts:154)
+__global__ void k_fcg_alpha(const double* part, FcgState* S) {
+  if (S->done) return;  // uniform over the block
+  double pq, pr;
+  sum2(part, pq, pr);
+  if (threadIdx.x) return;
+  S->pq = pq;
+  S->pr = pr;
+  if (fabs(S->pq) < 1e-30) {
+    S->breakdown = 1;
+    S->done = 1;
+    return;
+  }
+  S->alpha = S->pr / S->pq;
+}
+
+// x += alpha p; r -= alpha q; partial |r|^2
+__global__ void k_fcg_update(int64_t n, double* x, double* r, const double* p, const double* q,
+                             const FcgState* S, double* part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  if (!S->done) {
+    const double a = S->alpha;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      x[i] += a * p[i];
+      const double ri = r[i] - a * q[i];
+      r[i] = ri;
+      s += ri * ri;
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// |r| test (the OR rule) after an update
+__global__ void k_fcg_check(const double* part, FcgState* S) {
+  if (S->done) return;
+  double rr, unused;
+  sum2(part, rr, unused);
+  if (threadIdx.x) return;
+  S->rr = rr;
+  S->it++;
+  const double rn = sqrt(rr);
+  if (rn < S->tol || rn / S->fnorm < S->tol) {
+    S->converged = 1;
+    S->done = 1;
+  }
+}
+
+// after z = M(r): zq -> beta
+__global__ void k_fcg_beta(const double* part, FcgState* S) {
+  if (S->done) return;
+  double zq, unused;
+  sum2(part, zq, unused);
+  if (threadIdx.x) return;
+  S->zq = zq;
+  S->beta = -zq / S->pq;
+}
+
+// p = z + beta p (beta = 0: p = z)
+__global__ void k_fcg_dir(int64_t n, const double* z, double* p, const FcgState* S, int first) {
+  if (S->done && !first) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = first ? z[i] : z[i] + S->beta * p[i];
+}
+
+// dst = (float)(a - b) (b may be null)
+__global__ void k_err32(int64_t n, const double* a, const double* b, float* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)(a[i] - (b ? b[i] : 0.0));
+}
+
+__global__ void k_widen(int64_t n, const float* a, double* b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+double splitmix_uni(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_fcg_init(FcgState* S, double fnorm, double tol, double rr) {
+  if (threadIdx.x) return;
+  *S = FcgState{};
+  S->fnorm = fnorm > 0.0 ? fnorm : 1.0;
+  S->tol = tol;
+  S->rr = rr;
+}
+
+}  // namespace
+
+// FCG from x (zero on a cold start) with r = F - K x given in pl->r.
+static int fcg_run(pp_plan* pl, Cnn& net, double fnorm, double tol, int maxit, FcgState* hs,
+                   DevBuf<FcgState>& S, DevBuf<double>& part) {
+  hipStream_t s = pl->ctx->stream;
+  const int64_t n = pl->n_dof;
+  int rc;
+  hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(64), 0, s, S.p, fnorm, tol, 0.0);
+  // z = M(r), p = z
+  if ((rc = net.precondition(pl->r.p, pl->z.p, net.line_search != 0))) return rc;
+  hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p, pl->p0.p,
+                     (const FcgState*)S.p, 1);
+  PP_HIP(hipGetLastError());
+  for (int k = 0; k < maxit; k++) {
+    if ((rc = struct_apply_k(pl, pl->p0.p, pl->Ap.p))) return rc;
+    hipLaunchKernelGGL(k_dots, dim3(kBlocks), dim3(256), 0, s, n, (const double*)pl->p0.p,
+                       (const double*)pl->Ap.p, (const double*)pl->p0.p, (const double*)pl->r.p,
+                       part.p);
+    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p);
+    hipLaunchKernelGGL(k_fcg_update, dim3(kBlocks), dim3(256), 0, s, n, pl->x.p, pl->r.p,
+                       (const double*)pl->p0.p, (const double*)pl->Ap.p, (const FcgState*)S.p,
+                       part.p);
+    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p);
+    PP_HIP(hipGetLastError());
+    if ((k + 1) % kCheckEvery == 0 || k + 1 == maxit) {
+      PP_HIP(hipMemcpyAsync(hs, S.p, sizeof(FcgState), hipMemcpyDeviceToHost, s));
+      PP_HIP(hipStreamSynchronize(s));
+      if (hs->done) break;
+    }
+    if ((rc = net.precondition(pl->r.p, pl->z.p, net.line_search != 0))) return rc;
+    hipLaunchKernelGGL(k_dots, dim3(kBlocks), dim3(256), 0, s, n, (const double*)pl->z.p,
+                       (const double*)pl->Ap.p, (const double*)nullptr, (const double*)nullptr,
+                       part.p);
+    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p);
+    hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p,
+                       pl->p0.p, (const FcgState*)S.p, 0);
+    PP_HIP(hipGetLastError());
+  }
+  PP_HIP(hipMemcpyAsync(hs, S.p, sizeof(FcgState), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+// Flexible PCG with residual replacement (as the FP64 block-Jacobi path):
+// after convergence the true FP64 residual is measured; while it is above
+// tol, FCG restarts from x with that residual (at most 4 times).
+int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  Cnn& net = *pl->learned;
+  hipStream_t s = pl->ctx->stream;
+  const int64_t n = pl->n_dof;
+  DevBuf<FcgState> S;
+  DevBuf<double> part;
+  int rc;
+  if ((rc = S.alloc(1)) || (rc = part.alloc(2 * kBlocks))) return rc;
+  PP_HIP(hipEventRecord(pl->ctx->ev0, s));
+  PP_HIP(hipMemsetAsync(pl->x.p, 0, n * 8, s));
+  PP_HIP(hipMemcpyAsync(pl->r.p, pl->F.p, n * 8, hipMemcpyDeviceToDevice, s));
+  double res = 0, fnorm = 0;
+  if ((rc = launch_norms(pl, pl->F.p, pl->F.p, &res, &fnorm))) return rc;
+  FcgState hs;
+  memset(&hs, 0, sizeof(hs));
+  int total = 0, sweeps = 0;
+  double rel = 0.0;
+  if (fnorm < 1e-30) {
+    hs.converged = 1;
+  } else {
+    for (int k = 0; k <= 4; k++) {
+      if ((rc = fcg_run(pl, net, fnorm, o->tolerance, o->max_iterations - total, &hs, S, part)))
+        return rc;
+      if (sweeps < PP_MAX_SWEEPS) st->sweep_iterations[sweeps] = hs.it;
+      total += hs.it;
+      if ((rc = struct_apply_k(pl, pl->x.p, pl->tmp.p))) return rc;
+      if ((rc = launch_norms(pl, pl->F.p, pl->tmp.p, &res, &fnorm))) return rc;
+      rel = res / fnorm;
+      if (sweeps < PP_MAX_SWEEPS) st->sweep_residual[sweeps] = rel;
+      sweeps++;
+      if (!hs.converged || rel <= o->tolerance || total >= o->max_iterations) break;
+      if ((rc = launch_residual(pl, pl->F.p, pl->tmp.p, pl->r.p))) return rc;
+    }
+  }
+  PP_HIP(hipEventRecord(pl->ctx->ev1, s));
+  PP_HIP(hipEventSynchronize(pl->ctx->ev1));
+  float ms = 0;
+  PP_HIP(hipEventElapsedTime(&ms, pl->ctx->ev0, pl->ctx->ev1));
+  st->iterations = total;
+  st->converged = hs.converged && rel <= o->tolerance;
+  st->breakdown = hs.breakdown;
+  st->sweeps = sweeps;
+  st->final_residual = sqrt(hs.rr);
+  st->true_rel_residual = rel;
+  st->device_ms = ms;
+  return PP_OK;
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+extern "C" {
+
+int pp_cnn_create(pp_plan* pl, const pp_cnn_config* cfg, uint64_t seed, pp_cnn** out) {
+  if (!pl || !out) return fail(PP_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (!pl->structured) return fail(PP_ERR_UNSUPPORTED, "the learned preconditioner needs a structured grid");
+  PP_HIP(hipSetDevice(pl->ctx->device));
+  pp_cnn* c = new pp_cnn();
+  pl->refs++;
+  int rc = c->net.setup(pl, cfg, seed);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return PP_OK;
+}
+
+int pp_cnn_destroy(pp_cnn* c) {
+  if (!c) return PP_OK;
+  (void)hipSetDevice(c->net.pl->ctx->device);
+  delete c;
+  return PP_OK;
+}
+
+int64_t pp_cnn_param_count(const pp_cnn* c) { return c ? c->net.n_params : 0; }
+
+double pp_cnn_smoother_weight(const pp_cnn* c) { return c ? c->net.omega : 0.0; }
+
+int pp_cnn_get_params(pp_cnn* c, float* out) {
+  if (!c || !out) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  PP_HIP(hipSetDevice(n.pl->ctx->device));
+  PP_HIP(hipMemcpyAsync(out, n.params.p, n.n_params * 4, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
+int pp_cnn_set_params(pp_cnn* c, const float* in) {
+  if (!c || !in) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  PP_HIP(hipSetDevice(n.pl->ctx->device));
+  PP_HIP(hipMemcpyAsync(n.params.p, in, n.n_params * 4, hipMemcpyHostToDevice, n.stream));
+  int rc = n.pack_weights();
+  if (rc) return rc;
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
+int pp_cnn_apply(pp_cnn* c, const double* r, double* z) {
+  if (!c || !r || !z) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  pp_plan* pl = n.pl;
+  PP_HIP(hipSetDevice(pl->ctx->device));
+  PP_HIP(hipMemcpyAsync(n.evec.p, r, pl->n_dof * 8, hipMemcpyHostToDevice, n.stream));
+  int rc = n.precondition(n.evec.p, n.rvec.p, false);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(z, n.rvec.p, pl->n_dof * 8, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
+int pp_cnn_loss_grad(pp_cnn* c, const double* e, double* loss, float* grad) {
+  if (!c || !e || !loss) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  pp_plan* pl = n.pl;
+  PP_HIP(hipSetDevice(pl->ctx->device));
+  PP_HIP(hipMemcpyAsync(n.evec.p, e, pl->n_dof * 8, hipMemcpyHostToDevice, n.stream));
+  int rc = n.loss_grad(n.evec.p, n.scal.p + 6);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(loss, n.scal.p + 6, 8, hipMemcpyDeviceToHost, n.stream));
+  if (grad) PP_HIP(hipMemcpyAsync(grad, n.grads.p, n.n_params * 4, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
+int pp_cnn_gen_error(pp_cnn* c, uint64_t seed, double* e) {
+  if (!c || !e) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  PP_HIP(hipSetDevice(n.pl->ctx->device));
+  int rc = n.gen_error(seed, n.evec.p);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(e, n.evec.p, n.pl->n_dof * 8, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
+int pp_cnn_train(pp_cnn* c, const pp_train_opts* o, pp_train_stats* st) {
+  if (!c || !o || !st) return fail(PP_ERR_INVALID, "NULL argument");
+  if (o->pairs <= 0 || o->epochs <= 0 || !(o->lr > 0)) return fail(PP_ERR_INVALID, "bad training options");
+  if (o->source != PP_TRAIN_SYNTHETIC && o->source != PP_TRAIN_SOLVER)
+    return fail(PP_ERR_INVALID, "unknown training source");
+  Cnn& n = c->net;
+  pp_plan* pl = n.pl;
+  PP_HIP(hipSetDevice(pl->ctx->device));
+  memset(st, 0, sizeof(*st));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t nd = pl->n_dof;
+  int rc;
+  // solver pairs: per load case the FP64 block-Jacobi solution x* (to 1e-12)
+  // and the errors x* - x_k of its iterates k = 0, then geometric up to the
+  // iteration count, kept on the device in FP32
+  const bool solver = o->source == PP_TRAIN_SOLVER;
+  const int snaps = solver ? std::max(1, o->snapshots) : 1;
+  const int per_epoch = o->pairs * snaps;
+  DevBuf<float> bank;
+  if (solver) {
+    if ((rc = bank.alloc((size_t)per_epoch * nd))) return rc;
+    const double x0 = pl->h_nodes[0], x1 = pl->h_nodes[2 * pl->nx];
+    const double y0 = pl->h_nodes[1], y1 = pl->h_nodes[2 * ((size_t)pl->ny * (pl->nx + 1)) + 1];
+    std::vector<float> F(nd);
+    pp_solve_opts so;
+    memset(&so, 0, sizeof(so));
+    so.tolerance = 1e-12;
+    so.max_iterations = 1000000;
+    so.flags = PP_FLAG_NO_TRUE_RESIDUAL;
+    pp_solve_stats ss;
+    for (int l = 0; l < o->pairs; l++) {
+      uint64_t sd = o->seed0 + (uint64_t)l;
+      const int np = 1 + (int)(splitmix_uni(sd) * 3.0);
+      std::vector<double> pt(4 * 3);
+      for (int k = 0; k < np; k++) {
+        pt[4 * k] = 1e4 + 4e4 * splitmix_uni(sd);
+        pt[4 * k + 1] = x0 + (x1 - x0) * splitmix_uni(sd);
+        pt[4 * k + 2] = y0 + (y1 - y0) * splitmix_uni(sd);
+        pt[4 * k + 3] = 0.5 + 1.5 * splitmix_uni(sd);
+      }
+      if ((rc = pp_pressure_load(pl, 1e4, pt.data(), np, F.data()))) return rc;
+      if ((rc = pp_set_rhs(pl, F.data()))) return rc;
+      so.max_iterations = 1000000;
+      if ((rc = struct_solve_fp64(pl, &so, &ss))) return rc;
+      const int kstar = std::max(1, ss.iterations);
+      PP_HIP(hipMemcpyAsync(n.evec.p, pl->x.p, nd * 8, hipMemcpyDeviceToDevice, n.stream));
+      for (int j = 0; j < snaps; j++) {
+        const int k = j == 0 ? 0 : (int)llround(pow((double)kstar, (double)j / snaps));
+        float* dst = bank.p + ((size_t)l * snaps + j) * nd;
+        if (k == 0) {
+          hipLaunchKernelGGL(k_err32, dim3(nb(nd)), dim3(256), 0, n.stream, nd,
+                             (const double*)n.evec.p, (const double*)nullptr, dst);
+        } else {
+          so.max_iterations = k;
+          if ((rc = struct_solve_fp64(pl, &so, &ss))) return rc;
+          hipLaunchKernelGGL(k_err32, dim3(nb(nd)), dim3(256), 0, n.stream, nd,
+                             (const double*)n.evec.p, (const double*)pl->x.p, dst);
+        }
+        PP_HIP(hipGetLastError());
+      }
+    }
+    PP_HIP(hipStreamSynchronize(n.stream));
+  }
+  const int steps = per_epoch * o->epochs;
+  const int nbuck = std::min(steps, PP_TRAIN_CURVE);
+  std::vector<double> bucket(nbuck, 0.0);
+  std::vector<int> bcount(nbuck, 0);
+  DevBuf<double> losses;
+  if ((rc = losses.alloc(steps))) return rc;
+  const double lr1 = o->lr_final > 0 ? o->lr_final : o->lr;
+  for (int step = 0; step < steps; step++) {
+    const int pair = step % per_epoch;
+    if (solver) {
+      hipLaunchKernelGGL(k_widen, dim3(nb(nd)), dim3(256), 0, n.stream, nd,
+                         (const float*)bank.p + (size_t)pair * nd, n.evec.p);
+      PP_HIP(hipGetLastError());
+    } else if ((rc = n.gen_error(o->seed0 + (uint64_t)pair, n.evec.p))) {
+      return rc;
+    }
+    if ((rc = n.loss_grad(n.evec.p, losses.p + step))) return rc;
+    // cosine decay from lr to lr_final
+    const double f = steps > 1 ? (double)step / (steps - 1) : 0.0;
+    const double lr = lr1 + 0.5 * (o->lr - lr1) * (1.0 + cos(3.141592653589793 * f));
+    if ((rc = n.adam_step((float)lr, (float)o->max_grad_norm))) return rc;
+  }
+  std::vector<double> h(steps);
+  PP_HIP(hipMemcpyAsync(h.data(), losses.p, steps * 8, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  for (int i = 0; i < steps; i++) {
+    const int b = (int)((int64_t)i * nbuck / steps);
+    bucket[b] += h[i];
+    bcount[b]++;
+  }
+  for (int b = 0; b < nbuck; b++) st->loss_curve[b] = bcount[b] ? bucket[b] / bcount[b] : 0.0;
+  st->curve_points = nbuck;
+  st->steps = steps;
+  st->loss_first = h[0];
+  st->loss_last = h[steps - 1];
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return PP_OK;
+}
+
+int pp_plan_set_learned(pp_plan* pl, pp_cnn* c) {
+  if (!pl) return fail(PP_ERR_INVALID, "NULL plan");
+  if (c && c->net.pl != pl) return fail(PP_ERR_INVALID, "network was made for another plan");
+  pl->learned = c ? &c->net : nullptr;
+  return PP_OK;
+}
+
+}  // extern "C"

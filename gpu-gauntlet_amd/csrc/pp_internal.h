@@ -75,6 +75,8 @@ struct DevStats {
   unsigned long long ticks[6];
 };
 
+struct Cnn;
+
 }  // namespace pp
 
 struct pp_ctx {
@@ -87,6 +89,8 @@ struct pp_ctx {
 
 struct pp_plan {
   pp_ctx* ctx = nullptr;
+  int refs = 1;             // the owner plus one per live network on this plan
+  pp::Cnn* learned = nullptr;  // preconditioner for PP_PRECOND_LEARNED (not owned)
   int64_t n_nodes = 0, n_elems = 0, n_dof = 0;
   int npe = 4;
   bool structured = false;
@@ -146,6 +150,10 @@ int general_apply_precond(pp_plan* pl, const double* dr, double* dz);
 int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
 int general_true_residual(pp_plan* pl, const double* dx, double* rel);
 int general_pressure(pp_plan* pl, double q0, const double* patches, int np, float* F);
+
+// lifetime (pp_api.hip)
+void ctx_release(pp_ctx* c);
+void plan_release(pp_plan* pl);
 
 // shared helpers (pp_api.hip)
 int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel);

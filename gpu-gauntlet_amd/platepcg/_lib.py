@@ -60,6 +60,28 @@ class SolveStats(C.Structure):
                 ("sweep_residual", C.c_double * MAX_SWEEPS), ("phase_us", C.c_double * 6)]
 
 
+TRAIN_CURVE = 64
+TRAIN_SYNTHETIC, TRAIN_SOLVER = 0, 1
+
+
+class CnnConfig(C.Structure):
+    _fields_ = [("channels", C.c_int32 * 3), ("bottleneck_dilation", C.c_int32),
+                ("rotation_scale", C.c_double), ("smoother_weight", C.c_double),
+                ("relu", C.c_int32), ("fixed_step", C.c_int32)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("pairs", C.c_int32), ("epochs", C.c_int32), ("seed0", C.c_uint64),
+                ("source", C.c_int32), ("snapshots", C.c_int32),
+                ("lr", C.c_double), ("lr_final", C.c_double), ("max_grad_norm", C.c_double)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("curve_points", C.c_int32), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("seconds", C.c_double),
+                ("loss_curve", C.c_double * TRAIN_CURVE)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pp_device_count": (C.c_int, []),
@@ -80,6 +102,18 @@ SIGNATURES = {
     "pp_apply_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_get_block_inverse": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_pressure_load": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_void_p]),
+    "pp_cnn_create": (C.c_int, [C.c_void_p, C.POINTER(CnnConfig), C.c_uint64,
+                                C.POINTER(C.c_void_p)]),
+    "pp_cnn_destroy": (C.c_int, [C.c_void_p]),
+    "pp_cnn_param_count": (C.c_int64, [C.c_void_p]),
+    "pp_cnn_smoother_weight": (C.c_double, [C.c_void_p]),
+    "pp_cnn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pp_cnn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pp_cnn_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pp_cnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "pp_cnn_gen_error": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pp_cnn_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), C.POINTER(TrainStats)]),
+    "pp_plan_set_learned": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

@@ -47,6 +47,39 @@ class SplitMix64:
         return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
 
 
+def splitmix_normals(seed, n):
+    """n standard normals from the SplitMix64 stream of `seed` (Box-Muller on
+    consecutive pairs of 53-bit uniforms), vectorised; element i equals what
+    SplitMix64(seed) would draw as its i-th normal()."""
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.arange(1, 2 * n + 1, dtype=np.uint64) * gamma
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    u1 = np.maximum(u[0::2], 1e-300)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u[1::2])
+
+
+def gaussian_field(nx, ny, width, height, corr_len, seed):
+    """Unit-variance Gaussian random field on the element centres of an
+    nx x ny grid, squared-exponential covariance exp(-d^2 / (2 l^2)) with
+    l = corr_len, by spectral synthesis: white noise on a grid padded to
+    twice the size (no wrap-around correlation), filtered by the square root
+    of the spectral density exp(-l^2 |k|^2 / 2), cropped and normalised to
+    unit sample variance."""
+    hx, hy = width / nx, height / ny
+    Nx, Ny = 2 * nx, 2 * ny
+    w = splitmix_normals(seed, Nx * Ny).reshape(Ny, Nx)
+    kx = 2 * np.pi * np.fft.fftfreq(Nx, d=hx)
+    ky = 2 * np.pi * np.fft.fftfreq(Ny, d=hy)
+    K2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    g = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-0.25 * corr_len ** 2 * K2)))[:ny, :nx]
+    g -= g.mean()
+    return g / g.std()
+
+
 def rect_mesh(nx, ny=None, width=10.0, height=None):
     """Structured quad mesh; node coordinates are float32(i * (width / nx))."""
     ny = nx if ny is None else ny
@@ -173,6 +206,19 @@ def workload(name, seed=None):
         return dict(name=name, mesh=mesh, E=E_STEEL, nu=NU_STEEL, t=0.1,
                     mask=constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}]),
                     q0=10e3, patches=np.array([[50e3, cx, cy, 1.0]]))
+    if name == "c5":
+        # 1024 x 1024, t_e = 0.1 exp(0.5 g(x_e)), g unit-variance, correlation
+        # length 0.5 m (seed 5); 10 kPa + three Gaussian patches
+        n = 1024
+        mesh = rect_mesh(n)
+        g = gaussian_field(n, n, 10.0, 10.0, 0.5, 5 if seed is None else seed)
+        rng = SplitMix64(1005 if seed is None else seed + 1000)
+        patches = np.array([[rng.uniform(10e3, 50e3), rng.uniform(0, 10), rng.uniform(0, 10), 1.0]
+                            for _ in range(3)])
+        return dict(name=name, mesh=mesh, E=E_STEEL, nu=NU_STEEL, t=0.1,
+                    element_t=(0.1 * np.exp(0.5 * g)).astype(np.float32).ravel(),
+                    mask=constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}]),
+                    q0=10e3, patches=patches)
     if name == "c3":
         mesh = rect_mesh(700)
         return dict(name=name, mesh=mesh, E=E_STEEL, nu=NU_STEEL, t=0.05,

@@ -140,6 +140,77 @@ class Plan:
         return x64, x32, stats_dict(st)
 
 
+class Net:
+    """The learned preconditioner of one structured plan (pp_cnn_*)."""
+
+    def __init__(self, plan, seed=1, channels=(16, 32, 64), bottleneck_dilation=1,
+                 rotation_scale=0.3, smoother_weight=0.0, relu=False, fixed_step=False):
+        self.plan, self.lib = plan, plan.lib
+        cfg = _lib.CnnConfig((C.c_int32 * 3)(*channels), int(bottleneck_dilation),
+                             float(rotation_scale), float(smoother_weight), int(bool(relu)),
+                             int(bool(fixed_step)))
+        h = C.c_void_p()
+        check(self.lib.pp_cnn_create(plan.h, C.byref(cfg), int(seed), C.byref(h)))
+        self.h = h
+        self.n_params = int(self.lib.pp_cnn_param_count(h))
+        self.omega = float(self.lib.pp_cnn_smoother_weight(h))
+
+    def close(self):
+        if self.h:
+            self.lib.pp_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.empty(self.n_params, np.float32)
+        check(self.lib.pp_cnn_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        check(self.lib.pp_cnn_set_params(self.h, _ptr(p)))
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty_like(r)
+        check(self.lib.pp_cnn_apply(self.h, _ptr(r), _ptr(z)))
+        return z
+
+    def loss_grad(self, e):
+        e = np.ascontiguousarray(e, np.float64)
+        loss = C.c_double()
+        g = np.empty(self.n_params, np.float32)
+        check(self.lib.pp_cnn_loss_grad(self.h, _ptr(e), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def gen_error(self, seed):
+        e = np.empty(self.plan.n_dof, np.float64)
+        check(self.lib.pp_cnn_gen_error(self.h, int(seed), _ptr(e)))
+        return e
+
+    def train(self, pairs, epochs=1, seed0=1000, lr=1e-3, lr_final=0.0, max_grad_norm=0.0,
+              source=_lib.TRAIN_SYNTHETIC, snapshots=8):
+        o = _lib.TrainOpts(int(pairs), int(epochs), int(seed0), int(source), int(snapshots),
+                           float(lr), float(lr_final), float(max_grad_norm))
+        st = _lib.TrainStats()
+        check(self.lib.pp_cnn_train(self.h, C.byref(o), C.byref(st)))
+        return {"steps": st.steps, "loss_first": st.loss_first, "loss_last": st.loss_last,
+                "seconds": st.seconds,
+                "loss_curve": [st.loss_curve[i] for i in range(st.curve_points)]}
+
+    def attach(self):
+        check(self.lib.pp_plan_set_learned(self.plan.h, self.h))
+
+    def detach(self):
+        check(self.lib.pp_plan_set_learned(self.plan.h, None))
+
+
 def stats_dict(st):
     n = max(1, st.sweeps)
     return {

@@ -188,6 +188,80 @@ int pp_get_block_inverse(pp_plan* plan, double* blocks);
  * counterpart: the reference has only point loads (solver.ts:363-392). */
 int pp_pressure_load(pp_plan* plan, double q0, const double* patches, int n_patches, float* F);
 
+/* ------------------------------------------------------------------------
+ * Learned preconditioner (no reference counterpart: the reference has only
+ * block Jacobi, solver.ts:188-339).  A 3-level U-net on the nodal grid of a
+ * structured plan, BF16 MFMA with FP32 accumulation, trained on the device
+ * on pairs (r = K e, e) built from the plan's own operator, and used by
+ * pp_solve_resident with precond = PP_PRECOND_LEARNED inside flexible PCG.
+ * A network holds a reference to its plan; destroy it before or after the
+ * plan, either order is safe.
+ * ---------------------------------------------------------------------- */
+
+typedef struct pp_cnn pp_cnn;
+
+typedef struct {
+  int32_t channels[3];         /* per level; 0 = 16 / 32 / 64 */
+  int32_t bottleneck_dilation; /* dilation of the last coarse conv; 0 = 1 */
+  double rotation_scale;       /* input scale of thx/thy relative to w (metres); 0 = 0.3 */
+  double smoother_weight;      /* block-Jacobi smoothing weight; 0 = 4 / (3 lambda_max(B K)) */
+  int32_t relu;                /* 0: no activation and no biases, so the network is a fixed
+                                  linear operator (flexible PCG then converges like PCG);
+                                  1: ReLU after every conv but the last, with biases */
+  int32_t fixed_step;          /* 0: scale the network's correction by its energy-optimal
+                                  step in every application, so an application is never
+                                  worse than the smoothing alone; 1: unit step (with
+                                  relu = 0 the preconditioner is then a fixed linear map) */
+} pp_cnn_config;
+
+#define PP_TRAIN_CURVE 64
+
+#define PP_TRAIN_SYNTHETIC 0 /* e: seeded multi-scale smooth fields, r = K e */
+#define PP_TRAIN_SOLVER 1    /* e = x* - x_k: block-Jacobi PCG iterates on seeded pressure loads */
+
+typedef struct {
+  int32_t pairs;        /* synthetic: pairs (seeds seed0 ..); solver: load cases (seeds seed0 ..) */
+  int32_t epochs;
+  uint64_t seed0;
+  int32_t source;       /* PP_TRAIN_* */
+  int32_t snapshots;    /* solver: iterates per load case, k = 0 and geometric up to convergence */
+  double lr;            /* Adam learning rate (cosine-decayed to lr_final) */
+  double lr_final;      /* 0 = constant lr */
+  double max_grad_norm; /* global gradient-norm clip; 0 = none */
+} pp_train_opts;
+
+typedef struct {
+  int32_t steps;
+  int32_t curve_points;
+  double loss_first, loss_last; /* relative energy-norm error of the first / last pair */
+  double seconds;
+  double loss_curve[PP_TRAIN_CURVE]; /* mean loss over equal slices of the steps */
+} pp_train_stats;
+
+/* Build a network for a structured plan; weights He-initialised from seed. */
+int pp_cnn_create(pp_plan* plan, const pp_cnn_config* config, uint64_t seed, pp_cnn** out);
+int pp_cnn_destroy(pp_cnn* net);
+int64_t pp_cnn_param_count(const pp_cnn* net);
+/* FP32 parameters, per conv layer: weights [cout][3][3][cin] then bias [cout]. */
+int pp_cnn_get_params(pp_cnn* net, float* params);
+int pp_cnn_set_params(pp_cnn* net, const float* params);
+/* The smoothing weight w in use. */
+double pp_cnn_smoother_weight(const pp_cnn* net);
+/* z = M(r) for host vectors of 3 * node_count doubles: the preconditioner
+ * z0 = w B r; z1 = z0 + N(r - K z0); z = z1 + w B (r - K z1), B the 3x3
+ * block-Jacobi inverse, N the network (N(v) = B v + its learned correction). */
+int pp_cnn_apply(pp_cnn* net, const double* r, double* z);
+/* Loss (M(K e) - e)^T K (M(K e) - e) / e^T K e and its gradient in the
+ * parameters (grad may be NULL). */
+int pp_cnn_loss_grad(pp_cnn* net, const double* e, double* loss, float* grad);
+/* The training error field of one seed (host output, 3 * node_count). */
+int pp_cnn_gen_error(pp_cnn* net, uint64_t seed, double* e);
+/* Adam, one pair per step.  The solver source overwrites the plan's resident
+ * right-hand side and solution. */
+int pp_cnn_train(pp_cnn* net, const pp_train_opts* opts, pp_train_stats* stats);
+/* Attach (or detach with NULL) the preconditioner PP_PRECOND_LEARNED uses. */
+int pp_plan_set_learned(pp_plan* plan, pp_cnn* net);
+
 #ifdef __cplusplus
 }
 #endif
