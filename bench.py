@@ -32,8 +32,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"],
-                    help="c2: 142 x 142 FP64 (default); c3: 700 x 700 mixed precision")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2: 142 x 142 FP64 (default); c3: 700 x 700 mixed precision; "
+                         "c4: 256 batched Q4/DKT plates; c5: 1024 x 1024 variable thickness, "
+                         "learned preconditioner vs block Jacobi")
+    ap.add_argument("--validate", action="store_true",
+                    help="c4: compare every plate with the single-plate FP64 solve")
+    ap.add_argument("--train-loads", type=int, default=8,
+                    help="c5: load cases for the solver-generated training pairs")
+    ap.add_argument("--train-epochs", type=int, default=3, help="c5: training epochs")
     ap.add_argument("--cpu-sample-iters", type=int, default=-1,
                     help="iterations of the CPU baseline sample (0 = skip; default 1200, "
                          "150 for c3)")
@@ -75,6 +82,172 @@ def cpu_baseline(w, F, full_iters, sample_iters, basis="iterations it needs to r
     }
 
 
+def js_ms_per_dof_iteration(n=64, iters=300):
+    """Float32 solver.ts restatement in Node: ms per PCG iteration per DOF on
+    an n x n clamped Q4 plate (one thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    import numpy as np
+
+    from platepcg import problem as P
+    mesh = P.rect_mesh(n)
+    op = O.OracleProblem(mesh, 210e9, 0.3, 0.1, P.constrained_mask(
+        mesh, [{"type": "fixed", "location": "all_edges"}]))
+    F = op.pressure_load(1e4).astype(np.float32)
+    _, st = O.run_js_solver(n, F, 210e9, 0.3, 0.1, [{"type": "fixed", "location": "all_edges"}],
+                            tol=1e-30, max_iter=iters)
+    return st["solveTimeMs"] / max(1, st["iterations"]) / (3 * (n + 1) ** 2), st
+
+
+def bench_batch(a):
+    """256 plates (Q4 and DKT, random sizes, materials, supports and loads):
+    one autotuned persistent launch per step, every plate to 1e-8 (FP64 true
+    residual, restarts included)."""
+    import numpy as np
+
+    from platepcg import Batch, Context, Plan
+    from platepcg import problem as P
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    plates = P.batch_plates()
+    ctx = Context(0)
+    b = Batch(ctx, plates)
+    Fs = []
+    for i, p in enumerate(plates):
+        F = b.pressure_load(i, p["q0"], p["patches"])
+        b.set_rhs(i, F)
+        Fs.append(F)
+    t0 = time.perf_counter()
+    sched, tuned, trials = b.autotune(tolerance=1e-8)
+    tune_s = time.perf_counter() - t0
+    for _ in range(a.warmup):
+        b.solve(tolerance=1e-8)
+    dev, t0 = [], time.perf_counter()
+    for _ in range(a.steps):
+        st = b.solve(tolerance=1e-8)
+        dev.append(st["device_ms"])
+    wall = (time.perf_counter() - t0) * 1e3 / a.steps
+    its = np.array([b.result(i)[1]["iterations"] for i in range(len(plates))])
+    dofs = np.array(b.n_dof)
+    dof_its = float(np.sum(its * dofs))
+    kern_ms = float(np.mean(dev))
+    achieved = BYTES_PER_DOF_ITER_FP64 * dof_its / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": "ms per batch of 256 plates to 1e-8 rel. residual (Q4 + DKT, FP64 block-Jacobi PCG)",
+        "value": wall, "unit": "ms/batch", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": wall, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c4: 256 plates, n log-uniform in [16, 200], alternating Q4 / "
+                   "split-quad DKT, t U[0.01, 0.3], nu U[0.25, 0.35], edges fixed/pinned, "
+                   "10 kPa + 1-3 Gaussian patches (seed 4)",
+                   "dof": int(dofs.sum()), "plates": len(plates),
+                   "q4_plates": sum(p["kind"] == "q4" for p in plates),
+                   "max_iterations": int(its.max()), "total_iterations": int(its.sum()),
+                   "precond": "block-jacobi-3x3", "precision": "fp64"},
+        "device_ms_mean": kern_ms, "worst_true_rel_residual": st["worst_true_rel_residual"],
+        "converged": st["converged"],
+        "autotune": {"trials": trials, "seconds": tune_s,
+                     "schedule": {k: sched[k] for k in ("workgroups_per_cu", "tile_shape",
+                                                        "balance_iterations")},
+                     "workgroups": st["workgroups"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_pcg (persistent, all plates in one launch)",
+                     "algorithmic_bytes_per_launch": BYTES_PER_DOF_ITER_FP64 * dof_its},
+    }
+    if a.validate:
+        worst = 0.0
+        import oracle_api as O
+        for i, p in enumerate(plates):
+            col = P.colour_split_quads(p["mesh"]) if p["kind"] == "dkt" else None
+            plan = Plan(ctx, p["mesh"], p["E"], p["nu"], p["t"], p["mask"], col)
+            x1, _, _ = plan.solve(Fs[i], tolerance=1e-8, max_iterations=400000)
+            worst = max(worst, O.max_nodal_w_error(b.result(i)[0], x1))
+            plan.close()
+        out["validation"] = {"reference": "single-plate FP64 solve of each plate",
+                             "worst_max_nodal_w_error": worst}
+    if a.cpu_sample_iters != 0:
+        try:
+            per, st_js = js_ms_per_dof_iteration()
+            out["cpu_baseline"] = {
+                "value": per * dof_its, "unit": "ms/batch", "cores": 1, "kind": "port",
+                "sample": (f"float32 solver.ts restatement in Node {st_js['node']} on "
+                           f"{st_js['cpuModel']}: 300 PCG iterations of a 64 x 64 Q4 plate timed, "
+                           f"{per * 1e6:.3f} ns per DOF-iteration, extrapolated x the batch's "
+                           f"{dof_its:.3g} DOF-iterations (extrapolation)")}
+        except (OSError, subprocess.CalledProcessError) as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+    b.close()
+    ctx.close()
+
+
+def bench_learned(a):
+    """1024 x 1024 plate with a log-normal thickness field: block-Jacobi PCG
+    against flexible PCG with the learned preconditioner, both to 1e-8 true
+    relative residual.  The network is trained first (not timed) on pairs the
+    solver generates on independent load cases."""
+    import numpy as np
+
+    from platepcg import Context, Net, Plan, _lib
+    from platepcg import problem as P
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    w = P.workload("c5")
+    ctx = Context(0)
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"], element_t=w["element_t"])
+    net = Net(plan, seed=5)
+    t0 = time.perf_counter()
+    tr = net.train(pairs=a.train_loads, snapshots=8, epochs=a.train_epochs, seed0=1000,
+                   lr=1e-3, lr_final=1e-5, max_grad_norm=1.0, source=_lib.TRAIN_SOLVER)
+    train_s = time.perf_counter() - t0
+    F = plan.pressure_load(w["q0"], w["patches"])
+    plan.set_rhs(F)
+    bj = plan.options(tolerance=1e-8, max_iterations=1000000)
+    lo = plan.options(tolerance=1e-8, max_iterations=100000, precond=_lib.PRECOND_LEARNED)
+    net.attach()
+    res = {}
+    for tag, o in (("block_jacobi", bj), ("learned", lo)):
+        for _ in range(a.warmup):
+            plan.solve_resident(o)
+        t1, dev = time.perf_counter(), []
+        for _ in range(a.steps):
+            st = plan.solve_resident(o)
+            dev.append(st.device_ms)
+        res[tag] = {"ms": (time.perf_counter() - t1) * 1e3 / a.steps, "device_ms": float(np.mean(dev)),
+                    "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
+                    "converged": bool(st.converged)}
+        res[tag]["x"] = plan.solution()[0]
+    err = O.max_nodal_w_error(res["learned"].pop("x"), res["block_jacobi"].pop("x"))
+    n_dof = plan.n_dof
+    it_bj = res["block_jacobi"]["iterations"]
+    achieved = BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj / (res["block_jacobi"]["device_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "ms/solve to 1e-8 rel. residual (3.15M DOF, variable thickness): learned "
+                  "preconditioner in flexible PCG vs block-Jacobi PCG",
+        "value": res["learned"]["ms"], "unit": "ms/solve", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": res["learned"]["ms"], "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64+bf16", "data": "synthetic",
+        "config": {"workload": "c5: 1024 x 1024 Q4 plate, t_e = 0.1 exp(0.5 g), g unit Gaussian "
+                   "field with correlation length 0.5 m (seed 5), clamped, 10 kPa + 3 Gaussian "
+                   "patches", "dof": n_dof, "precond": "learned U-net (16/32/64, BF16 MFMA) + "
+                   "block-Jacobi smoothing, flexible PCG", "precision": "fp64 outer, bf16 network"},
+        "learned": res["learned"], "block_jacobi": res["block_jacobi"],
+        "speedup_vs_block_jacobi": res["block_jacobi"]["ms"] / res["learned"]["ms"],
+        "iteration_ratio": it_bj / max(1, res["learned"]["iterations"]),
+        "max_nodal_w_error_learned_vs_bj": err,
+        "training": {"pairs": a.train_loads * 8, "epochs": a.train_epochs, "seconds": train_s,
+                     "loss_first": tr["loss_first"], "loss_last": tr["loss_last"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pcg (block-Jacobi FP64 PCG, tiles strided; the comparison leg)",
+                     "algorithmic_bytes_per_launch": BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj},
+    }
+    print(json.dumps(out))
+    net.close()
+    plan.close()
+    ctx.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -82,6 +255,12 @@ def main():
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.workload == "c4":
+        bench_batch(a)
+        return
+    if a.workload == "c5":
+        bench_learned(a)
         return
     import numpy as np
 

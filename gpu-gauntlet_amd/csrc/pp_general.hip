@@ -513,9 +513,11 @@ int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) 
 }
 
 // Consistent pressure load on a general mesh, on the host (set-up work).
-int general_pressure(pp_plan* pl, double q0, const double* pt, int np, float* F) {
-  const int npe = pl->npe;
-  std::vector<double> f((size_t)pl->n_dof, 0.0);
+// Consistent nodal pressure load of a mesh (host, FP64 sums, float out).
+int mesh_pressure(const float* h_nodes, const uint32_t* h_elems, int npe, int64_t n_nodes,
+                  int64_t n_elems, const uint8_t* h_cmask, double q0, const double* pt, int np,
+                  float* F) {
+  std::vector<double> f((size_t)(3 * n_nodes), 0.0);
   auto pres = [&](double x, double y) {
     double p = q0;
     for (int k = 0; k < np; k++) {
@@ -524,13 +526,13 @@ int general_pressure(pp_plan* pl, double q0, const double* pt, int np, float* F)
     }
     return p;
   };
-  for (int64_t e = 0; e < pl->n_elems; e++) {
+  for (int64_t e = 0; e < n_elems; e++) {
     double xy[8];
     uint32_t nodes[4];
     for (int i = 0; i < npe; i++) {
-      nodes[i] = pl->h_elems[(size_t)e * npe + i];
-      xy[2 * i] = pl->h_nodes[2 * (size_t)nodes[i]];
-      xy[2 * i + 1] = pl->h_nodes[2 * (size_t)nodes[i] + 1];
+      nodes[i] = h_elems[(size_t)e * npe + i];
+      xy[2 * i] = h_nodes[2 * (size_t)nodes[i]];
+      xy[2 * i + 1] = h_nodes[2 * (size_t)nodes[i] + 1];
     }
     if (npe == 4) {
       const double g = 0.5773502691896257;
@@ -563,10 +565,23 @@ int general_pressure(pp_plan* pl, double q0, const double* pt, int np, float* F)
       }
     }
   }
-  for (int64_t n = 0; n < pl->n_nodes; n++)
+  for (int64_t n = 0; n < n_nodes; n++)
     for (int d = 0; d < 3; d++)
-      F[3 * n + d] = (d == 0 && !(pl->h_cmask[n] & PP_FIX_W)) ? (float)f[3 * n] : 0.0f;
+      F[3 * n + d] = (d == 0 && !(h_cmask[n] & PP_FIX_W)) ? (float)f[3 * n] : 0.0f;
   return PP_OK;
+}
+
+int general_pressure(pp_plan* pl, double q0, const double* pt, int np, float* F) {
+  return mesh_pressure(pl->h_nodes.data(), pl->h_elems.data(), pl->npe, pl->n_nodes, pl->n_elems,
+                       pl->h_cmask.data(), q0, pt, np, F);
+}
+
+// Element stiffness of one element (host, FP64), for the batched path.
+void element_ke_q4(const double* xy, double E, double nu, double t, double* Ke) {
+  ke_q4(xy, E, nu, t, Ke);
+}
+void element_ke_dkt(const double* xy, double E, double nu, double t, double* Ke) {
+  ke_dkt(xy, E, nu, t, Ke);
 }
 
 }  // namespace pp

@@ -82,6 +82,18 @@ class TrainStats(C.Structure):
                 ("loss_curve", C.c_double * TRAIN_CURVE)]
 
 
+class BatchSchedule(C.Structure):
+    _fields_ = [("workgroups_per_cu", C.c_int32), ("tile_shape", C.c_int32),
+                ("balance_iterations", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("plates", C.c_int32), ("converged", C.c_int32), ("workgroups", C.c_int32),
+                ("max_iterations", C.c_int32), ("total_iterations", C.c_int64),
+                ("total_dof", C.c_int64), ("device_ms", C.c_double),
+                ("worst_true_rel_residual", C.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pp_device_count": (C.c_int, []),
@@ -114,6 +126,20 @@ SIGNATURES = {
     "pp_cnn_gen_error": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "pp_cnn_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), C.POINTER(TrainStats)]),
     "pp_plan_set_learned": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pp_batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Mesh), C.POINTER(Material),
+                                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "pp_batch_destroy": (C.c_int, [C.c_void_p]),
+    "pp_batch_plate_count": (C.c_int32, [C.c_void_p]),
+    "pp_batch_pressure_load": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
+                                         C.c_int32, C.c_void_p]),
+    "pp_batch_set_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "pp_batch_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule)]),
+    "pp_batch_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule), C.c_void_p]),
+    "pp_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(BatchSchedule),
+                                    C.POINTER(BatchStats), C.POINTER(C.c_int32)]),
+    "pp_batch_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(BatchStats)]),
+    "pp_batch_plate_result": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.POINTER(SolveStats)]),
 }
 
 _lib = None

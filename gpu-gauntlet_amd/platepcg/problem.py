@@ -123,6 +123,17 @@ def colour_structured(mesh):
     return _pack_colours(colors, 4)
 
 
+def colour_split_quads(mesh):
+    """Split-quad triangles (2 per cell, interleaved): 8 colours
+    c = ((i % 2) * 2 + (j % 2)) * 2 + triangle; triangles of one colour
+    share no node (same-parity cells are two cells apart)."""
+    e = np.arange(mesh["elementCount"])
+    cell, tri = e // 2, e % 2
+    i, j = cell % mesh["gridNx"], cell // mesh["gridNx"]
+    colors = (((i % 2) * 2 + (j % 2)) * 2 + tri).astype(np.uint8)
+    return _pack_colours(colors, 8)
+
+
 def colour_greedy(mesh):
     """Smallest colour not used by an element sharing a node, in element order."""
     npe = mesh["nodesPerElement"]
@@ -190,6 +201,34 @@ def dofs_to_mask(n_nodes, dofs):
 
 
 # ------------------------------------------------------------- workloads
+
+def batch_plates(count=256, seed=4, n_min=16, n_max=200):
+    """The batched workload: `count` 10 m x 10 m plates from one SplitMix64
+    stream.  Per plate, in draw order: side n = floor(exp(U(ln n_min,
+    ln(n_max + 1)))) clamped to [n_min, n_max]; t ~ U[0.01, 0.3] m;
+    nu ~ U[0.25, 0.35]; each edge (bottom, right, top, left) fixed or pinned
+    with probability 1/2; k in {1, 2, 3} Gaussian patches (peak U[10, 50] kPa,
+    sigma 1 m, centre uniform) on 10 kPa.  Even plates are Q4, odd plates DKT
+    (each quad split into (n0,n1,n2), (n0,n2,n3))."""
+    rng = SplitMix64(seed)
+    plates = []
+    for i in range(count):
+        n = int(math.floor(math.exp(rng.uniform(math.log(n_min), math.log(n_max + 1)))))
+        n = min(max(n, n_min), n_max)
+        t = rng.uniform(0.01, 0.3)
+        nu = rng.uniform(0.25, 0.35)
+        edges = ["fixed" if rng.uniform() < 0.5 else "pinned" for _ in range(4)]
+        k = 1 + int(rng.uniform() * 3)
+        patches = np.array([[rng.uniform(10e3, 50e3), rng.uniform(0, 10), rng.uniform(0, 10), 1.0]
+                            for _ in range(k)])
+        mesh = rect_mesh(n)
+        if i % 2:
+            mesh = split_quads(mesh)
+        plates.append(dict(mesh=mesh, E=E_STEEL, nu=nu, t=t, mask=edge_mask(n, n, edges),
+                           edges=edges, q0=10e3, patches=patches, kind="dkt" if i % 2 else "q4",
+                           n=n))
+    return plates
+
 
 def workload(name, seed=None):
     """The named synthetic workloads.  Returns a dict with mesh, material,

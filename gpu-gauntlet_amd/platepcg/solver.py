@@ -211,6 +211,97 @@ class Net:
         check(self.lib.pp_plan_set_learned(self.plan.h, None))
 
 
+class Batch:
+    """Many rectangular-grid plates (Q4 or split-quad DKT) solved by one
+    persistent launch (pp_batch_*).  plates: list of dicts with mesh, E, nu,
+    t, mask."""
+
+    def __init__(self, ctx, plates):
+        self.ctx, self.lib = ctx, ctx.lib
+        n = len(plates)
+        meshes = (_lib.Mesh * n)()
+        mats = (_lib.Material * n)()
+        masks = (C.c_void_p * n)()
+        self._keep = []
+        self.n_dof = []
+        for i, p in enumerate(plates):
+            m = p["mesh"]
+            nodes = np.ascontiguousarray(m["nodes"], np.float32)
+            elems = np.ascontiguousarray(m["elements"], np.uint32)
+            mask = np.ascontiguousarray(p["mask"], np.uint8)
+            self._keep += [nodes, elems, mask]
+            meshes[i] = _lib.Mesh(nodes.ctypes.data, elems.ctypes.data, int(m["nodeCount"]),
+                                  int(m["elementCount"]), int(m.get("nodesPerElement", 4)),
+                                  int(m.get("gridNx", 0)), int(m.get("gridNy", 0)))
+            mats[i] = _lib.Material(float(p["E"]), float(p["nu"]), float(p["t"]), None)
+            masks[i] = mask.ctypes.data
+            self.n_dof.append(3 * int(m["nodeCount"]))
+        h = C.c_void_p()
+        check(self.lib.pp_batch_create(ctx.h, n, meshes, mats, masks, C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if self.h:
+            self.lib.pp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pressure_load(self, plate, q0, patches=None):
+        pt = np.ascontiguousarray(np.zeros((0, 4)) if patches is None else patches, np.float64)
+        F = np.empty(self.n_dof[plate], np.float32)
+        check(self.lib.pp_batch_pressure_load(self.h, plate, float(q0), _ptr(pt) if pt.size else None,
+                                              int(pt.shape[0]), _ptr(F)))
+        return F
+
+    def set_rhs(self, plate, F):
+        F = np.ascontiguousarray(F, np.float32)
+        if 0 <= plate < self.n and F.size != self.n_dof[plate]:
+            raise ValueError(f"plate {plate} needs {self.n_dof[plate]} load values")
+        check(self.lib.pp_batch_set_rhs(self.h, int(plate), _ptr(F)))
+
+    def schedule(self):
+        s = _lib.BatchSchedule()
+        wg = np.empty(self.n, np.int32)
+        check(self.lib.pp_batch_get_schedule(self.h, C.byref(s), _ptr(wg)))
+        return {"workgroups_per_cu": s.workgroups_per_cu, "tile_shape": s.tile_shape,
+                "balance_iterations": s.balance_iterations, "wg_per_plate": wg}
+
+    def set_schedule(self, workgroups_per_cu=2, tile_shape=0, balance_iterations=0):
+        s = _lib.BatchSchedule(workgroups_per_cu, tile_shape, balance_iterations)
+        check(self.lib.pp_batch_set_schedule(self.h, C.byref(s)))
+
+    @staticmethod
+    def _stats(st):
+        return {k: getattr(st, k) for k, _ in _lib.BatchStats._fields_}
+
+    def autotune(self, tolerance=1e-8, max_iterations=200000):
+        o = Plan.options(tolerance=tolerance, max_iterations=max_iterations)
+        best = _lib.BatchSchedule()
+        st = _lib.BatchStats()
+        trials = C.c_int32()
+        check(self.lib.pp_batch_autotune(self.h, C.byref(o), C.byref(best), C.byref(st),
+                                         C.byref(trials)))
+        return self.schedule(), self._stats(st), trials.value
+
+    def solve(self, tolerance=1e-8, max_iterations=200000, flags=0):
+        o = Plan.options(tolerance=tolerance, max_iterations=max_iterations, flags=flags)
+        st = _lib.BatchStats()
+        check(self.lib.pp_batch_solve(self.h, C.byref(o), C.byref(st)))
+        return self._stats(st)
+
+    def result(self, plate):
+        x64 = np.empty(self.n_dof[plate], np.float64)
+        st = _lib.SolveStats()
+        check(self.lib.pp_batch_plate_result(self.h, plate, _ptr(x64), None, C.byref(st)))
+        return x64, stats_dict(st)
+
+
 def stats_dict(st):
     n = max(1, st.sweeps)
     return {

@@ -262,6 +262,56 @@ int pp_cnn_train(pp_cnn* net, const pp_train_opts* opts, pp_train_stats* stats);
 /* Attach (or detach with NULL) the preconditioner PP_PRECOND_LEARNED uses. */
 int pp_plan_set_learned(pp_plan* plan, pp_cnn* net);
 
+/* ------------------------------------------------------------------------
+ * Batched plates (no reference counterpart: the reference solves one plate
+ * per solveGPU call).  Many rectangular-grid plates, Q4 (grid mesh) or DKT
+ * (each cell split into (n0,n1,n2), (n0,n2,n3)), each with its own material,
+ * supports and load, solved to the same tolerance by one persistent launch;
+ * the FP64 block-Jacobi PCG of each plate as in pp_solve_resident.  Meshes
+ * must carry grid_nx / grid_ny; uniform thickness per plate.
+ * ---------------------------------------------------------------------- */
+
+typedef struct pp_batch pp_batch;
+
+typedef struct {
+  int32_t workgroups_per_cu;  /* resident workgroups per CU of the persistent launch */
+  int32_t tile_shape;         /* 0: 15x15-node tiles, 1: 31x7, 2: 7x31,
+                                 3: per plate the shape with the fewest tiles */
+  int32_t balance_iterations; /* 1: deal workgroups by measured iterations x tiles,
+                                 0: by tiles only */
+} pp_batch_schedule;
+
+typedef struct {
+  int32_t plates, converged, workgroups, max_iterations;
+  int64_t total_iterations, total_dof;
+  double device_ms;               /* HIP-event time of the persistent launch */
+  double worst_true_rel_residual; /* max over plates of |F - K x| / |F| (FP64) */
+} pp_batch_stats;
+
+int pp_batch_create(pp_ctx* ctx, int32_t n_plates, const pp_mesh* meshes,
+                    const pp_material* materials, const uint8_t* const* constrained_masks,
+                    pp_batch** out);
+int pp_batch_destroy(pp_batch* batch);
+int32_t pp_batch_plate_count(const pp_batch* batch);
+/* Consistent nodal pressure load of one plate, as pp_pressure_load. */
+int pp_batch_pressure_load(pp_batch* batch, int32_t plate, double q0, const double* patches,
+                           int32_t n_patches, float* F);
+/* F of one plate (3 * node_count floats). */
+int pp_batch_set_rhs(pp_batch* batch, int32_t plate, const float* F);
+int pp_batch_set_schedule(pp_batch* batch, const pp_batch_schedule* schedule);
+/* The schedule in use; wg_per_plate (may be NULL) receives each plate's workgroups. */
+int pp_batch_get_schedule(const pp_batch* batch, pp_batch_schedule* schedule, int32_t* wg_per_plate);
+/* Autotuner: measures every plate's iteration count once, then times each
+ * candidate schedule (workgroups per CU x tile policy x balancing) on the
+ * resident loads and keeps the fastest (also left applied). */
+int pp_batch_autotune(pp_batch* batch, const pp_solve_opts* opts, pp_batch_schedule* best,
+                      pp_batch_stats* best_stats, int32_t* trials);
+/* One persistent launch over every plate (opts: tolerance, max_iterations, flags). */
+int pp_batch_solve(pp_batch* batch, const pp_solve_opts* opts, pp_batch_stats* stats);
+/* Solution and statistics of one plate after pp_batch_solve. */
+int pp_batch_plate_result(pp_batch* batch, int32_t plate, double* x64, float* x32,
+                          pp_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "platepcg.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(pp_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|double|const char\*)\s+(pp_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

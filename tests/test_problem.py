@@ -42,3 +42,15 @@ def test_c5_workload_shape():
     assert t.shape == (1024 * 1024,) and t.dtype == np.float32
     assert abs(np.median(t) - 0.1) < 0.005
     assert abs(np.std(np.log(t / 0.1)) - 0.5) < 1e-3
+
+
+def test_split_quad_colouring_is_valid():
+    mesh = P.split_quads(P.rect_mesh(9, 7))
+    col = P.colour_split_quads(mesh)
+    el = mesh["elements"].reshape(-1, 3)
+    off = col["colour_offsets"]
+    assert off[-1] == mesh["elementCount"]
+    for c in range(col["colour_count"]):
+        members = col["elements_by_colour"][off[c]:off[c + 1]]
+        nodes = el[members].ravel()
+        assert len(np.unique(nodes)) == nodes.size, c
