@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--train-loads", type=int, default=8,
                     help="c5: load cases for the solver-generated training pairs")
     ap.add_argument("--train-epochs", type=int, default=3, help="c5: training epochs")
+    ap.add_argument("--learned-max-iterations", type=int, default=40000,
+                    help="c5: iteration cap of the learned-preconditioner solve")
     ap.add_argument("--cpu-sample-iters", type=int, default=-1,
                     help="iterations of the CPU baseline sample (0 = skip; default 1200, "
                          "150 for c3)")
@@ -197,13 +199,18 @@ def bench_learned(a):
     plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"], element_t=w["element_t"])
     net = Net(plan, seed=5)
     t0 = time.perf_counter()
-    tr = net.train(pairs=a.train_loads, snapshots=8, epochs=a.train_epochs, seed0=1000,
-                   lr=1e-3, lr_final=1e-5, max_grad_norm=1.0, source=_lib.TRAIN_SOLVER)
+    tr = None
+    for k in range(a.train_loads):  # one load case per call: progress stays visible
+        tr = net.train(pairs=1, snapshots=8, epochs=a.train_epochs, seed0=1000 + k,
+                       lr=1e-3, lr_final=1e-5, max_grad_norm=1.0, source=_lib.TRAIN_SOLVER)
+        print(f"training: load case {k + 1}/{a.train_loads} done, loss {tr['loss_first']:.3f} -> "
+              f"{tr['loss_last']:.3f}, {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     train_s = time.perf_counter() - t0
     F = plan.pressure_load(w["q0"], w["patches"])
     plan.set_rhs(F)
     bj = plan.options(tolerance=1e-8, max_iterations=1000000)
-    lo = plan.options(tolerance=1e-8, max_iterations=100000, precond=_lib.PRECOND_LEARNED)
+    lo = plan.options(tolerance=1e-8, max_iterations=a.learned_max_iterations,
+                      precond=_lib.PRECOND_LEARNED)
     net.attach()
     res = {}
     for tag, o in (("block_jacobi", bj), ("learned", lo)):
@@ -216,6 +223,8 @@ def bench_learned(a):
         res[tag] = {"ms": (time.perf_counter() - t1) * 1e3 / a.steps, "device_ms": float(np.mean(dev)),
                     "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
                     "converged": bool(st.converged)}
+        print(f"{tag}: {st.iterations} iterations, {res[tag]['ms']:.0f} ms", file=sys.stderr,
+              flush=True)
         res[tag]["x"] = plan.solution()[0]
     err = O.max_nodal_w_error(res["learned"].pop("x"), res["block_jacobi"].pop("x"))
     n_dof = plan.n_dof

@@ -872,20 +872,22 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
   double best_ms = 1e300;
   int n_trials = 0;
   pp_batch_schedule cand;
-  for (int wpc = 1; wpc <= std::min(4, b->per_cu_max); wpc++)
+  // most workgroups first (usually fastest, so slow candidates are cut short)
+  for (int wpc = std::min(4, b->per_cu_max); wpc >= 1; wpc--)
     for (int shape = 0; shape <= kShapes; shape += kShapes)  // square tiles, or per-plate fewest tiles
-      for (int bal = 0; bal <= 1; bal++) {
+      for (int bal = 1; bal >= 0; bal--) {
         cand.workgroups_per_cu = wpc;
         cand.tile_shape = shape;
         cand.balance_iterations = bal;
         if ((int)b->P.size() > wpc * b->ctx->cu_count && wpc < b->per_cu_max) continue;
         for (size_t p = 0; p < b->P.size(); p++) b->P[p].last_iterations = its[p];
         if ((rc = apply_schedule(b, &cand))) return rc;
-        // best of two runs
+        // a second run only for candidates within 10% of the best so far
         double ms = 1e300;
         for (int k = 0; k < 2; k++) {
           if ((rc = batch_run(b, &q, &s))) return rc;
           ms = std::min(ms, (double)s.device_ms);
+          if (ms > 1.1 * best_ms) break;
         }
         n_trials++;
         if (ms < best_ms) {
