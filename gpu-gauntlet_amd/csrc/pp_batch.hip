@@ -869,7 +869,7 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
   if ((rc = apply_schedule(b, &base)) || (rc = batch_run(b, &q, &s))) return rc;
   std::vector<int> its(b->P.size());
   for (size_t p = 0; p < b->P.size(); p++) its[p] = b->P[p].last_iterations;
-  double best_ms = 1e300;
+  double best_ms = 1e300, best_pred = 1e300;
   int n_trials = 0;
   pp_batch_schedule cand;
   // most workgroups first (usually fastest, so slow candidates are cut short)
@@ -882,6 +882,15 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
         if ((int)b->P.size() > wpc * b->ctx->cu_count && wpc < b->per_cu_max) continue;
         for (size_t p = 0; p < b->P.size(); p++) b->P[p].last_iterations = its[p];
         if ((rc = apply_schedule(b, &cand))) return rc;
+        // model: the slowest plate's iterations x tile sweeps per workgroup;
+        // candidates predicted 1.5x slower than the best one measured are skipped
+        double pred = 0;
+        for (size_t p = 0; p < b->P.size(); p++) {
+          const BatchPlate& bp = b->P[p];
+          const int tl = ceil_div(bp.nx + 1, kShapeTX[bp.shape]) * ceil_div(bp.ny + 1, kShapeTY[bp.shape]);
+          pred = std::max(pred, (double)its[p] * ceil_div(tl, bp.nwg));
+        }
+        if (pred > 1.5 * best_pred) continue;
         // a second run only for candidates within 10% of the best so far
         double ms = 1e300;
         for (int k = 0; k < 2; k++) {
@@ -892,6 +901,7 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
         n_trials++;
         if (ms < best_ms) {
           best_ms = ms;
+          best_pred = pred;
           *best = cand;
           if (best_stats) {
             *best_stats = s;

@@ -75,7 +75,7 @@ def test_autotune_keeps_the_answer_and_is_not_slower(ctx):
     sched, best, trials = b.autotune(tolerance=1e-8)
     print("autotune", trials, "trials:", {k: sched[k] for k in ("workgroups_per_cu", "tile_shape",
           "balance_iterations")}, "%.2f ms (default %.2f ms)" % (best["device_ms"], st0["device_ms"]))
-    assert trials >= 4
+    assert trials >= 2
     assert sched["wg_per_plate"].min() >= 1
     assert best["device_ms"] <= 1.05 * st0["device_ms"]
     st1 = b.solve(tolerance=1e-8)
