@@ -241,8 +241,11 @@ def bench_learned(a):
                    "patches", "dof": n_dof, "precond": "learned U-net (16/32/64, BF16 MFMA) + "
                    "block-Jacobi smoothing, flexible PCG", "precision": "fp64 outer, bf16 network"},
         "learned": res["learned"], "block_jacobi": res["block_jacobi"],
-        "speedup_vs_block_jacobi": res["block_jacobi"]["ms"] / res["learned"]["ms"],
-        "iteration_ratio": it_bj / max(1, res["learned"]["iterations"]),
+        # only meaningful when the learned solve reached the tolerance
+        "speedup_vs_block_jacobi": (res["block_jacobi"]["ms"] / res["learned"]["ms"]
+                                    if res["learned"]["converged"] else None),
+        "iteration_ratio": (it_bj / max(1, res["learned"]["iterations"])
+                            if res["learned"]["converged"] else None),
         "max_nodal_w_error_learned_vs_bj": err,
         "training": {"pairs": a.train_loads * 8, "epochs": a.train_epochs, "seconds": train_s,
                      "loss_first": tr["loss_first"], "loss_last": tr["loss_last"]},

@@ -102,7 +102,12 @@ __device__ __forceinline__ double untag_word(unsigned long long w) {
   return __longlong_as_double((long long)(w & ~0xffull));
 }
 
-template <int NV>
+//
+// Polling: lane l reads slots l, l + 64, ... in rounds of U loads that are
+// all in flight before any is consumed, so a poll of nwg slots costs
+// ceil(nwg / (64 U)) coherent round trips instead of ceil(nwg / 64); each
+// lane still adds its slots in increasing order (same bits as one by one).
+template <int NV, int U = 4>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* part, GridSync* gs,
                                                unsigned& epoch, double* lds, unsigned nwg,
                                                unsigned me) {
@@ -123,14 +128,25 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* part, Gr
       int cur = 1;
 #pragma unroll
       for (int k = 0; k < NV; k++) s[k] = 0;
-      for (unsigned i = threadIdx.x; i < nwg; i += 64)
+      for (unsigned base = 0; base < nwg; base += 64 * U) {
+        unsigned long long w[U][NV];
 #pragma unroll
-        for (int k = 0; k < NV; k++) {
-          const unsigned long long w =
-              __hip_atomic_load(&slot[k * nwg + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          cur &= (unsigned)(w & 0xffu) == tag;
-          s[k] += untag_word(w);
+        for (int j = 0; j < U; j++) {
+          const unsigned i = base + threadIdx.x + 64 * j;
+#pragma unroll
+          for (int k = 0; k < NV; k++)
+            w[j][k] = i < nwg ? __hip_atomic_load(&slot[k * nwg + i], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : (unsigned long long)tag;
         }
+#pragma unroll
+        for (int j = 0; j < U; j++)
+#pragma unroll
+          for (int k = 0; k < NV; k++) {
+            cur &= (unsigned)(w[j][k] & 0xffu) == tag;
+            s[k] += untag_word(w[j][k]);
+          }
+      }
       if (__all(cur)) break;
       if (++spins > kSpinLimit ||
           __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
