@@ -6,11 +6,14 @@
 // preconditioner is 6 doubles per node (upper triangle of the symmetric 3x3
 // inverse); the geometry is two small arrays of column widths / row heights.
 //
-// One persistent PCG kernel body, three launch shapes:
-//  - up to 64 tiles of 31x31 nodes (e.g. the 61k-DOF plate): one
-//    1024-thread workgroup per tile, per-node state in registers;
-//  - 15x15-node tiles, one 256-thread workgroup each, state in registers;
-//  - 15x15-node tiles strided over a resident grid, state in HBM.
+// Persistent PCG kernels and their launch shapes:
+//  - default while the plate is resident (the 61k-DOF plate: 64 tiles):
+//    single-reduction PCG, one 384-thread workgroup per 18x18-node tile,
+//    per-node state in registers;
+//  - the recurrence order of pcg.ts (two reductions), resident on up to 64
+//    tiles of 31x31 nodes or on 15x15-node tiles, state in registers;
+//  - larger plates: 18x18- or 23x23-node tiles strided over one workgroup
+//    per CU, state in HBM.
 
 #include <math.h>
 
@@ -205,6 +208,8 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
   const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
   const bool node_thread = threadIdx.x < C::TX * C::TY;
   unsigned epoch = 0;
+  const bool timing = me == 0 && threadIdx.x == 0;
+  unsigned long long tk[4] = {0, 0, 0, 0}, t0 = 0, t1 = 0, t2 = 0, t3 = 0;
   auto allreduce = [&](auto& v) -> bool {
     publish();  // every thread's stores complete before the arrival
     block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
@@ -267,6 +272,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     const double* pold = (it & 1) ? A.pb1 : A.pb0;
     double* pnew = (it & 1) ? A.pb0 : A.pb1;
     // ---------------- phase A: p, Ap = K p, p.Ap
+    if (timing) t0 = wall_clock64();
     double pap[1] = {0};
     for (int tile = me; tile < ntiles; tile += nwg) {
       const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
@@ -292,7 +298,9 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
       }
       __syncthreads();
     }
+    if (timing) t1 = wall_clock64();
     if (!allreduce(pap)) return;
+    if (timing) t2 = wall_clock64();
     if (fabs(pap[0]) < 1e-30) { breakdown = 1; it++; break; }
     const double alpha = rz / pap[0];
 
@@ -334,7 +342,12 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
         }
       }
     }
+    if (timing) t3 = wall_clock64();
     if (!allreduce(acc)) return;
+    if (timing) {
+      const unsigned long long t4 = wall_clock64();
+      tk[0] += t1 - t0; tk[1] += t2 - t1; tk[2] += t3 - t2; tk[3] += t4 - t3;
+    }
     rr = acc[1];
     const double rn = sqrt(rr);
     if (rn < A.tol || rn / bnorm < A.tol) { converged = 1; it++; break; }
@@ -349,6 +362,8 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     A.st->breakdown = breakdown;
     A.st->final_residual = sqrt(rr);
     A.st->bnorm = bnorm;
+    // phase A, its all-reduce, phase B, its all-reduce
+    for (int k = 0; k < 4; k++) A.st->ticks[k] = tk[k];
   }
 }
 
@@ -719,8 +734,17 @@ __global__ void k_zero_sync(GridSync* gs) {
 }
 
 // Launch shape of the persistent PCG for this plan.
+// Resident single-reduction kernel and the strided kernel's two tile sizes.
+// Measured on MI355X (us per iteration): C2 (143^2 nodes) 5.46 on 18x18
+// tiles vs 5.63 on 15x15; 700^2 elements strided 55.8 / 57.0 / 63.1 on
+// 18 / 23 / 15; 1024^2 with per-element thickness 115.8 / 112.2 / 130.3.
+using CfgR = Tile<18, 18, 384>;
+using CfgL = Tile<23, 23, 576>;
+
 struct PcgShape {
-  int kind;  // 0 = 31x31 tiles resident, 1 = 15x15 resident, 2 = 15x15 strided
+  // 0 = 31x31 resident, 1 = 15x15 resident, 2 = 18x18 strided,
+  // 3 = 18x18 single-reduction resident, 4 = 23x23 strided
+  int kind;
   int grid;
 };
 
@@ -729,40 +753,49 @@ struct PcgShape {
 constexpr int kMaxLargeTiles = 64;
 
 int pcg_shape(pp_plan* pl, int variant, PcgShape* sh) {
-  const int nx = pl->nx, ny = pl->ny;
-  const int nl = ((nx + 1 + CfgX::TX - 1) / CfgX::TX) * ((ny + 1 + CfgX::TY - 1) / CfgX::TY);
-  const int ns = ((nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((ny + 1 + CfgS::TY - 1) / CfgS::TY);
+  const int nx = pl->nx, ny = pl->ny, cu = pl->ctx->cu_count;
+  auto tiles = [&](int T) { return ((nx + T) / T) * ((ny + T) / T); };
+  const int nl = tiles(CfgX::TX), ns = tiles(CfgS::TX);
   int per_cu = 0;
   const bool small_tiles = variant == PP_FLAG_SMALL_TILES;
   if (variant == 0) {  // default: single-reduction kernel while it is resident
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_cg<CfgS>, CfgS::NT, 0));
-    if (ns <= per_cu * pl->ctx->cu_count) {
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_cg<CfgR>, CfgR::NT, 0));
+    if (tiles(CfgR::TX) <= per_cu * cu) {
       sh->kind = 3;
-      sh->grid = ns;
+      sh->grid = tiles(CfgR::TX);
       return PP_OK;
     }
   }
   if (!small_tiles && nl <= kMaxLargeTiles) {
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_res<CfgX>, CfgX::NT, 0));
-    if (nl <= per_cu * pl->ctx->cu_count) {
+    if (nl <= per_cu * cu) {
       sh->kind = 0;
       sh->grid = nl;
       return PP_OK;
     }
   }
   PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_res<CfgS>, CfgS::NT, 0));
-  if (ns <= per_cu * pl->ctx->cu_count) {
+  if (ns <= per_cu * cu) {
     sh->kind = 1;
     sh->grid = ns;
     return PP_OK;
   }
-  // Strided: one workgroup per CU.  Measured on MI355X at 700x700 (2,209
-  // tiles): 63, 64, 70 and 78 us per iteration at 1, 2, 3 and 4 workgroups
-  // per CU; more workgroups add all-reduce slots and tile-count imbalance
-  // without hiding more latency.
-  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgS, false>, CfgS::NT, 0));
-  sh->kind = 2;
-  sh->grid = std::min(ns, std::min(per_cu, 1) * pl->ctx->cu_count);
+  // Strided: one workgroup per CU (more add all-reduce slots and tile-count
+  // imbalance without hiding more latency: 63, 64, 70 and 78 us per
+  // iteration at 1-4 per CU on 15x15 tiles at 700^2).  Of the two tile
+  // sizes, the one whose last round over the CUs is fuller.
+  const int n18 = tiles(CfgR::TX), n23 = tiles(CfgL::TX);
+  auto fill = [&](int nt) {
+    const int rounds = (nt + cu - 1) / cu;
+    return (double)nt / ((double)rounds * cu);
+  };
+  const bool large = fill(n23) > fill(n18);
+  if (large)
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgL, false>, CfgL::NT, 0));
+  else
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgR, false>, CfgR::NT, 0));
+  sh->kind = large ? 4 : 2;
+  sh->grid = std::min(large ? n23 : n18, std::min(per_cu, 1) * cu);
   if (sh->grid <= 0) return fail(PP_ERR_HIP, "persistent PCG kernel cannot be made resident");
   return PP_OK;
 }
@@ -802,9 +835,11 @@ int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxi
   else if (sh.kind == 1)
     hipLaunchKernelGGL((k_pcg_res<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
   else if (sh.kind == 2)
-    hipLaunchKernelGGL((k_pcg<CfgS, false>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
+    hipLaunchKernelGGL((k_pcg<CfgR, false>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
+  else if (sh.kind == 3)
+    hipLaunchKernelGGL((k_pcg_cg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
   else
-    hipLaunchKernelGGL((k_pcg_cg<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
+    hipLaunchKernelGGL((k_pcg<CfgL, false>), dim3(sh.grid), dim3(CfgL::NT), 0, s, A);
   PP_HIP(hipGetLastError());
   GridSync gsh;
   PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
