@@ -149,7 +149,7 @@ def bench_batch(a):
         "converged": st["converged"],
         "autotune": {"trials": trials, "seconds": tune_s,
                      "schedule": {k: sched[k] for k in ("workgroups_per_cu", "tile_shape",
-                                                        "balance_iterations")},
+                                                        "balance_iterations", "resident_plates")},
                      "workgroups": st["workgroups"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -17,8 +17,13 @@
 // Workgroups are dealt to plates; the workgroups of one plate run its PCG
 // (the two-reduction recurrence of pcg.ts:146-206, OR stopping rule, FP64)
 // with a per-plate atomic-free all-reduce, and leave when their plate has
-// converged.  The schedule (tile shape per plate, workgroups per plate, total
-// workgroups per CU) is chosen by pp_batch_autotune: it measures each
+// converged.  A plate dealt one workgroup per tile may instead keep every
+// per-node vector in registers and run the single-reduction recurrences
+// (one pass and one all-reduce per iteration, only the tile edge through
+// memory): latency-bound small plates with many iterations gain about 4x
+// per iteration, but the kernel then holds one workgroup per CU less.  The
+// schedule (tile shape per plate, workgroups per plate, workgroups per CU,
+// resident plates or not) is chosen by pp_batch_autotune: it measures each
 // plate's iteration count once, then times candidate schedules (workgroups
 // dealt greedily to minimise the slowest plate's iterations x tiles per
 // workgroup, at several occupancies and tile shapes) and keeps the fastest.
@@ -136,7 +141,168 @@ __device__ __forceinline__ void gather(const PlateDesc& D, int tx, int ty, const
            ye_s[(3 + d) * kNT + nw];
 }
 
+// Ring slot k of a TX x TY tile's one-node halo (PW = TX + 2, PH = TY + 2):
+// bottom row, top row, left column, right column.
+__device__ __forceinline__ void ring_slot(int k, int PW, int PH, int TY, int& lx, int& ly) {
+  if (k < PW) { lx = k; ly = 0; }
+  else if (k < 2 * PW) { lx = k - PW; ly = PH - 1; }
+  else if (k < 2 * PW + TY) { lx = 0; ly = 1 + (k - 2 * PW); }
+  else { lx = PW - 1; ly = 1 + (k - 2 * PW - TY); }
+}
+
+// Resident single-reduction PCG (Chronopoulos-Gear) for a plate that has
+// one workgroup per tile: the recurrences of the structured path's
+//   x += a p; r -= a s; u = M^-1 r; w = K u; (g, d, rho) = (r.u, w.u, r.r)
+//   b = g'/g; pAp = d' - b g'/a; a' = g'/pAp; p = u + b p; s = w + b s
+// with every per-node vector in registers.  K u needs u on the halo ring:
+// the workgroup keeps r and s of its ring nodes itself, updated with the
+// owners' arithmetic from the w values the owners publish (tile edge, in
+// p0/p1 by iteration parity) before the all-reduce, so the copies stay
+// bit-identical and one all-reduce per iteration is the only barrier.
+// Same iterates as pcg.ts:146-206 in exact arithmetic; OR stopping rule,
+// breakdown and zero right-hand side as the strided path.
+__device__ __forceinline__ void batch_cg(const PlateDesc& D, unsigned me, unsigned nwg,
+                                         double tol, int maxit, const double* src, bool restart,
+                                         double bnorm_in, double* u_s, double* ye_s,
+                                         double* red) {
+  const int TX = D.TX, TY = D.TY, PW = TX + 2, PH = TY + 2;
+  const int NRING = 2 * PW + 2 * TY;  // <= 80 for every shape
+  const int i0 = (me % D.ntx) * TX, j0 = (me / D.ntx) * TY;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int gi = i0 + tx, gj = j0 + ty;
+  const bool node_thread = threadIdx.x < TX * TY;
+  const bool mine = node_thread && gi <= D.nx && gj <= D.ny;
+  const int64_t n = mine ? (int64_t)gj * (D.nx + 1) + gi : 0;
+  const bool edge = mine && (tx == 0 || ty == 0 || tx == TX - 1 || ty == TY - 1 ||
+                             gi == D.nx || gj == D.ny);
+  const int c = (ty + 1) * PW + tx + 1;
+  int rpos = -1;
+  int64_t rnode = -1;
+  if ((int)threadIdx.x < NRING) {
+    int lx, ly;
+    ring_slot(threadIdx.x, PW, PH, TY, lx, ly);
+    rpos = ly * PW + lx;
+    const int hi = i0 - 1 + lx, hj = j0 - 1 + ly;
+    if (hi >= 0 && hi <= D.nx && hj >= 0 && hj <= D.ny) rnode = (int64_t)hj * (D.nx + 1) + hi;
+  }
+  unsigned epoch = 0;
+  double x[3] = {0, 0, 0}, r[3] = {0, 0, 0}, u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+  double p[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+  double rr_[3] = {0, 0, 0}, rs_[3] = {0, 0, 0}, rm[6] = {0, 0, 0, 0, 0, 0};
+  const unsigned fix = mine ? D.cmask[n] : 0u;
+  const double* mown = D.minv + 6 * n;
+  if (mine)
+    for (int d = 0; d < 3; d++) {
+      r[d] = src[3 * n + d];
+      x[d] = restart ? D.x[3 * n + d] : 0.0;
+    }
+  if (rnode >= 0) {
+    for (int d = 0; d < 3; d++) rr_[d] = src[3 * rnode + d];
+    for (int k = 0; k < 6; k++) rm[k] = D.minv[6 * rnode + k];
+  }
+  // u = M^-1 r (own nodes and ring), w = K u, edge w published, partials
+  auto matvec = [&](double* wpub, double (&acc)[3]) {
+    if (mine) {
+      sym3_apply(mown, r, u);
+      for (int d = 0; d < 3; d++) u_s[d * kMaxNP + c] = u[d];
+    } else if (node_thread) {
+      for (int d = 0; d < 3; d++) u_s[d * kMaxNP + c] = 0.0;
+    }
+    if (rpos >= 0) {
+      double ru[3] = {0, 0, 0};
+      if (rnode >= 0) sym3_apply(rm, rr_, ru);
+      for (int d = 0; d < 3; d++) u_s[d * kMaxNP + rpos] = ru[d];
+    }
+    __syncthreads();
+    cells(D, i0, j0, u_s, ye_s);
+    __syncthreads();
+    acc[0] = acc[1] = acc[2] = 0;
+    if (mine) {
+      double v[3];
+      gather(D, tx, ty, ye_s, v);
+      for (int d = 0; d < 3; d++) {
+        w[d] = ((fix >> d) & 1u) ? u[d] : v[d];
+        if (edge) st_coh(&wpub[3 * n + d], w[d]);
+        acc[0] += r[d] * u[d];
+        acc[1] += w[d] * u[d];
+        acc[2] += r[d] * r[d];
+      }
+    }
+  };
+  auto allreduce = [&](double (&v)[3]) -> bool {
+    publish();
+    block_sum<3, kNT>(v, red + 16);
+    return grid_allreduce<3, 2>(v, D.part, D.gs, epoch, red, nwg, me);
+  };
+  auto ring_w = [&](const double* wpub, double (&wv)[3]) {
+    for (int d = 0; d < 3; d++) wv[d] = rnode >= 0 ? ld_coh(&wpub[3 * rnode + d]) : 0.0;
+  };
+  auto sw = [](double b, const double (&wv)[3], double (&sv)[3]) {
+    for (int d = 0; d < 3; d++) sv[d] = fma(b, sv[d], wv[d]);
+  };
+  auto rs = [](double a, const double (&sv)[3], double (&rv)[3]) {
+    for (int d = 0; d < 3; d++) rv[d] = fma(-a, sv[d], rv[d]);
+  };
+
+  double acc[3];
+  matvec(D.p0, acc);
+  if (!allreduce(acc)) return;
+  double g = acc[0], pap = acc[1], rho = acc[2];
+  const double bnorm = restart ? bnorm_in : sqrt(rho);
+  int it = 0, converged = 0, breakdown = 0;
+  const bool done0 = restart ? (bnorm < 1e-30 || sqrt(rho) < tol || sqrt(rho) / bnorm < tol)
+                             : bnorm < 1e-30;
+  if (done0) {
+    converged = 1;
+  } else if (maxit > 0 && fabs(pap) < 1e-30) {
+    breakdown = 1;
+    it = 1;
+  } else {
+    double a = g / pap;
+    for (int d = 0; d < 3; d++) { p[d] = u[d]; s[d] = w[d]; }
+    if (rpos >= 0) ring_w(D.p0, rs_);
+    while (it < maxit) {
+      double* wpub = (it & 1) ? D.p0 : D.p1;
+      if (mine)
+        for (int d = 0; d < 3; d++) x[d] = fma(a, p[d], x[d]);
+      rs(a, s, r);
+      if (rnode >= 0) rs(a, rs_, rr_);
+      matvec(wpub, acc);
+      if (!allreduce(acc)) return;
+      it++;
+      const double g1 = acc[0], d1 = acc[1];
+      rho = acc[2];
+      const double rn = sqrt(rho);
+      if (rn < tol || rn / bnorm < tol) { converged = 1; break; }
+      const double b = g1 / g;
+      pap = d1 - b * g1 / a;
+      if (fabs(pap) < 1e-30) { breakdown = 1; it++; break; }
+      a = g1 / pap;
+      g = g1;
+      for (int d = 0; d < 3; d++) p[d] = fma(b, p[d], u[d]);
+      sw(b, w, s);
+      if (rpos >= 0) {
+        double wv[3];
+        ring_w(wpub, wv);
+        if (rnode >= 0) sw(b, wv, rs_);
+      }
+    }
+  }
+  if (mine)
+    for (int d = 0; d < 3; d++) D.x[3 * n + d] = x[d];
+  if (me == 0 && threadIdx.x == 0) {
+    D.st->iterations = it;
+    D.st->converged = converged;
+    D.st->breakdown = breakdown;
+    D.st->final_residual = sqrt(rho);
+    D.st->bnorm = bnorm;
+  }
+}
+
 // restart: r = r0 (F - K x, precomputed), x kept, |F| from the cold run.
+// RES: plates with one workgroup per tile take the resident path (batch_cg);
+// without it the kernel keeps fewer registers and one more workgroup per CU.
+template <bool RES>
 __global__ __launch_bounds__(kNT) void k_batch_pcg(const PlateDesc* descs, const int* wg_plate,
                                                    double tol, int maxit, const double* r0pool,
                                                    const int64_t* dof_off) {
@@ -145,6 +311,15 @@ __global__ __launch_bounds__(kNT) void k_batch_pcg(const PlateDesc* descs, const
   __shared__ double red[80];
   const PlateDesc D = descs[wg_plate[blockIdx.x]];
   const unsigned me = blockIdx.x - D.wg0, nwg = D.nwg;
+  if (RES && (int)nwg == D.ntiles) {  // one workgroup per tile: registers, one all-reduce
+    // These plates are latency-bound chains of barriers sharing their CUs
+    // with streaming workgroups of the strided plates: their waves issue first.
+    __builtin_amdgcn_s_setprio(2);
+    const bool rs = r0pool != nullptr;
+    batch_cg(D, me, nwg, tol, maxit, rs ? r0pool + dof_off[wg_plate[blockIdx.x]] : D.F, rs,
+             rs ? D.st->bnorm : 0.0, p_s, ye_s, red);
+    return;
+  }
   const int tx = threadIdx.x % D.TX, ty = threadIdx.x / D.TX;
   const bool node_thread = threadIdx.x < D.TX * D.TY;
   const int PW = D.TX + 2;
@@ -365,7 +540,7 @@ struct pp_batch {
   DevBuf<GridSync> gs;
   DevBuf<DevStats> st;
   std::vector<PlateDesc> hdesc;
-  int total_wg = 0, per_cu_max = 0;
+  int total_wg = 0, per_cu_max = 0, per_cu_res = 0;
   bool last_checked = false;
   pp_batch_schedule sched{};
 };
@@ -492,9 +667,12 @@ int apply_schedule(pp_batch* b, const pp_batch_schedule* s) {
       q.shape = best;
     }
   }
-  const int wpc = std::max(1, std::min(s->workgroups_per_cu, b->per_cu_max));
+  if (s->resident_plates != 0 && s->resident_plates != 1)
+    return fail(PP_ERR_INVALID, "resident_plates must be 0 or 1");
+  const int cap = s->resident_plates ? b->per_cu_res : b->per_cu_max;
+  const int wpc = std::max(1, std::min(s->workgroups_per_cu, cap));
   const int total = std::max(n, wpc * b->ctx->cu_count);
-  if (total > b->per_cu_max * b->ctx->cu_count)
+  if (total > cap * b->ctx->cu_count)
     return fail(PP_ERR_UNSUPPORTED, "more plates than resident workgroups");
   deal(b, total, s->balance_iterations != 0);  // else in proportion to tiles only
   // descriptors, workgroup table, partial slots
@@ -557,10 +735,15 @@ int launch(pp_batch* b, const pp_solve_opts* o, bool restart, std::vector<DevSta
   const int n = (int)b->P.size();
   hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(n, 256)), dim3(256), 0, s, b->gs.p, b->st.p, n);
   PP_HIP(hipMemsetAsync(b->part.p, 0, b->part.n * 8, s));
-  hipLaunchKernelGGL(k_batch_pcg, dim3(b->total_wg), dim3(kNT), 0, s, (const PlateDesc*)b->descs.p,
-                     (const int*)b->wg_plate.p, o->tolerance, o->max_iterations,
-                     restart ? (const double*)b->tmp.p : (const double*)nullptr,
-                     (const int64_t*)b->dof_off.p);
+  const double* r0 = restart ? (const double*)b->tmp.p : (const double*)nullptr;
+  if (b->sched.resident_plates)
+    hipLaunchKernelGGL(k_batch_pcg<true>, dim3(b->total_wg), dim3(kNT), 0, s,
+                       (const PlateDesc*)b->descs.p, (const int*)b->wg_plate.p, o->tolerance,
+                       o->max_iterations, r0, (const int64_t*)b->dof_off.p);
+  else
+    hipLaunchKernelGGL(k_batch_pcg<false>, dim3(b->total_wg), dim3(kNT), 0, s,
+                       (const PlateDesc*)b->descs.p, (const int*)b->wg_plate.p, o->tolerance,
+                       o->max_iterations, r0, (const int64_t*)b->dof_off.p);
   PP_HIP(hipGetLastError());
   std::vector<GridSync> gs(n);
   ds.resize(n);
@@ -780,16 +963,20 @@ int pp_batch_create(pp_ctx* ctx, int32_t n, const pp_mesh* meshes, const pp_mate
     pp_batch_destroy(b);
     return fail(PP_ERR_HIP, "batch set-up kernels failed");
   }
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_pcg, kNT, 0) != hipSuccess || per_cu <= 0) {
+  int per_cu = 0, per_cu_res = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_pcg<false>, kNT, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_res, k_batch_pcg<true>, kNT, 0) != hipSuccess ||
+      per_cu <= 0 || per_cu_res <= 0) {
     pp_batch_destroy(b);
     return fail(PP_ERR_HIP, "batched PCG kernel cannot be made resident");
   }
   b->per_cu_max = per_cu;
+  b->per_cu_res = per_cu_res;
   pp_batch_schedule def;
-  def.workgroups_per_cu = std::min(2, per_cu);
+  def.workgroups_per_cu = std::min(2, per_cu_res);
   def.tile_shape = 0;
   def.balance_iterations = 0;
+  def.resident_plates = 1;
   if ((rc = apply_schedule(b, &def))) {
     pp_batch_destroy(b);
     return rc;
@@ -866,6 +1053,7 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
   base.workgroups_per_cu = std::min(2, b->per_cu_max);
   base.tile_shape = 0;
   base.balance_iterations = 0;
+  base.resident_plates = 0;
   if ((rc = apply_schedule(b, &base)) || (rc = batch_run(b, &q, &s))) return rc;
   std::vector<int> its(b->P.size());
   for (size_t p = 0; p < b->P.size(); p++) its[p] = b->P[p].last_iterations;
@@ -874,21 +1062,28 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
   pp_batch_schedule cand;
   // most workgroups first (usually fastest, so slow candidates are cut short)
   for (int wpc = std::min(4, b->per_cu_max); wpc >= 1; wpc--)
+    for (int res = 1; res >= 0; res--)
     for (int shape = 0; shape <= kShapes; shape += kShapes)  // square tiles, or per-plate fewest tiles
       for (int bal = 1; bal >= 0; bal--) {
         cand.workgroups_per_cu = wpc;
         cand.tile_shape = shape;
         cand.balance_iterations = bal;
-        if ((int)b->P.size() > wpc * b->ctx->cu_count && wpc < b->per_cu_max) continue;
+        cand.resident_plates = res;
+        const int cap = res ? b->per_cu_res : b->per_cu_max;
+        if (wpc > cap) continue;
+        if ((int)b->P.size() > wpc * b->ctx->cu_count && wpc < cap) continue;
         for (size_t p = 0; p < b->P.size(); p++) b->P[p].last_iterations = its[p];
         if ((rc = apply_schedule(b, &cand))) return rc;
-        // model: the slowest plate's iterations x tile sweeps per workgroup;
-        // candidates predicted 1.5x slower than the best one measured are skipped
+        // model: the slowest plate's iterations x tile sweeps per workgroup
+        // (a resident plate's iteration counted as half a sweep: one pass,
+        // one all-reduce); candidates predicted 1.5x slower than the best
+        // one measured are skipped
         double pred = 0;
         for (size_t p = 0; p < b->P.size(); p++) {
           const BatchPlate& bp = b->P[p];
           const int tl = ceil_div(bp.nx + 1, kShapeTX[bp.shape]) * ceil_div(bp.ny + 1, kShapeTY[bp.shape]);
-          pred = std::max(pred, (double)its[p] * ceil_div(tl, bp.nwg));
+          const double sweeps = res && bp.nwg == tl ? 0.5 : (double)ceil_div(tl, bp.nwg);
+          pred = std::max(pred, (double)its[p] * sweeps);
         }
         if (pred > 1.5 * best_pred) continue;
         // a second run only for candidates within 10% of the best so far

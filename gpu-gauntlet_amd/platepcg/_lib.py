@@ -84,7 +84,7 @@ class TrainStats(C.Structure):
 
 class BatchSchedule(C.Structure):
     _fields_ = [("workgroups_per_cu", C.c_int32), ("tile_shape", C.c_int32),
-                ("balance_iterations", C.c_int32)]
+                ("balance_iterations", C.c_int32), ("resident_plates", C.c_int32)]
 
 
 class BatchStats(C.Structure):

@@ -270,10 +270,13 @@ class Batch:
         wg = np.empty(self.n, np.int32)
         check(self.lib.pp_batch_get_schedule(self.h, C.byref(s), _ptr(wg)))
         return {"workgroups_per_cu": s.workgroups_per_cu, "tile_shape": s.tile_shape,
-                "balance_iterations": s.balance_iterations, "wg_per_plate": wg}
+                "balance_iterations": s.balance_iterations,
+                "resident_plates": s.resident_plates, "wg_per_plate": wg}
 
-    def set_schedule(self, workgroups_per_cu=2, tile_shape=0, balance_iterations=0):
-        s = _lib.BatchSchedule(workgroups_per_cu, tile_shape, balance_iterations)
+    def set_schedule(self, workgroups_per_cu=2, tile_shape=0, balance_iterations=0,
+                     resident_plates=1):
+        s = _lib.BatchSchedule(workgroups_per_cu, tile_shape, balance_iterations,
+                               resident_plates)
         check(self.lib.pp_batch_set_schedule(self.h, C.byref(s)))
 
     @staticmethod

@@ -279,6 +279,10 @@ typedef struct {
                                  3: per plate the shape with the fewest tiles */
   int32_t balance_iterations; /* 1: deal workgroups by measured iterations x tiles,
                                  0: by tiles only */
+  int32_t resident_plates;    /* 1: a plate dealt one workgroup per tile keeps its
+                                 state in registers and runs the single-reduction
+                                 recurrences (kernel limited to 3 workgroups per CU),
+                                 0: every plate strided, state in HBM (up to 4 per CU) */
 } pp_batch_schedule;
 
 typedef struct {

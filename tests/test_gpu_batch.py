@@ -29,8 +29,11 @@ def load_batch(ctx, plates):
 
 def test_small_batch_matches_fp64_oracle(ctx):
     """12 plates (n <= 40, alternating Q4 / DKT, random materials, edges and
-    loads): every plate within 1e-4 of the FP64 oracle (max nodal w error,
-    in practice ~1e-7), same PCG iteration count, true residual <= 1e-8."""
+    loads): every plate within 1e-6 of the FP64 oracle (max nodal w error),
+    true residual <= 1e-8, and the oracle's PCG iteration count to within
+    3 + 1% (these plates have one workgroup per tile and run the
+    single-reduction recurrences, whose rounding differs from pcg.ts's order
+    and may add a residual-replacement restart)."""
     plates = P.batch_plates(count=12, seed=4, n_max=40)
     b, Fs = load_batch(ctx, plates)
     st = b.solve(tolerance=1e-8)
@@ -42,7 +45,8 @@ def test_small_batch_matches_fp64_oracle(ctx):
         np.testing.assert_allclose(Fs[i], F_ref.astype(np.float32), rtol=2e-7, atol=1e-9 * np.abs(F_ref).max())
         x, s = b.result(i)
         xo, so = op.pcg(Fs[i].astype(np.float64), tol=1e-8)
-        assert abs(s["iterations"] - so["iterations"]) <= 3, (i, p["kind"], s["iterations"], so["iterations"])
+        assert abs(s["iterations"] - so["iterations"]) <= 3 + so["iterations"] // 100, \
+            (i, p["kind"], s["iterations"], so["iterations"])
         assert O.max_nodal_w_error(x, xo) < 1e-6, (i, p["kind"])
         assert op.true_rel_residual(x, Fs[i].astype(np.float64)) <= 1.5e-8
         assert s["trueRelResidual"] <= 1e-8
@@ -67,6 +71,29 @@ def test_batch_equals_single_plate_solves(ctx):
     b.close()
 
 
+def test_resident_plates_equal_strided_plates(ctx):
+    """Plates dealt one workgroup per tile run the register-resident
+    single-reduction path (resident_plates=1) or the strided two-reduction
+    path (0): same displacements, both at true residual <= 1e-8."""
+    plates = P.batch_plates(count=10, seed=9, n_max=48)
+    b, Fs = load_batch(ctx, plates)
+    xs = []
+    for res in (1, 0):
+        b.set_schedule(workgroups_per_cu=2, tile_shape=0, balance_iterations=0, resident_plates=res)
+        sc = b.schedule()
+        assert sc["resident_plates"] == res
+        tiles = np.array([((p["n"] + 1 + 14) // 15) ** 2 for p in plates])
+        np.testing.assert_array_equal(sc["wg_per_plate"], tiles)  # one workgroup per tile
+        st = b.solve(tolerance=1e-8)
+        assert st["converged"] == len(plates) and st["worst_true_rel_residual"] <= 1e-8, st
+        xs.append([b.result(i)[0] for i in range(len(plates))])
+    for i in range(len(plates)):
+        assert O.max_nodal_w_error(xs[0][i], xs[1][i]) < 1e-6, (i, plates[i]["kind"])
+    with pytest.raises(PlatePCGError):
+        b.set_schedule(resident_plates=2)
+    b.close()
+
+
 def test_autotune_keeps_the_answer_and_is_not_slower(ctx):
     plates = P.batch_plates(count=48, seed=4, n_max=96)
     b, Fs = load_batch(ctx, plates)
@@ -74,7 +101,8 @@ def test_autotune_keeps_the_answer_and_is_not_slower(ctx):
     x0 = [b.result(i)[0] for i in range(len(plates))]
     sched, best, trials = b.autotune(tolerance=1e-8)
     print("autotune", trials, "trials:", {k: sched[k] for k in ("workgroups_per_cu", "tile_shape",
-          "balance_iterations")}, "%.2f ms (default %.2f ms)" % (best["device_ms"], st0["device_ms"]))
+          "balance_iterations", "resident_plates")},
+          "%.2f ms (default %.2f ms)" % (best["device_ms"], st0["device_ms"]))
     assert trials >= 2
     assert sched["wg_per_plate"].min() >= 1
     assert best["device_ms"] <= 1.05 * st0["device_ms"]
