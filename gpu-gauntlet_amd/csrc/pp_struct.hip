@@ -193,15 +193,21 @@ struct PcgArgs {
   int maxit;
 };
 
-// RES: one tile per workgroup, per-node x, r, z in registers (p stays in
-// LDS, Ap in ap_s between the phases).  !RES: tiles strided over the
-// workgroups, state in HBM.
-template <class C, bool RES>
+// Tiles strided over a resident grid of workgroups, state in HBM; the
+// two-reduction recurrence of pcg.ts:146-206.  Phase A stages p = z + b p
+// (tile plus halo) and applies K; phase B updates x, r, z.  Phase A loads
+// the next tile's staging values into registers while the current tile is
+// applied (the current tile's element data is loaded before that prefetch,
+// because loads complete in issue order; the workgroup barriers wait on LDS
+// only, so the prefetch stays in flight across them): 5-6% per iteration at
+// 700^2 and 1024^2.
+template <class C>
 __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
+  static_assert(C::NE <= C::NT, "one element per thread");
+  constexpr int SPT = (C::NP + C::NT - 1) / C::NT;  // staged nodes per thread
   __shared__ double p_s[3 * C::NP];
   __shared__ double ye_s[12 * C::NE];
   __shared__ double red[80];
-  __shared__ double ap_s[RES ? 3 * C::NT : 1];
   const unsigned me = blockIdx.x, nwg = gridDim.x;
   const KGrid& G = A.G;
   const int ntx = tiles_x<C>(G), ntiles = tiles_total<C>(G);
@@ -215,10 +221,6 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
     return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
   };
-
-  double xr[3] = {0, 0, 0}, rr_[3] = {0, 0, 0}, zr[3] = {0, 0, 0};
-  int64_t my_n = -1;
-  unsigned my_fix = 0;
 
   // ---------------- init: r = F (or r0 on a restart), z = M^-1 r, r.z, r.r
   const bool restart = A.r0 != nullptr;
@@ -234,21 +236,10 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
       sym3_apply(m, rv, zv);
       for (int d = 0; d < 3; d++) {
         st_coh(&A.z[3 * n + d], zv[d]);
-        if (!RES) {
-          A.r[3 * n + d] = rv[d];
-          if (!restart) A.x[3 * n + d] = 0.0;
-        }
+        A.r[3 * n + d] = rv[d];
+        if (!restart) A.x[3 * n + d] = 0.0;
         acc[0] += rv[d] * zv[d];
         acc[1] += rv[d] * rv[d];
-      }
-      if (RES) {
-        my_n = n;
-        my_fix = G.cmask[n];
-        for (int d = 0; d < 3; d++) {
-          rr_[d] = rv[d];
-          zr[d] = zv[d];
-          xr[d] = restart ? A.x[3 * n + d] : 0.0;
-        }
       }
     }
   }
@@ -257,8 +248,6 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
   const double bnorm = restart ? A.bnorm_in : sqrt(rr);
   const bool done0 = restart ? (sqrt(rr) < A.tol || sqrt(rr) / bnorm < A.tol) : bnorm < 1e-30;
   if (done0) {
-    if (RES && my_n >= 0)
-      for (int d = 0; d < 3; d++) A.x[3 * my_n + d] = xr[d];
     if (me == 0 && threadIdx.x == 0) {
       A.st->iterations = 0; A.st->converged = 1; A.st->breakdown = 0;
       A.st->final_residual = restart ? sqrt(rr) : 0.0; A.st->bnorm = bnorm;
@@ -266,24 +255,88 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     return;
   }
 
+  // staging values (z, and p_old when there is one) of this thread's nodes
+  // of a tile's staged block, loaded ahead of use
+  double sv[SPT][6];
+  auto stage_load = [&](int tile, const double* pold) {
+    const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+      const int idx = threadIdx.x + k * C::NT;
+#pragma unroll
+      for (int q = 0; q < 6; q++) sv[k][q] = 0.0;
+      if (idx < C::NP) {
+        const int gi = i0 - 1 + idx % C::PW, gj = j0 - 1 + idx / C::PW;
+        if (gi >= 0 && gi <= G.nx && gj >= 0 && gj <= G.ny) {
+          const size_t n = (size_t)gj * (G.nx + 1) + gi;
+#pragma unroll
+          for (int d = 0; d < 3; d++) sv[k][d] = ld_coh(A.z + 3 * n + d);
+          if (pold)
+#pragma unroll
+            for (int d = 0; d < 3; d++) sv[k][3 + d] = ld_coh(pold + 3 * n + d);
+        }
+      }
+    }
+  };
+  auto stage_store = [&](bool with_pold, double beta) {
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+      const int idx = threadIdx.x + k * C::NT;
+      if (idx < C::NP)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          p_s[d * C::NP + idx] = with_pold ? sv[k][d] + beta * sv[k][3 + d] : sv[k][d];
+    }
+  };
+
   double beta = 0;
   int it = 0, converged = 0, breakdown = 0;
   for (; it < A.maxit; it++) {
-    const double* pold = (it & 1) ? A.pb1 : A.pb0;
+    const double* pold = it == 0 ? nullptr : (it & 1) ? A.pb1 : A.pb0;
     double* pnew = (it & 1) ? A.pb0 : A.pb1;
     // ---------------- phase A: p, Ap = K p, p.Ap
     if (timing) t0 = wall_clock64();
     double pap[1] = {0};
+    if (me < (unsigned)ntiles) stage_load(me, pold);
     for (int tile = me; tile < ntiles; tile += nwg) {
       const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
-      stage_tile<C, double, double, true>(G, i0, j0, A.z, it == 0 ? nullptr : pold, beta, p_s);
-      __syncthreads();
-      tile_elements<C, double>(G, i0, j0, p_s, ye_s);
-      __syncthreads();
+      // this tile's element data and node mask, then its staged block, then
+      // the next tile's staging loads
+      const int le = threadIdx.x, ex = le % C::EW, ey = le / C::EW;
+      const int ei = i0 - 1 + ex, ej = j0 - 1 + ey;
+      const bool elem = le < C::NE && ei >= 0 && ei < G.nx && ej >= 0 && ej < G.ny;
+      double ea = 0, eb = 0, eia = 0, eib = 0, eD = 0, eDs = 0;
+      if (elem) {
+        ea = G.ax[2 * ei]; eia = G.ax[2 * ei + 1];
+        eb = G.by[2 * ej]; eib = G.by[2 * ej + 1];
+        elem_material(G, ei, ej, eD, eDs);
+      }
       const int gi = i0 + tx, gj = j0 + ty;
-      if (node_thread && gi <= G.nx && gj <= G.ny) {
-        const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
-        const unsigned fix = RES ? my_fix : G.cmask[n];
+      const bool mine = node_thread && gi <= G.nx && gj <= G.ny;
+      const int64_t n = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
+      const unsigned fix = mine ? G.cmask[n] : 0u;
+      stage_store(pold != nullptr, beta);
+      if (tile + (int)nwg < ntiles) stage_load(tile + nwg, pold);
+      __syncthreads();
+      if (le < C::NE) {
+        double y[12];
+        if (elem) {
+          const int c0 = ey * C::PW + ex, c1 = c0 + 1, c2 = c0 + C::PW + 1, c3 = c0 + C::PW;
+          constexpr int NP = C::NP;
+          const double u[12] = {p_s[c0], p_s[NP + c0], p_s[2 * NP + c0], p_s[c1], p_s[NP + c1],
+                                p_s[2 * NP + c1], p_s[c2], p_s[NP + c2], p_s[2 * NP + c2],
+                                p_s[c3], p_s[NP + c3], p_s[2 * NP + c3]};
+          q4_rect_apply_fast<double>(ea, eb, eia, eib, eD, eD * G.nu, eD * (1.0 - G.nu) / 2.0,
+                                     eDs, u, y);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 12; k++) y[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 12; k++) ye_s[k * C::NE + le] = y[k];
+      }
+      __syncthreads();
+      if (mine) {
         double v[3];
         node_sum<C, double>(tx, ty, ye_s, v);
         const int c = (ty + 1) * C::PW + tx + 1;
@@ -292,8 +345,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
           const double ad = ((fix >> d) & 1u) ? pd : v[d];
           st_coh(&pnew[3 * n + d], pd);
           pap[0] += pd * ad;
-          if (RES) ap_s[d * C::NT + threadIdx.x] = ad;
-          else A.Ap[3 * n + d] = ad;
+          A.Ap[3 * n + d] = ad;
         }
       }
       __syncthreads();
@@ -306,39 +358,22 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
 
     // ---------------- phase B: x, r, z updates, r.z, r.r
     acc[0] = 0; acc[1] = 0;
-    if (RES) {
-      if (my_n >= 0) {
-        const int c = (ty + 1) * C::PW + tx + 1;
+    for (int tile = me; tile < ntiles; tile += nwg) {
+      const int gi = (tile % ntx) * C::TX + tx, gj = (tile / ntx) * C::TY + ty;
+      if (node_thread && gi <= G.nx && gj <= G.ny) {
+        const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
+        double rv[3], zv[3], m[6];
+        for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
         for (int d = 0; d < 3; d++) {
-          xr[d] += alpha * p_s[d * C::NP + c];
-          rr_[d] -= alpha * ap_s[d * C::NT + threadIdx.x];
+          A.x[3 * n + d] += alpha * ld_coh(&pnew[3 * n + d]);
+          rv[d] = A.r[3 * n + d] - alpha * A.Ap[3 * n + d];
+          A.r[3 * n + d] = rv[d];
         }
-        sym3_apply(A.minv + 6 * my_n, rr_, zr);
+        sym3_apply(m, rv, zv);
         for (int d = 0; d < 3; d++) {
-          st_coh(&A.z[3 * my_n + d], zr[d]);
-          acc[0] += rr_[d] * zr[d];
-          acc[1] += rr_[d] * rr_[d];
-        }
-      }
-    } else {
-      for (int tile = me; tile < ntiles; tile += nwg) {
-        const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
-        const int gi = i0 + tx, gj = j0 + ty;
-        if (node_thread && gi <= G.nx && gj <= G.ny) {
-          const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
-          double rv[3], zv[3], m[6];
-          for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
-          for (int d = 0; d < 3; d++) {
-            A.x[3 * n + d] += alpha * ld_coh(&pnew[3 * n + d]);
-            rv[d] = A.r[3 * n + d] - alpha * A.Ap[3 * n + d];
-            A.r[3 * n + d] = rv[d];
-          }
-          sym3_apply(m, rv, zv);
-          for (int d = 0; d < 3; d++) {
-            st_coh(&A.z[3 * n + d], zv[d]);
-            acc[0] += rv[d] * zv[d];
-            acc[1] += rv[d] * rv[d];
-          }
+          st_coh(&A.z[3 * n + d], zv[d]);
+          acc[0] += rv[d] * zv[d];
+          acc[1] += rv[d] * rv[d];
         }
       }
     }
@@ -354,8 +389,6 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     beta = acc[0] / rz;
     rz = acc[0];
   }
-  if (RES && my_n >= 0)
-    for (int d = 0; d < 3; d++) A.x[3 * my_n + d] = xr[d];
   if (me == 0 && threadIdx.x == 0) {
     A.st->iterations = it;
     A.st->converged = converged;
@@ -734,16 +767,16 @@ __global__ void k_zero_sync(GridSync* gs) {
 }
 
 // Launch shape of the persistent PCG for this plan.
-// Resident single-reduction kernel and the strided kernel's two tile sizes.
-// Measured on MI355X (us per iteration): C2 (143^2 nodes) 5.46 on 18x18
-// tiles vs 5.63 on 15x15; 700^2 elements strided 55.8 / 57.0 / 63.1 on
-// 18 / 23 / 15; 1024^2 with per-element thickness 115.8 / 112.2 / 130.3.
+// Tile of the resident single-reduction kernel and of the strided kernel.
+// Measured on MI355X (us per iteration): 143^2 nodes 5.46 on 18x18 tiles vs
+// 5.63 on 15x15; 700^2 elements strided 55.8 / 57.0 / 63.1 on 18 / 23 / 15
+// (before the staging prefetch, which leaves a 23x23 tile short of
+// registers for one 576-thread workgroup per CU).
 using CfgR = Tile<18, 18, 384>;
-using CfgL = Tile<23, 23, 576>;
 
 struct PcgShape {
   // 0 = 31x31 resident, 1 = 15x15 resident, 2 = 18x18 strided,
-  // 3 = 18x18 single-reduction resident, 4 = 23x23 strided
+  // 3 = 18x18 single-reduction resident
   int kind;
   int grid;
 };
@@ -780,22 +813,12 @@ int pcg_shape(pp_plan* pl, int variant, PcgShape* sh) {
     sh->grid = ns;
     return PP_OK;
   }
-  // Strided: one workgroup per CU (more add all-reduce slots and tile-count
-  // imbalance without hiding more latency: 63, 64, 70 and 78 us per
-  // iteration at 1-4 per CU on 15x15 tiles at 700^2).  Of the two tile
-  // sizes, the one whose last round over the CUs is fuller.
-  const int n18 = tiles(CfgR::TX), n23 = tiles(CfgL::TX);
-  auto fill = [&](int nt) {
-    const int rounds = (nt + cu - 1) / cu;
-    return (double)nt / ((double)rounds * cu);
-  };
-  const bool large = fill(n23) > fill(n18);
-  if (large)
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgL, false>, CfgL::NT, 0));
-  else
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgR, false>, CfgR::NT, 0));
-  sh->kind = large ? 4 : 2;
-  sh->grid = std::min(large ? n23 : n18, std::min(per_cu, 1) * cu);
+  // Strided: 18x18-node tiles, one workgroup per CU (more add all-reduce
+  // slots and tile-count imbalance without hiding more latency: 63, 64, 70
+  // and 78 us per iteration at 1-4 per CU on 15x15 tiles at 700^2).
+  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<CfgR>, CfgR::NT, 0));
+  sh->kind = 2;
+  sh->grid = std::min(tiles(CfgR::TX), std::min(per_cu, 1) * cu);
   if (sh->grid <= 0) return fail(PP_ERR_HIP, "persistent PCG kernel cannot be made resident");
   return PP_OK;
 }
@@ -835,11 +858,9 @@ int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxi
   else if (sh.kind == 1)
     hipLaunchKernelGGL((k_pcg_res<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
   else if (sh.kind == 2)
-    hipLaunchKernelGGL((k_pcg<CfgR, false>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
-  else if (sh.kind == 3)
-    hipLaunchKernelGGL((k_pcg_cg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
+    hipLaunchKernelGGL((k_pcg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
   else
-    hipLaunchKernelGGL((k_pcg<CfgL, false>), dim3(sh.grid), dim3(CfgL::NT), 0, s, A);
+    hipLaunchKernelGGL((k_pcg_cg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
   PP_HIP(hipGetLastError());
   GridSync gsh;
   PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
