@@ -124,6 +124,16 @@ struct pp_plan {
   pp::DevBuf<double> scal;            // coloured path device scalars
   pp::DevBuf<double> normp;           // norm-reduction partials
 
+  // post-processing (built on first use): mesh copies, node -> element
+  // list (CSR), element moments, node moments, argmax partials
+  pp::DevBuf<float> post_nodes;
+  pp::DevBuf<uint32_t> post_elems;
+  pp::DevBuf<int64_t> post_off;
+  pp::DevBuf<uint32_t> post_adj;
+  pp::DevBuf<double> post_em;
+  pp::DevBuf<float> post_out;
+  pp::DevBuf<double> post_red;
+
   // mixed precision state (FP32 inner solve)
   pp::DevBuf<float> xs, rs, zs, ps0, ps1, Aps;
   pp::DevBuf<float> kmix;             // shared element matrix: bending | shear, 16x16 each

@@ -114,6 +114,9 @@ SIGNATURES = {
     "pp_apply_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_get_block_inverse": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_pressure_load": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_void_p]),
+    "pp_moments": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pp_max_displacement": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     "pp_cnn_create": (C.c_int, [C.c_void_p, C.POINTER(CnnConfig), C.c_uint64,
                                 C.POINTER(C.c_void_p)]),
     "pp_cnn_destroy": (C.c_int, [C.c_void_p]),

@@ -110,6 +110,31 @@ class Plan:
                                         int(pt.shape[0]), _ptr(F)))
         return F
 
+    # ---- post-processing (postprocess.ts)
+    def _post_x(self, x):
+        if x is None:
+            return None, None
+        x = np.ascontiguousarray(x, np.float64)
+        if x.size != self.n_dof:
+            raise ValueError(f"x needs {self.n_dof} values")
+        return x, _ptr(x)
+
+    def moments(self, x=None):
+        """Nodal Mx, My, Mxy (float32) of x, or of the plan's current solution."""
+        keep, px = self._post_x(x)
+        out = [np.empty(self.n_nodes, np.float32) for _ in range(3)]
+        check(self.lib.pp_moments(self.h, px, *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def max_displacement(self, x=None):
+        """{maxW, nodeIndex, x, y}: the largest |w| (signed), lowest node on ties."""
+        keep, px = self._post_x(x)
+        w, node = C.c_double(), C.c_int64()
+        cx, cy = C.c_float(), C.c_float()
+        check(self.lib.pp_max_displacement(self.h, px, C.byref(w), C.byref(node), C.byref(cx),
+                                           C.byref(cy)))
+        return {"maxW": w.value, "nodeIndex": node.value, "x": cx.value, "y": cy.value}
+
     # ---- solve
     @staticmethod
     def options(tolerance=1e-8, max_iterations=20000, precision=_lib.PRECISION_FP64,

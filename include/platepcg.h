@@ -188,6 +188,23 @@ int pp_get_block_inverse(pp_plan* plan, double* blocks);
  * counterpart: the reference has only point loads (solver.ts:363-392). */
 int pp_pressure_load(pp_plan* plan, double q0, const double* patches, int n_patches, float* F);
 
+/* Bending moments at the nodes: replaces computeMoments
+ * (postprocess.ts:40-270).  Curvatures at each element's centroid (Q4: the
+ * isoparametric derivatives at xi = eta = 0; triangles: linear shape
+ * functions, degenerate ones skipped), M = D [k_x + nu k_y, k_y + nu k_x,
+ * (1 - nu) k_xy / 2] with D = E t^3 / (12 (1 - nu^2)) (per element with
+ * per-element thickness), averaged over each node's elements.  x: DOF vector
+ * [w, thx, thy] per node, or NULL for the plan's current solution on the
+ * device.  Mx, My, Mxy: node_count floats each (any may be NULL).  FP64
+ * arithmetic, float32 results. */
+int pp_moments(pp_plan* plan, const double* x, float* Mx, float* My, float* Mxy);
+
+/* Largest |w| (findMaxDisplacement, postprocess.ts:296-319): the signed w,
+ * its node (the lowest index among equal |w|) and that node's coordinates.
+ * x as for pp_moments.  Any output may be NULL. */
+int pp_max_displacement(pp_plan* plan, const double* x, double* w, int64_t* node, float* node_x,
+                        float* node_y);
+
 /* ------------------------------------------------------------------------
  * Learned preconditioner (no reference counterpart: the reference has only
  * block Jacobi, solver.ts:188-339).  A 3-level U-net on the nodal grid of a

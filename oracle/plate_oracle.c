@@ -16,6 +16,7 @@
  *   - matrix-free colour-ordered K.x      solver.ts:422-483
  *   - PCG with the OR stopping rule       pcg.ts:98-215
  *   - tridiagonal known-answer system     pcg.ts:221-247
+ *   - nodal bending moments               postprocess.ts:40-270
  * and adds what the reference lacks but the workloads need:
  *   - consistent pressure-load integration (uniform + Gaussian patches)
  *     over Q4 (2x2 Gauss) and triangles (3-point rule).  The reference has
@@ -516,4 +517,76 @@ ORC_API void orc_pressure_load(const float* nodes, const uint32_t* elems, int64_
   }
   for (int64_t i = 0; i < n_nodes * 3; i++)
     if (cmask[i]) F[i] = 0;
+}
+
+/* -------------------------------------------------------- post-processing */
+
+/* Nodal bending moments, postprocess.ts:40-270 (computeMomentsQuad :57-155,
+ * computeMomentsTriangle :162-270): curvatures k_x = d thy/dx,
+ * k_y = -d thx/dy, k_xy = d thy/dy - d thx/dx at each element's centroid
+ * (Q4: element.ts:84-184 at xi = eta = 0; triangles: linear shape functions,
+ * |2A| < 1e-15 skipped), M = D (k_x + nu k_y, k_y + nu k_x, (1 - nu) k_xy / 2)
+ * with D = E t^3 / (12 (1 - nu^2)) (types.ts:26-28), and each node the mean
+ * over its elements, added in element order.
+ * ref32 = 1 restates the reference's precision as well: the displacements
+ * arrive rounded to float32 (its solution is a Float32Array), the triangle
+ * shape derivatives are stored in Float32Arrays, and the sums and counts are
+ * Float32Arrays that round after every addition; the rest is double.
+ * ref32 = 0 is the same algorithm in double throughout. */
+ORC_API void orc_moments(const float* nodes, const uint32_t* elems, int64_t n_nodes,
+                         int64_t n_elems, int npe, double E, double nu, double t,
+                         const double* elem_t, const double* x, int ref32, double* Mx,
+                         double* My, double* Mxy) {
+  double* s = (double*)calloc((size_t)n_nodes * 4, sizeof(double));
+  for (int64_t e = 0; e < n_elems; e++) {
+    double xy[8], thx[4], thy[4], dNdx[4], dNdy[4];
+    for (int i = 0; i < npe; i++) {
+      const uint32_t n = elems[e * npe + i];
+      xy[2 * i] = (double)nodes[2 * n];
+      xy[2 * i + 1] = (double)nodes[2 * n + 1];
+      thx[i] = ref32 ? (double)(float)x[3 * (int64_t)n + 1] : x[3 * (int64_t)n + 1];
+      thy[i] = ref32 ? (double)(float)x[3 * (int64_t)n + 2] : x[3 * (int64_t)n + 2];
+    }
+    if (npe == 4) {
+      q4_phys(xy, 0.0, 0.0, dNdx, dNdy);
+    } else {
+      const double twoA = (xy[2] - xy[0]) * (xy[5] - xy[1]) - (xy[4] - xy[0]) * (xy[3] - xy[1]);
+      if (fabs(twoA) < 1e-15) continue;
+      const double inv = 1 / twoA;
+      dNdx[0] = (xy[3] - xy[5]) * inv;
+      dNdy[0] = (xy[4] - xy[2]) * inv;
+      dNdx[1] = (xy[5] - xy[1]) * inv;
+      dNdy[1] = (xy[0] - xy[4]) * inv;
+      dNdx[2] = (xy[1] - xy[3]) * inv;
+      dNdy[2] = (xy[2] - xy[0]) * inv;
+      if (ref32)
+        for (int i = 0; i < 3; i++) {
+          dNdx[i] = (double)(float)dNdx[i];
+          dNdy[i] = (double)(float)dNdy[i];
+        }
+    }
+    double kx = 0, ky = 0, kxy = 0;
+    for (int i = 0; i < npe; i++) {
+      kx += dNdx[i] * thy[i];
+      ky += -dNdy[i] * thx[i];
+      kxy += dNdy[i] * thy[i] - dNdx[i] * thx[i];
+    }
+    const double te = elem_t ? elem_t[e] : t;
+    const double D = E * pow(te, 3) / (12 * (1 - nu * nu));
+    const double m[3] = {D * (kx + nu * ky), D * (ky + nu * kx), D * (1 - nu) * kxy / 2};
+    for (int i = 0; i < npe; i++) {
+      double* a = s + 4 * (int64_t)elems[e * npe + i];
+      for (int k = 0; k < 3; k++) a[k] = ref32 ? (double)(float)(a[k] + m[k]) : a[k] + m[k];
+      a[3] += 1;
+    }
+  }
+  for (int64_t n = 0; n < n_nodes; n++) {
+    const double* a = s + 4 * n;
+    double* out[3] = {Mx, My, Mxy};
+    for (int k = 0; k < 3; k++) {
+      const double v = a[3] > 0 ? a[k] / a[3] : 0.0;
+      out[k][n] = ref32 ? (double)(float)v : v;
+    }
+  }
+  free(s);
 }

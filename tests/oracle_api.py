@@ -108,6 +108,17 @@ class OracleProblem:
                                 _p(F))
         return F
 
+    def moments(self, x, ref32=False):
+        """Nodal (Mx, My, Mxy) of x by postprocess.ts (ref32: in its float32
+        storage as well)."""
+        x = np.ascontiguousarray(x, np.float64)
+        out = [np.zeros(self.nn) for _ in range(3)]
+        lib().orc_moments(_p(self.nodes), _p(self.elems), C.c_int64(self.nn), C.c_int64(self.ne),
+                          C.c_int(self.npe), C.c_double(self.E), C.c_double(self.nu),
+                          C.c_double(self.t), _p(self.et), _p(x), C.c_int(int(ref32)),
+                          *[_p(a) for a in out])
+        return tuple(out)
+
     def true_rel_residual(self, x, F):
         return float(np.linalg.norm(F - self.apply_k(x)) / np.linalg.norm(F))
 
