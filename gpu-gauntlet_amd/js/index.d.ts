@@ -78,5 +78,40 @@ export declare class PlateSolver {
   solve(F: Float32Array, options?: GPUSolverOptions): Promise<Omit<GPUSolveResult, 'gpuTimeMs' | 'usedGPU'>>;
   applyK(x: Float64Array, path?: 'auto' | 'structured' | 'coloured'): Float64Array;
   pressureLoad(q0: number, patches?: GaussianPatch[]): Float32Array;
+  /** computeMoments (postprocess.ts:40-270) of x, or of the last solution on the device */
+  computeMoments(solution64?: Float64Array): { Mx: Float32Array; My: Float32Array; Mxy: Float32Array };
+  /** findMaxDisplacement (postprocess.ts:296-319) */
+  findMaxDisplacement(solution64?: Float64Array): { maxW: number; nodeIndex: number; x: number; y: number };
   destroy(): void;
 }
+
+/** extractVerticalDisplacements (postprocess.ts:279-288) */
+export declare function extractVerticalDisplacements(displacements: Float32Array | Float64Array,
+                                                     nodeCount: number): Float32Array;
+
+/** PlateResult (types.ts:119-138) fields produced by the device solve and
+ * post-processing, plus the FP64 solution and true residual. */
+export interface PlateResultGPU {
+  mesh: PlateMesh;
+  displacements: Float32Array;
+  solution64: Float64Array;
+  w: Float32Array;
+  Mx: Float32Array;
+  My: Float32Array;
+  Mxy: Float32Array;
+  trueRelResidual: number;
+  solverInfo: {
+    iterations: number; finalResidual: number; converged: boolean;
+    solveTimeMs: number; postprocessTimeMs: number; totalTimeMs: number;
+  };
+}
+
+/** The solve + post-processing steps of solvePlate (solver.ts:497-574) on the device. */
+export declare function solvePlateGPU(
+  mesh: PlateMesh,
+  material: PlateMaterial,
+  coloring: ElementColoring | null,
+  F: Float32Array,
+  constrainedDOFs: Set<number>,
+  options?: GPUSolverOptions
+): Promise<PlateResultGPU>;

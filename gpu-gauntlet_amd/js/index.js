@@ -130,9 +130,62 @@ class PlateSolver {
     return F;
   }
 
+  // Nodal bending moments (computeMoments, postprocess.ts:40-270) of a
+  // Float64Array DOF vector, or of the last solve's solution on the device.
+  computeMoments(solution64) {
+    const Mx = new Float32Array(this.nodeCount), My = new Float32Array(this.nodeCount);
+    const Mxy = new Float32Array(this.nodeCount);
+    this.lib.moments(this.handle, solution64 || null, Mx, My, Mxy);
+    return { Mx, My, Mxy };
+  }
+
+  // Largest |w| (findMaxDisplacement, postprocess.ts:296-319): signed maxW,
+  // nodeIndex (lowest among equals), x, y.
+  findMaxDisplacement(solution64) {
+    return this.lib.maxDisplacement(this.handle, solution64 || null);
+  }
+
   destroy() {
     if (this.handle) this.lib.destroyPlan(this.handle);
     this.handle = null;
+  }
+}
+
+// w of every node from the DOF vector (extractVerticalDisplacements,
+// postprocess.ts:279-288).
+function extractVerticalDisplacements(displacements, nodeCount) {
+  const w = new Float32Array(nodeCount);
+  for (let i = 0; i < nodeCount; i++) w[i] = displacements[3 * i];
+  return w;
+}
+
+// The solve and post-processing steps of solvePlate (solver.ts:497-574) on
+// the device, for a caller that has the mesh, colouring, load vector and
+// constrained DOFs: the PlateResult fields displacements, w, Mx, My, Mxy
+// and solverInfo (iterations, finalResidual, converged, solveTimeMs,
+// postprocessTimeMs, totalTimeMs), plus solution64 / trueRelResidual.
+// Defaults follow solvePlate: tolerance 1e-6, 1000 iterations.
+async function solvePlateGPU(mesh, material, coloring, F, constrainedDOFs, options) {
+  const t0 = process.hrtime.bigint();
+  const ms = (a, b) => Number(b - a) / 1e6;
+  const solver = new PlateSolver(mesh, material, coloring, constrainedDOFs);
+  try {
+    const t1 = process.hrtime.bigint();
+    const r = await solver.solve(F, Object.assign({ tolerance: 1e-6, maxIterations: 1000 }, options || {}));
+    const t2 = process.hrtime.bigint();
+    const { Mx, My, Mxy } = solver.computeMoments();
+    const t3 = process.hrtime.bigint();
+    return {
+      mesh, displacements: r.solution, solution64: r.solution64,
+      w: extractVerticalDisplacements(r.solution, mesh.nodeCount), Mx, My, Mxy,
+      trueRelResidual: r.trueRelResidual,
+      solverInfo: {
+        iterations: r.iterations, finalResidual: r.finalResidual, converged: r.converged,
+        solveTimeMs: ms(t1, t2), postprocessTimeMs: ms(t2, t3), totalTimeMs: ms(t0, process.hrtime.bigint()),
+      },
+    };
+  } finally {
+    solver.destroy();
   }
 }
 
@@ -156,6 +209,7 @@ async function solveGPU(mesh, material, coloring, F, constrainedDOFs, options) {
 }
 
 module.exports = {
-  solveGPU, isGPUAvailable, PlateSolver, destroyContext, constrainedMask, packColouring,
+  solveGPU, solvePlateGPU, extractVerticalDisplacements, isGPUAvailable, PlateSolver,
+  destroyContext, constrainedMask, packColouring,
   PRECISION, PRECOND, PATH, FIX,
 };

@@ -346,6 +346,55 @@ napi_value PressureLoad(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// moments(plan, x64 | null, Mx, My, Mxy): Float32Arrays of node count; null
+// x = the plan's solution on the device (computeMoments, postprocess.ts:40-270)
+napi_value Moments(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  double* x = nullptr;
+  float* M[3] = {nullptr, nullptr, nullptr};
+  size_t lx = 0, lm[3] = {0, 0, 0};
+  bool ok = argc == 5 && (is_null(env, argv[1]) || typed(env, argv[1], napi_float64_array, &x, &lx));
+  for (int k = 0; k < 3 && ok; k++) ok = typed(env, argv[2 + k], napi_float32_array, &M[k], &lm[k]);
+  if (!ok || lm[0] != lm[1] || lm[1] != lm[2] || (x && lx != 3 * lm[0])) {
+    napi_throw_type_error(env, nullptr, "moments(plan, x64|null, Mx, My, Mxy)");
+    return nullptr;
+  }
+  int rc = pp_moments((pp_plan*)external_ptr(env, argv[0]), x, M[0], M[1], M[2]);
+  if (rc) return throw_pp(env, rc);
+  return nullptr;
+}
+
+// maxDisplacement(plan, x64 | null) -> {maxW, nodeIndex, x, y}
+// (findMaxDisplacement, postprocess.ts:296-319)
+napi_value MaxDisplacement(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  double* x = nullptr;
+  size_t lx = 0;
+  if (argc < 2 || !(is_null(env, argv[1]) || typed(env, argv[1], napi_float64_array, &x, &lx))) {
+    napi_throw_type_error(env, nullptr, "maxDisplacement(plan, x64|null)");
+    return nullptr;
+  }
+  double w = 0;
+  int64_t node = 0;
+  float cx = 0, cy = 0;
+  int rc = pp_max_displacement((pp_plan*)external_ptr(env, argv[0]), x, &w, &node, &cx, &cy);
+  if (rc) return throw_pp(env, rc);
+  napi_value out, v;
+  NAPI_CALL(env, napi_create_object(env, &out));
+  napi_create_double(env, w, &v);
+  napi_set_named_property(env, out, "maxW", v);
+  napi_set_named_property(env, out, "nodeIndex", make_int(env, node));
+  napi_create_double(env, cx, &v);
+  napi_set_named_property(env, out, "x", v);
+  napi_create_double(env, cy, &v);
+  napi_set_named_property(env, out, "y", v);
+  return out;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   const napi_property_descriptor d[] = {
       {"deviceCount", nullptr, DeviceCount, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -357,6 +406,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"solve", nullptr, Solve, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"applyK", nullptr, ApplyK, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"pressureLoad", nullptr, PressureLoad, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"moments", nullptr, Moments, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"maxDisplacement", nullptr, MaxDisplacement, nullptr, nullptr, nullptr, napi_default,
+       nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

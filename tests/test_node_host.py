@@ -49,3 +49,28 @@ def test_solve_gpu_through_node():
     assert r["usedGPU"] and r["converged"] and r["forceCpuRejected"]
     assert r["trueRelResidual"] <= 1e-8
     assert r["maxWRelError"] < 1e-4
+
+
+@pytest.mark.gpu
+def test_solve_plate_through_node():
+    """solvePlateGPU: the PlateResult fields from the device (moments of the
+    device solution) equal the oracle restatement of postprocess.ts on the
+    same FP64 solution; w is the displacements' w; findMaxDisplacement is the
+    first largest |w|."""
+    import numpy as np
+    import oracle_api as O
+    from platepcg import problem as P
+    out = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "solve_plate_c1.js")],
+                         check=True, capture_output=True, text=True, timeout=120)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    info = r["solverInfo"]
+    assert info["converged"] and r["trueRelResidual"] <= 1e-8
+    assert info["postprocessTimeMs"] >= 0 and info["totalTimeMs"] >= info["solveTimeMs"]
+    x = np.array(r["solution64"])
+    np.testing.assert_array_equal(np.array(r["w"], np.float32), np.array(r["displacements"], np.float32)[0::3])
+    w = P.workload("c1")
+    op = O.OracleProblem(w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    for got, want in zip((r["Mx"], r["My"], r["Mxy"]), op.moments(x)):
+        np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-7 * np.abs(want).max())
+    i = int(np.argmax(np.abs(x[0::3])))
+    assert r["maxDisp"]["nodeIndex"] == i and r["maxDisp"]["maxW"] == x[3 * i]
