@@ -12,8 +12,8 @@
 //    per-node state in registers;
 //  - the recurrence order of pcg.ts (two reductions), resident on up to 64
 //    tiles of 31x31 nodes or on 15x15-node tiles, state in registers;
-//  - larger plates: 18x18- or 23x23-node tiles strided over one workgroup
-//    per CU, state in HBM.
+//  - larger plates: 18x18-node tiles strided over one workgroup per CU,
+//    state in HBM.
 
 #include <math.h>
 
