@@ -68,22 +68,22 @@ struct MixArgs {
 template <class C>
 __device__ __forceinline__ void tile_elements_mfma(const KGrid& G, int i0, int j0, const float* p_s,
                                                    float* ye_s, const float (&ab)[4],
-                                                   const float (&as)[4], bool uniform) {
+                                                   const float (&as)[4], bool uniform,
+                                                   const float (&s1v)[4], const float (&s3v)[4]) {
+  static_assert(C::NE / 16 <= 4 * (C::NT / 64), "four element blocks per wave");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  for (int blk = wave; blk < C::NE / 16; blk += C::NT / 64) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int blk = wave + q * (C::NT / 64);
+    if (blk >= C::NE / 16) break;
     const int le = blk * 16 + col;
     const int ex = le % C::EW, ey = le / C::EW;
     const int gi = i0 - 1 + ex, gj = j0 - 1 + ey;
     const bool valid = gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny;
     const int c0 = ey * C::PW + ex;
     const int cn[4] = {c0, c0 + 1, c0 + C::PW + 1, c0 + C::PW};
-    float s3 = 1.0f, s1 = 1.0f;
-    if (!uniform && valid && G.elem_t) {
-      const double t = (double)G.elem_t[(size_t)gj * G.nx + gi] / G.t0;
-      s1 = (float)t;
-      s3 = (float)(t * t * t);
-    }
+    const float s3 = s3v[q], s1 = s1v[q];
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
@@ -105,8 +105,32 @@ __device__ __forceinline__ void tile_elements_mfma(const KGrid& G, int i0, int j
   }
 }
 
+// Thickness scales (t/t0, (t/t0)^3) of the elements this lane applies in
+// tile_elements_mfma (1 on uniform plates and outside the grid).
+template <class C>
+__device__ __forceinline__ void elem_scales(const KGrid& G, int i0, int j0, bool uniform,
+                                            float (&s1v)[4], float (&s3v)[4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    s1v[q] = s3v[q] = 1.0f;
+    const int blk = wave + q * (C::NT / 64);
+    if (uniform || !G.elem_t || blk >= C::NE / 16) continue;
+    const int le = blk * 16 + (lane & 15);
+    const int gi = i0 - 1 + le % C::EW, gj = j0 - 1 + le / C::EW;
+    if (gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny) {
+      const double t = (double)G.elem_t[(size_t)gj * G.nx + gi] / G.t0;
+      s1v[q] = (float)t;
+      s3v[q] = (float)(t * t * t);
+    }
+  }
+}
+
 // Inner FP32 PCG on K~ e = r64, tiles strided over a resident grid, state in
-// HBM (the two-reduction recurrence of pcg.ts:146-206 in this order).
+// HBM (the two-reduction recurrence of pcg.ts:146-206 in this order).  Phase
+// A loads the next tile's staging values into registers while the current
+// tile is applied; the current tile's element scales and node mask are
+// loaded before that prefetch, because loads complete in issue order.
 template <class C>
 __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
   __shared__ float p_s[3 * C::NP];
@@ -134,6 +158,41 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
       as[kk] = A.ks[idx];
     }
   }
+
+  // staging values (z, and p_old when there is one) of this thread's nodes
+  // of a tile's staged block, loaded ahead of use
+  constexpr int SPT = (C::NP + C::NT - 1) / C::NT;
+  float sv[SPT][6];
+  auto stage_load = [&](int tile, const float* pold) {
+    const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+      const int idx = threadIdx.x + k * C::NT;
+#pragma unroll
+      for (int q = 0; q < 6; q++) sv[k][q] = 0.0f;
+      if (idx < C::NP) {
+        const int gi = i0 - 1 + idx % C::PW, gj = j0 - 1 + idx / C::PW;
+        if (gi >= 0 && gi <= G.nx && gj >= 0 && gj <= G.ny) {
+          const size_t n = (size_t)gj * (G.nx + 1) + gi;
+#pragma unroll
+          for (int d = 0; d < 3; d++) sv[k][d] = ld_coh(A.z + 3 * n + d);
+          if (pold)
+#pragma unroll
+            for (int d = 0; d < 3; d++) sv[k][3 + d] = ld_coh(pold + 3 * n + d);
+        }
+      }
+    }
+  };
+  auto stage_store = [&](bool with_pold, float beta) {
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+      const int idx = threadIdx.x + k * C::NT;
+      if (idx < C::NP)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          p_s[d * C::NP + idx] = with_pold ? sv[k][d] + beta * sv[k][3 + d] : sv[k][d];
+    }
+  };
 
   // ---------------- init: r = (float) r64, e = 0, z = M^-1 r, r.z, r.r
   double acc[2] = {0, 0};
@@ -173,16 +232,24 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
       const bool fresh = it == 0 && !resume;
       // ---------------- phase A: p = z + beta pold, Ap = K~ p, p.Ap
       double pap[1] = {0};
+      const float* po = fresh ? nullptr : pold;
+      if (me < (unsigned)ntiles) stage_load(me, po);
       for (int tile = me; tile < ntiles; tile += nwg) {
         const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
-        stage_tile<C, float, float, true>(G, i0, j0, A.z, fresh ? nullptr : pold, (float)beta, p_s);
-        __syncthreads();
-        tile_elements_mfma<C>(G, i0, j0, p_s, ye_s, ab, as, A.uniform != 0);
-        __syncthreads();
+        float s1v[4], s3v[4];
+        elem_scales<C>(G, i0, j0, A.uniform != 0, s1v, s3v);
         const int gi = i0 + tx, gj = j0 + ty;
-        if (node_thread && gi <= G.nx && gj <= G.ny) {
-          const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
-          const unsigned fix = G.cmask[n];
+        const bool mine = node_thread && gi <= G.nx && gj <= G.ny;
+        const int64_t nm = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
+        const unsigned fixm = mine ? G.cmask[nm] : 0u;
+        stage_store(po != nullptr, (float)beta);
+        if (tile + (int)nwg < ntiles) stage_load(tile + nwg, po);
+        __syncthreads();
+        tile_elements_mfma<C>(G, i0, j0, p_s, ye_s, ab, as, A.uniform != 0, s1v, s3v);
+        __syncthreads();
+        if (mine) {
+          const int64_t n = nm;
+          const unsigned fix = fixm;
           float v[3];
           node_sum<C, float>(tx, ty, ye_s, v);
           const int c = (ty + 1) * C::PW + tx + 1;
