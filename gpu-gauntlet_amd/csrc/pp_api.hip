@@ -443,7 +443,6 @@ int pp_get_block_inverse(pp_plan* pl, double* blocks) {
 int pp_pressure_load(pp_plan* pl, double q0, const double* patches, int n_patches, float* F) {
   if (!pl || !F || n_patches < 0 || (n_patches > 0 && !patches))
     return fail(PP_ERR_INVALID, "bad argument");
-  if (!pl->structured) return general_pressure(pl, q0, patches, n_patches, F);
   DevBuf<double> pt;
   DevBuf<float> f;
   int rc;
@@ -451,7 +450,9 @@ int pp_pressure_load(pp_plan* pl, double q0, const double* patches, int n_patche
   hipStream_t s = pl->ctx->stream;
   if (n_patches)
     PP_HIP(hipMemcpyAsync(pt.p, patches, 4 * n_patches * 8, hipMemcpyHostToDevice, s));
-  if ((rc = struct_pressure(pl, q0, pt.p, n_patches, f.p))) return rc;
+  rc = pl->structured ? struct_pressure(pl, q0, pt.p, n_patches, f.p)
+                      : general_pressure(pl, q0, pt.p, n_patches, f.p);
+  if (rc) return rc;
   PP_HIP(hipMemcpyAsync(F, f.p, pl->n_dof * 4, hipMemcpyDeviceToHost, s));
   PP_HIP(hipStreamSynchronize(s));
   return PP_OK;

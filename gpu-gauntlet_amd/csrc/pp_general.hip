@@ -26,14 +26,15 @@ constexpr int kBatch = 64;       // iterations enqueued between flag polls
 // device scalar slots
 enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BNORM, S_ITER, S_DONE, S_CONV, S_BREAK, S_N };
 
-// ------------------------------------------------ host element stiffness
+// ----------------------------------------------------- element stiffness
+// (evaluated per element on the device at plan set-up; also host-callable)
 
 // Q4 Mindlin 12x12 for a general quadrilateral (element.ts:258-483).
-void ke_q4(const double* xy, double E, double nu, double t, double* Ke) {
+__host__ __device__ void ke_q4(const double* xy, double E, double nu, double t, double* Ke) {
   const double D = E * t * t * t / (12.0 * (1.0 - nu * nu));
   const double D11 = D, D12 = D * nu, D33 = D * (1.0 - nu) / 2.0;
   const double Ds = (5.0 / 6.0) * E / (2.0 * (1.0 + nu)) * t;
-  std::fill(Ke, Ke + 144, 0.0);
+  for (int k = 0; k < 144; k++) Ke[k] = 0.0;
   auto phys = [&](double xi, double eta, double* dx, double* dy) {
     const double dxi[4] = {-(1 - eta) / 4, (1 - eta) / 4, (1 + eta) / 4, -(1 + eta) / 4};
     const double deta[4] = {-(1 - xi) / 4, -(1 + xi) / 4, (1 + xi) / 4, (1 - xi) / 4};
@@ -81,8 +82,8 @@ void ke_q4(const double* xy, double E, double nu, double t, double* Ke) {
 
 // The reference's 9x9 triangle (element.ts:642-943) exactly as written,
 // including its signed 1/(2A) and its B columns.
-void ke_dkt(const double* xy, double E, double nu, double t, double* Ke) {
-  std::fill(Ke, Ke + 81, 0.0);
+__host__ __device__ void ke_dkt(const double* xy, double E, double nu, double t, double* Ke) {
+  for (int k = 0; k < 81; k++) Ke[k] = 0.0;
   const double x12 = xy[0] - xy[2], y12 = xy[1] - xy[3];
   const double x23 = xy[2] - xy[4], y23 = xy[3] - xy[5];
   const double x31 = xy[4] - xy[0], y31 = xy[5] - xy[1];
@@ -355,67 +356,152 @@ int apply_all(pp_plan* pl, const double* dx, double* dy, const double* sc) {
   return PP_OK;
 }
 
+// Ke of every element (one thread each), from the float32 coordinates as
+// the reference's element routines read them.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_elem_ke(int64_t ne, const uint32_t* elems,
+                                                 const float* nodes, double E, double nu, double t,
+                                                 const float* elem_t, double* Ke) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  double xy[2 * NPE];
+#pragma unroll
+  for (int i = 0; i < NPE; i++) {
+    const uint32_t n = elems[e * NPE + i];
+    xy[2 * i] = (double)nodes[2 * (size_t)n];
+    xy[2 * i + 1] = (double)nodes[2 * (size_t)n + 1];
+  }
+  const double te = elem_t ? (double)elem_t[e] : t;
+  constexpr int nd = 3 * NPE;
+  if constexpr (NPE == 4) ke_q4(xy, E, nu, te, Ke + e * nd * nd);
+  else ke_dkt(xy, E, nu, te, Ke + e * nd * nd);
+}
+
+// 3x3 nodal diagonal blocks (solver.ts:188-225) summed over each node's
+// (element, corner) list in ascending element order -- the order of the
+// reference's element loop -- then the constrained-DOF rules and inverse of
+// solver.ts:231-304 (one thread per node).
+__global__ __launch_bounds__(256) void k_block_csr(int64_t nn, int npe, const int64_t* off,
+                                                   const uint32_t* adj, const double* Ke,
+                                                   const uint8_t* cmask, double* minv) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= nn) return;
+  const int nd = 3 * npe;
+  double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t k = off[n]; k < off[n + 1]; k++) {
+    const uint32_t ec = adj[k];  // element * npe + corner
+    const double* K = Ke + (size_t)(ec / npe) * nd * nd;
+    const int c = (int)(ec % npe);
+    for (int r = 0; r < 3; r++)
+      for (int q = 0; q < 3; q++) A[r * 3 + q] += K[(3 * c + r) * nd + 3 * c + q];
+  }
+  block_inverse_sym(A, cmask[n], minv + 6 * n);
+}
+
+// Consistent nodal pressure load, one thread per node: the node's share of
+// every adjacent element's quadrature (Q4 2x2 Gauss with |J|, triangles the
+// 3-point rule with linear N and area/3), summed in element order and then
+// quadrature order (the order of mesh_pressure's element loop).  Rotation
+// loads and constrained w are zero.
+__device__ double pressure_value(double x, double y, double q0, const double* pt, int np) {
+  double p = q0;
+  for (int k = 0; k < np; k++) {
+    const double dx = x - pt[4 * k + 1], dy = y - pt[4 * k + 2], s = pt[4 * k + 3];
+    p += pt[4 * k] * exp(-(dx * dx + dy * dy) / (2 * s * s));
+  }
+  return p;
+}
+
+__global__ __launch_bounds__(256) void k_pressure_csr(int64_t nn, int npe, const int64_t* off,
+                                                      const uint32_t* adj, const uint32_t* elems,
+                                                      const float* nodes, const uint8_t* cmask,
+                                                      double q0, const double* pt, int np,
+                                                      float* F) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= nn) return;
+  double f = 0;
+  for (int64_t k = off[n]; k < off[n + 1]; k++) {
+    const uint32_t ec = adj[k];
+    const int64_t e = ec / npe;
+    const int c = (int)(ec % npe);
+    double xy[8];
+    for (int i = 0; i < npe; i++) {
+      const uint32_t m = elems[e * npe + i];
+      xy[2 * i] = (double)nodes[2 * (size_t)m];
+      xy[2 * i + 1] = (double)nodes[2 * (size_t)m + 1];
+    }
+    if (npe == 4) {
+      const double g = 0.5773502691896257;
+      const double gx[4] = {-g, g, g, -g}, gy[4] = {-g, -g, g, g};
+      for (int q = 0; q < 4; q++) {
+        const double xi = gx[q], eta = gy[q];
+        const double N[4] = {(1 - xi) * (1 - eta) / 4, (1 + xi) * (1 - eta) / 4,
+                             (1 + xi) * (1 + eta) / 4, (1 - xi) * (1 + eta) / 4};
+        const double dxi[4] = {-(1 - eta) / 4, (1 - eta) / 4, (1 + eta) / 4, -(1 + eta) / 4};
+        const double deta[4] = {-(1 - xi) / 4, -(1 + xi) / 4, (1 + xi) / 4, (1 - xi) / 4};
+        double J11 = 0, J12 = 0, J21 = 0, J22 = 0, px = 0, py = 0;
+        for (int i = 0; i < 4; i++) {
+          J11 += dxi[i] * xy[2 * i]; J12 += dxi[i] * xy[2 * i + 1];
+          J21 += deta[i] * xy[2 * i]; J22 += deta[i] * xy[2 * i + 1];
+          px += N[i] * xy[2 * i]; py += N[i] * xy[2 * i + 1];
+        }
+        const double w = pressure_value(px, py, q0, pt, np) * fabs(J11 * J22 - J12 * J21);
+        f += N[c] * w;
+      }
+    } else {
+      const double gL[3][3] = {{1.0 / 6, 1.0 / 6, 2.0 / 3}, {2.0 / 3, 1.0 / 6, 1.0 / 6},
+                               {1.0 / 6, 2.0 / 3, 1.0 / 6}};
+      const double twoA = (xy[0] - xy[2]) * (xy[5] - xy[1]) - (xy[4] - xy[0]) * (xy[1] - xy[3]);
+      const double area = fabs(twoA) / 2;
+      for (int q = 0; q < 3; q++) {
+        const double px = gL[q][0] * xy[0] + gL[q][1] * xy[2] + gL[q][2] * xy[4];
+        const double py = gL[q][0] * xy[1] + gL[q][1] * xy[3] + gL[q][2] * xy[5];
+        const double w = pressure_value(px, py, q0, pt, np) * area / 3.0;
+        f += gL[q][c] * w;
+      }
+    }
+  }
+  F[3 * n] = (cmask[n] & PP_FIX_W) ? 0.0f : (float)f;
+  F[3 * n + 1] = 0.0f;
+  F[3 * n + 2] = 0.0f;
+}
+
 }  // namespace
 
+// Set-up on the device: every element's FP64 Ke, then the nodal block
+// inverses from a node -> (element, corner) list built on the host (an
+// O(elements) integer pass, kept for the pressure loads).
 int general_setup(pp_plan* pl, const pp_colouring* col) {
   const int npe = pl->npe, nd = 3 * npe;
   const int64_t ne = pl->n_elems, nn = pl->n_nodes;
-  std::vector<double> Ke((size_t)ne * nd * nd);
-  for (int64_t e = 0; e < ne; e++) {
-    double xy[8];
-    for (int i = 0; i < npe; i++) {
-      const uint32_t n = pl->h_elems[(size_t)e * npe + i];
-      xy[2 * i] = pl->h_nodes[2 * (size_t)n];
-      xy[2 * i + 1] = pl->h_nodes[2 * (size_t)n + 1];
-    }
-    const double t = pl->var_t ? (double)pl->h_elem_t[e] : pl->t;
-    if (npe == 4) ke_q4(xy, pl->E, pl->nu, t, &Ke[(size_t)e * nd * nd]);
-    else ke_dkt(xy, pl->E, pl->nu, t, &Ke[(size_t)e * nd * nd]);
-  }
-  // block diagonal (solver.ts:188-225) and inverse (solver.ts:231-304)
-  std::vector<double> blk((size_t)nn * 9, 0.0), minv((size_t)nn * 6);
-  for (int64_t e = 0; e < ne; e++) {
-    const double* K = &Ke[(size_t)e * nd * nd];
-    for (int i = 0; i < npe; i++) {
-      const uint32_t n = pl->h_elems[(size_t)e * npe + i];
-      for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) blk[(size_t)n * 9 + r * 3 + c] += K[(3 * i + r) * nd + 3 * i + c];
-    }
-  }
-  for (int64_t n = 0; n < nn; n++) {
-    double* A = &blk[(size_t)n * 9];
-    const unsigned fix = pl->h_cmask[n];
-    const bool cw = fix & PP_FIX_W, cx = fix & PP_FIX_THX, cy = fix & PP_FIX_THY;
-    double* o = &minv[(size_t)n * 6];
-    if (cw && cx && cy) { o[0] = 1; o[1] = 0; o[2] = 0; o[3] = 1; o[4] = 0; o[5] = 1; continue; }
-    if (cw) { A[0] = 1; A[1] = 0; A[2] = 0; A[3] = 0; A[6] = 0; }
-    if (cx) { A[4] = 1; A[1] = 0; A[7] = 0; A[3] = 0; A[5] = 0; }
-    if (cy) { A[8] = 1; A[2] = 0; A[5] = 0; A[6] = 0; A[7] = 0; }
-    const double a00 = A[0], a01 = A[1], a02 = A[2], a10 = A[3], a11 = A[4], a12 = A[5];
-    const double a20 = A[6], a21 = A[7], a22 = A[8];
-    const double det = a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20) +
-                       a02 * (a10 * a21 - a11 * a20);
-    if (fabs(det) < 1e-30) { o[0] = 1; o[1] = 0; o[2] = 0; o[3] = 1; o[4] = 0; o[5] = 1; continue; }
-    const double id = 1.0 / det;
-    o[0] = id * (a11 * a22 - a12 * a21);
-    o[1] = id * (a02 * a21 - a01 * a22);
-    o[2] = id * (a01 * a12 - a02 * a11);
-    o[3] = id * (a00 * a22 - a02 * a20);
-    o[4] = id * (a02 * a10 - a00 * a12);
-    o[5] = id * (a00 * a11 - a01 * a10);
+  std::vector<int64_t> off(nn + 1, 0);
+  for (int64_t e = 0; e < ne; e++)
+    for (int i = 0; i < npe; i++) off[pl->h_elems[(size_t)e * npe + i] + 1]++;
+  for (int64_t n = 0; n < nn; n++) off[n + 1] += off[n];
+  std::vector<uint32_t> adj(std::max<int64_t>(1, off[nn]));
+  {
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (int64_t e = 0; e < ne; e++)
+      for (int i = 0; i < npe; i++)
+        adj[fill[pl->h_elems[(size_t)e * npe + i]]++] = (uint32_t)(e * npe + i);
   }
   std::vector<uint8_t> cdof((size_t)nn * 3);
   for (int64_t n = 0; n < nn; n++)
     for (int d = 0; d < 3; d++) cdof[(size_t)n * 3 + d] = (pl->h_cmask[n] >> d) & 1u;
+  DevBuf<float>& nodes = pl->gen_nodes;
+  DevBuf<int64_t>& doff = pl->gen_off;
+  DevBuf<uint32_t>& dadj = pl->gen_adj;
   int rc;
-  if ((rc = pl->ke.alloc(Ke.size())) || (rc = pl->minv.alloc(minv.size())) ||
+  if ((rc = pl->ke.alloc((size_t)ne * nd * nd)) || (rc = pl->minv.alloc((size_t)nn * 6)) ||
       (rc = pl->cmask_dof.alloc(cdof.size())) || (rc = pl->elems.alloc(pl->h_elems.size())) ||
       (rc = pl->by_colour.alloc(std::max<size_t>(1, col->colour_offsets[col->colour_count]))) ||
-      (rc = pl->scal.alloc(S_N)) || (rc = pl->partials.alloc(2 * kRedBlocks)))
+      (rc = pl->scal.alloc(S_N)) || (rc = pl->partials.alloc(2 * kRedBlocks)) ||
+      (rc = nodes.alloc(2 * nn)) || (rc = doff.alloc(nn + 1)) || (rc = dadj.alloc(adj.size())))
     return rc;
   hipStream_t s = pl->ctx->stream;
-  PP_HIP(hipMemcpyAsync(pl->ke.p, Ke.data(), Ke.size() * 8, hipMemcpyHostToDevice, s));
-  PP_HIP(hipMemcpyAsync(pl->minv.p, minv.data(), minv.size() * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(nodes.p, pl->h_nodes.data(), 2 * nn * 4, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(doff.p, off.data(), (nn + 1) * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(dadj.p, adj.data(), adj.size() * 4, hipMemcpyHostToDevice, s));
   PP_HIP(hipMemcpyAsync(pl->cmask_dof.p, cdof.data(), cdof.size(), hipMemcpyHostToDevice, s));
   PP_HIP(hipMemcpyAsync(pl->elems.p, pl->h_elems.data(), pl->h_elems.size() * 4,
                         hipMemcpyHostToDevice, s));
@@ -423,6 +509,20 @@ int general_setup(pp_plan* pl, const pp_colouring* col) {
   if (total)
     PP_HIP(hipMemcpyAsync(pl->by_colour.p, col->elements_by_colour, (size_t)total * 4,
                           hipMemcpyHostToDevice, s));
+  const float* et = pl->var_t ? pl->elem_t.p : nullptr;
+  const dim3 ge((unsigned)std::max<int64_t>(1, (ne + 255) / 256)), gn((unsigned)((nn + 255) / 256));
+  if (ne > 0) {
+    if (npe == 4)
+      hipLaunchKernelGGL(k_elem_ke<4>, ge, dim3(256), 0, s, ne, (const uint32_t*)pl->elems.p,
+                         (const float*)nodes.p, pl->E, pl->nu, pl->t, et, pl->ke.p);
+    else
+      hipLaunchKernelGGL(k_elem_ke<3>, ge, dim3(256), 0, s, ne, (const uint32_t*)pl->elems.p,
+                         (const float*)nodes.p, pl->E, pl->nu, pl->t, et, pl->ke.p);
+  }
+  hipLaunchKernelGGL(k_block_csr, gn, dim3(256), 0, s, nn, npe, (const int64_t*)doff.p,
+                     (const uint32_t*)dadj.p, (const double*)pl->ke.p,
+                     (const uint8_t*)pl->cmask.p, pl->minv.p);
+  PP_HIP(hipGetLastError());
   PP_HIP(hipStreamSynchronize(s));
   return PP_OK;
 }
@@ -571,9 +671,14 @@ int mesh_pressure(const float* h_nodes, const uint32_t* h_elems, int npe, int64_
   return PP_OK;
 }
 
-int general_pressure(pp_plan* pl, double q0, const double* pt, int np, float* F) {
-  return mesh_pressure(pl->h_nodes.data(), pl->h_elems.data(), pl->npe, pl->n_nodes, pl->n_elems,
-                       pl->h_cmask.data(), q0, pt, np, F);
+int general_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF) {
+  hipLaunchKernelGGL(k_pressure_csr, dim3((unsigned)((pl->n_nodes + 255) / 256)), dim3(256), 0,
+                     pl->ctx->stream, (int64_t)pl->n_nodes, pl->npe,
+                     (const int64_t*)pl->gen_off.p, (const uint32_t*)pl->gen_adj.p,
+                     (const uint32_t*)pl->elems.p, (const float*)pl->gen_nodes.p,
+                     (const uint8_t*)pl->cmask.p, q0, d_patches, np, dF);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
 }
 
 // Element stiffness of one element (host, FP64), for the batched path.

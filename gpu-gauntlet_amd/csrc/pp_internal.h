@@ -114,6 +114,9 @@ struct pp_plan {
   pp::DevBuf<double> ke;              // coloured path: per-element FP64 Ke
   pp::DevBuf<uint32_t> elems;         // coloured path: connectivity
   pp::DevBuf<uint32_t> by_colour;     // coloured path: elements by colour
+  pp::DevBuf<float> gen_nodes;        // coloured path: f32 coordinates
+  pp::DevBuf<int64_t> gen_off;        // coloured path: node -> (element, corner) list,
+  pp::DevBuf<uint32_t> gen_adj;       //   CSR, entries element * npe + corner
 
   // solve state (FP64)
   pp::DevBuf<double> F, x, r, z, p0, p1, Ap, tmp;
@@ -159,7 +162,7 @@ int general_apply_k(pp_plan* pl, const double* dx, double* dy, int colour, bool 
 int general_apply_precond(pp_plan* pl, const double* dr, double* dz);
 int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
 int general_true_residual(pp_plan* pl, const double* dx, double* rel);
-int general_pressure(pp_plan* pl, double q0, const double* patches, int np, float* F);
+int general_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF);
 int mesh_pressure(const float* nodes, const uint32_t* elems, int npe, int64_t n_nodes,
                   int64_t n_elems, const uint8_t* cmask, double q0, const double* patches, int np,
                   float* F);

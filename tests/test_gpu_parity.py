@@ -169,3 +169,28 @@ def test_large_grids_match_oracle_iterates(ctx, n):
     assert st["iterations"] == 30 and not st["converged"]
     xo, so = op.pcg(F.astype(np.float64), max_iter=30)
     assert rel(x64, xo) < 1e-9
+
+
+def test_general_mesh_device_setup_and_loads(ctx):
+    """Coloured path on a non-grid Q4 mesh with per-element thickness: the
+    element matrices and block inverses built on the device, and the
+    pressure load integrated on the device, against the oracle."""
+    from test_post_oracle import distorted
+    mesh = distorted(P.rect_mesh(20, 14))
+    col = P.colour_greedy(mesh)
+    mask = P.constrained_mask(mesh, [{"type": "pinned", "location": "all_edges"}])
+    et = np.random.default_rng(2).uniform(0.05, 0.2, mesh["elementCount"]).astype(np.float32)
+    plan = Plan(ctx, mesh, E, NU, 0.1, mask, col, element_t=et)
+    assert not plan.structured
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask, element_t=et,
+                         order=col["elements_by_colour"])
+    b = plan.block_inverse().reshape(-1, 9)
+    ob = op.blk.reshape(-1, 9)
+    assert np.max(np.abs(b - ob) / np.abs(ob).max(axis=1, keepdims=True)) < 1e-10
+    x = np.random.default_rng(3).standard_normal(plan.n_dof)
+    assert rel(plan.apply_k(x), op.apply_k(x)) < 1e-12
+    pt = np.array([[4e4, 3.0, 7.0, 1.2], [2e4, 6.5, 2.5, 0.8]])
+    F = plan.pressure_load(1e4, pt)
+    np.testing.assert_allclose(F, op.pressure_load(1e4, pt).astype(np.float32), rtol=2e-7,
+                               atol=0)
+    plan.close()
