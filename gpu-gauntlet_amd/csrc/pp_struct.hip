@@ -200,7 +200,8 @@ struct PcgArgs {
 // applied (the current tile's element data is loaded before that prefetch,
 // because loads complete in issue order; the workgroup barriers wait on LDS
 // only, so the prefetch stays in flight across them): 5-6% per iteration at
-// 700^2 and 1024^2.
+// 700^2 and 1024^2.  Phase B and the init run over equal node ranges instead
+// of tiles (Ap then goes through device-coherent stores and loads).
 template <class C>
 __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
   static_assert(C::NE <= C::NT, "one element per thread");
@@ -222,15 +223,19 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
   };
 
+  // Node-wise work (init, phase B) goes by equal contiguous node ranges,
+  // one per workgroup, so every workgroup reaches the all-reduce after the
+  // same share and the accesses are whole rows; x and r are only ever
+  // touched by their range's owner.
+  const int64_t nn = (int64_t)(G.nx + 1) * (G.ny + 1);
+  const int64_t lo = nn * me / nwg, hi = nn * (me + 1) / nwg;
+
   // ---------------- init: r = F (or r0 on a restart), z = M^-1 r, r.z, r.r
   const bool restart = A.r0 != nullptr;
   const double* src = restart ? A.r0 : A.F;
   double acc[2] = {0, 0};
-  for (int tile = me; tile < ntiles; tile += nwg) {
-    const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
-    const int gi = i0 + tx, gj = j0 + ty;
-    if (node_thread && gi <= G.nx && gj <= G.ny) {
-      const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
+  for (int64_t n = lo + threadIdx.x; n < hi; n += C::NT) {
+    {
       double rv[3] = {src[3 * n], src[3 * n + 1], src[3 * n + 2]}, zv[3], m[6];
       for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
       sym3_apply(m, rv, zv);
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
           const double ad = ((fix >> d) & 1u) ? pd : v[d];
           st_coh(&pnew[3 * n + d], pd);
           pap[0] += pd * ad;
-          A.Ap[3 * n + d] = ad;
+          st_coh(&A.Ap[3 * n + d], ad);  // read by the node range's owner
         }
       }
       __syncthreads();
@@ -358,15 +363,13 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
 
     // ---------------- phase B: x, r, z updates, r.z, r.r
     acc[0] = 0; acc[1] = 0;
-    for (int tile = me; tile < ntiles; tile += nwg) {
-      const int gi = (tile % ntx) * C::TX + tx, gj = (tile / ntx) * C::TY + ty;
-      if (node_thread && gi <= G.nx && gj <= G.ny) {
-        const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
+    for (int64_t n = lo + threadIdx.x; n < hi; n += C::NT) {
+      {
         double rv[3], zv[3], m[6];
         for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
         for (int d = 0; d < 3; d++) {
           A.x[3 * n + d] += alpha * ld_coh(&pnew[3 * n + d]);
-          rv[d] = A.r[3 * n + d] - alpha * A.Ap[3 * n + d];
+          rv[d] = A.r[3 * n + d] - alpha * ld_coh(&A.Ap[3 * n + d]);
           A.r[3 * n + d] = rv[d];
         }
         sym3_apply(m, rv, zv);
