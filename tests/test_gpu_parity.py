@@ -194,3 +194,43 @@ def test_general_mesh_device_setup_and_loads(ctx):
     np.testing.assert_allclose(F, op.pressure_load(1e4, pt).astype(np.float32), rtol=2e-7,
                                atol=0)
     plan.close()
+
+
+def test_rectangular_plate_full_solve(ctx):
+    """nx != ny and dx != dy (90 x 37 elements on 10 m x 4 m): the resident
+    single-reduction kernel, the mixed path and the moments against the
+    oracle."""
+    mesh = P.rect_mesh(90, 37, width=10.0, height=4.0)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    plan = Plan(ctx, mesh, E, NU, 0.08, mask)
+    assert plan.structured
+    op = O.OracleProblem(mesh, E, NU, 0.08, mask)
+    F = plan.pressure_load(1e4, np.array([[3e4, 7.0, 1.5, 0.8]]))
+    np.testing.assert_allclose(F, op.pressure_load(1e4, np.array([[3e4, 7.0, 1.5, 0.8]])).astype(np.float32),
+                               rtol=2e-7, atol=0)
+    x64, _, st = plan.solve(F, tolerance=1e-8, max_iterations=20000)
+    xo, so = op.pcg(F.astype(np.float64), tol=1e-8)
+    assert st["converged"] and st["trueRelResidual"] <= 1e-8
+    assert abs(st["iterations"] - so["iterations"]) <= 3 + so["iterations"] // 100
+    assert O.max_nodal_w_error(x64, xo) < 1e-6
+    xm, _, sm = plan.solve(F, tolerance=1e-8, max_iterations=200000, precision=_lib.PRECISION_MIXED)
+    assert sm["converged"] and sm["trueRelResidual"] <= 1e-8
+    assert O.max_nodal_w_error(xm, xo) < 1e-6
+    for d, r in zip(plan.moments(x64), op.moments(x64)):
+        np.testing.assert_allclose(d, r, rtol=2e-7, atol=2e-7 * np.abs(r).max())
+    plan.close()
+
+
+def test_rectangular_plate_strided_iterates(ctx):
+    """700 x 300 elements on 10 m x 4.3 m (too large to be resident): the
+    strided kernel's first 30 iterates equal the oracle's."""
+    mesh = P.rect_mesh(700, 300, width=10.0, height=4.3)
+    mask = P.constrained_mask(mesh, [{"type": "pinned", "location": "all_edges"}])
+    plan = Plan(ctx, mesh, E, NU, 0.05, mask)
+    op = O.OracleProblem(mesh, E, NU, 0.05, mask)
+    F = plan.pressure_load(1e4, np.array([[3e4, 3.0, 2.0, 1.0]]))
+    x64, _, st = plan.solve(F, max_iterations=30)
+    assert st["iterations"] == 30 and not st["converged"]
+    xo, so = op.pcg(F.astype(np.float64), max_iter=30)
+    assert rel(x64, xo) < 1e-9
+    plan.close()
