@@ -34,7 +34,8 @@ struct Cnn {
   int adam_t = 0;
   double last_grad_norm = 0.0;
   int linear = 1;       // no ReLU, no biases
-  int line_search = 1;  // energy-optimal step on the network's correction
+  int line_search = 1;  // energy-optimal step on the network's correction (else unit step)
+  int fcg_window = 8;   // directions kept by flexible PCG (FCG(m))
 
   DevBuf<float> params, grads, adam_m, adam_v;
   DevBuf<uint16_t> wf, wt;

@@ -704,6 +704,8 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   st = cfg && cfg->rotation_scale > 0 ? cfg->rotation_scale : 0.3;
   linear = cfg && cfg->relu ? 0 : 1;
   line_search = cfg && cfg->fixed_step ? 0 : 1;
+  fcg_window = cfg && cfg->fcg_window > 0 ? cfg->fcg_window : 8;
+  if (fcg_window > 16) return fail(PP_ERR_INVALID, "fcg_window must be at most 16");
   // padded level-0 grid: 4k + 1 nodes per side
   W[0] = ((nx + 3) / 4) * 4 + 1;
   H[0] = ((ny + 3) / 4) * 4 + 1;

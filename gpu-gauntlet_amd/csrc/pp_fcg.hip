@@ -1,16 +1,20 @@
 // Flexible PCG with the learned preconditioner, the training loop, and the
 // C ABI of networks.
 //
-// FCG(1) (Notay's flexible CG truncated to one previous direction):
-//   r = F, z = M(r), p = z
-//   loop: q = K p; alpha = p.r / p.q; x += alpha p; r -= alpha q
-//         stop when |r| < tol or |r| / |F| < tol (the rule of pcg.ts:174)
-//         z = M(r); beta = -(z.q) / (p.q); p = z + beta p
-// The K-orthogonalisation of z against the last direction keeps the method
-// convergent for a preconditioner that is not a fixed SPD matrix (the
-// network is nonlinear through its ReLUs and its input scaling).  Scalars
-// stay on the device; the host reads the convergence flag every few
-// iterations, and the update kernels stop changing x and r once it is set.
+// FCG(m) (Notay's flexible CG truncated to the last m directions):
+//   r = F, z = M(r), p_0 = z
+//   loop k: q_k = K p_k; alpha = p_k.r / p_k.q_k; x += alpha p_k; r -= alpha q_k
+//           stop when |r| < tol or |r| / |F| < tol (the rule of pcg.ts:174)
+//           z = M(r); p_{k+1} = z - sum over the last m directions j of
+//                               (z.q_j / p_j.q_j) p_j
+// The K-orthogonalisation of z against recent directions keeps the method
+// convergent for a preconditioner that is not a fixed SPD matrix (a trained
+// network is neither symmetric nor, with the energy-optimal step, linear);
+// with m = 1 the orthogonality lost to such a preconditioner stalls it on
+// large plates.  Directions and their K-products live in rings of m + 1
+// vectors.  Scalars stay on the device; the host reads the convergence flag
+// every few iterations, and the update kernels stop changing x and r once it
+// is set.
 
 #include <math.h>
 #include <string.h>
@@ -25,9 +29,19 @@ namespace pp {
 
 namespace {
 
+constexpr int kMaxWin = 16;
+
 struct FcgState {
-  double pq, pr, alpha, rr, zq, beta, fnorm, tol;
+  double pq, pr, alpha, rr, fnorm, tol;
+  double pqr[kMaxWin + 1];  // p.q of the directions in the ring
+  double beta[kMaxWin];     // the coefficients of the next direction
   int it, done, breakdown, converged;
+};
+
+// the directions a new one is orthogonalised against: ring slots, count
+struct Win {
+  int slot[kMaxWin];
+  int w;
 };
 
 constexpr int kBlocks = 256;
@@ -81,13 +95,14 @@ __device__ void sum2(const double* part, double& a, double& b) {
 }
 
 // after q = K p: pq, pr -> alpha (breakdown when |p.q| < 1e-30, pcg.ts:154)
-__global__ void k_fcg_alpha(const double* part, FcgState* S) {
+__global__ void k_fcg_alpha(const double* part, FcgState* S, int slot) {
   if (S->done) return;  // uniform over the block
   double pq, pr;
   sum2(part, pq, pr);
   if (threadIdx.x) return;
   S->pq = pq;
   S->pr = pr;
+  S->pqr[slot] = pq;
   if (fabs(S->pq) < 1e-30) {
     S->breakdown = 1;
     S->done = 1;
@@ -138,22 +153,50 @@ __global__ void k_fcg_check(const double* part, FcgState* S) {
   }
 }
 
-// after z = M(r): zq -> beta
-__global__ void k_fcg_beta(const double* part, FcgState* S) {
+// after z = M(r): per-block partials of z.q_j for the window (block row j)
+__global__ void k_fcg_zq(int64_t n, const double* z, const double* Q, Win win, const FcgState* S,
+                         double* part) {
+  __shared__ double red[256];
   if (S->done) return;
-  double zq, unused;
-  sum2(part, zq, unused);
-  if (threadIdx.x) return;
-  S->zq = zq;
-  S->beta = -zq / S->pq;
+  const double* q = Q + (size_t)win.slot[blockIdx.y] * n;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += z[i] * q[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * kBlocks + blockIdx.x] = red[0];
 }
 
-// p = z + beta p (beta = 0: p = z)
-__global__ void k_fcg_dir(int64_t n, const double* z, double* p, const FcgState* S, int first) {
+// beta_j = -(z.q_j) / (p_j.q_j), partials summed in a fixed order
+__global__ void k_fcg_betas(const double* part, Win win, FcgState* S) {
+  __shared__ double sa[kBlocks];
+  if (S->done) return;
+  for (int j = 0; j < win.w; j++) {
+    sa[threadIdx.x] = part[(size_t)j * kBlocks + threadIdx.x];
+    __syncthreads();
+    for (int k = kBlocks / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) sa[threadIdx.x] += sa[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) S->beta[j] = -sa[0] / S->pqr[win.slot[j]];
+    __syncthreads();
+  }
+}
+
+// p_out = z + sum_j beta_j p_j (no window: p_out = z)
+__global__ void k_fcg_dir(int64_t n, const double* z, double* P, Win win, int out,
+                          const FcgState* S, int first) {
   if (S->done && !first) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  p[i] = first ? z[i] : z[i] + S->beta * p[i];
+  double v = z[i];
+  for (int j = 0; j < win.w; j++) v += S->beta[j] * P[(size_t)win.slot[j] * n + i];
+  P[(size_t)out * n + i] = v;
 }
 
 // dst = (float)(a - b) (b may be null)
@@ -185,27 +228,31 @@ __global__ void k_fcg_init(FcgState* S, double fnorm, double tol, double rr) {
 
 }  // namespace
 
-// FCG from x (zero on a cold start) with r = F - K x given in pl->r.
+// FCG from x (zero on a cold start) with r = F - K x given in pl->r; P and Q
+// are rings of m + 1 vectors each (directions, their K-products).
 static int fcg_run(pp_plan* pl, Cnn& net, double fnorm, double tol, int maxit, FcgState* hs,
-                   DevBuf<FcgState>& S, DevBuf<double>& part) {
+                   DevBuf<FcgState>& S, DevBuf<double>& part, double* P, double* Q) {
   hipStream_t s = pl->ctx->stream;
   const int64_t n = pl->n_dof;
+  const int m = net.fcg_window, R = m + 1;
   int rc;
   hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(64), 0, s, S.p, fnorm, tol, 0.0);
-  // z = M(r), p = z
+  // z = M(r), p_0 = z
   if ((rc = net.precondition(pl->r.p, pl->z.p, net.line_search != 0))) return rc;
-  hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p, pl->p0.p,
-                     (const FcgState*)S.p, 1);
+  Win none{};
+  hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p, P, none,
+                     0, (const FcgState*)S.p, 1);
   PP_HIP(hipGetLastError());
   for (int k = 0; k < maxit; k++) {
-    if ((rc = struct_apply_k(pl, pl->p0.p, pl->Ap.p))) return rc;
-    hipLaunchKernelGGL(k_dots, dim3(kBlocks), dim3(256), 0, s, n, (const double*)pl->p0.p,
-                       (const double*)pl->Ap.p, (const double*)pl->p0.p, (const double*)pl->r.p,
-                       part.p);
-    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p);
+    const int cur = k % R;
+    double* p = P + (size_t)cur * n;
+    double* q = Q + (size_t)cur * n;
+    if ((rc = struct_apply_k(pl, p, q))) return rc;
+    hipLaunchKernelGGL(k_dots, dim3(kBlocks), dim3(256), 0, s, n, (const double*)p,
+                       (const double*)q, (const double*)p, (const double*)pl->r.p, part.p);
+    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p, cur);
     hipLaunchKernelGGL(k_fcg_update, dim3(kBlocks), dim3(256), 0, s, n, pl->x.p, pl->r.p,
-                       (const double*)pl->p0.p, (const double*)pl->Ap.p, (const FcgState*)S.p,
-                       part.p);
+                       (const double*)p, (const double*)q, (const FcgState*)S.p, part.p);
     hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p);
     PP_HIP(hipGetLastError());
     if ((k + 1) % kCheckEvery == 0 || k + 1 == maxit) {
@@ -214,12 +261,15 @@ static int fcg_run(pp_plan* pl, Cnn& net, double fnorm, double tol, int maxit, F
       if (hs->done) break;
     }
     if ((rc = net.precondition(pl->r.p, pl->z.p, net.line_search != 0))) return rc;
-    hipLaunchKernelGGL(k_dots, dim3(kBlocks), dim3(256), 0, s, n, (const double*)pl->z.p,
-                       (const double*)pl->Ap.p, (const double*)nullptr, (const double*)nullptr,
+    Win win{};
+    win.w = std::min(k + 1, m);
+    for (int j = 0; j < win.w; j++) win.slot[j] = (k - j) % R;
+    hipLaunchKernelGGL(k_fcg_zq, dim3(kBlocks, win.w), dim3(256), 0, s, n,
+                       (const double*)pl->z.p, (const double*)Q, win, (const FcgState*)S.p,
                        part.p);
-    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, S.p);
-    hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p,
-                       pl->p0.p, (const FcgState*)S.p, 0);
+    hipLaunchKernelGGL(k_fcg_betas, dim3(1), dim3(kBlocks), 0, s, (const double*)part.p, win, S.p);
+    hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p, P, win,
+                       (k + 1) % R, (const FcgState*)S.p, 0);
     PP_HIP(hipGetLastError());
   }
   PP_HIP(hipMemcpyAsync(hs, S.p, sizeof(FcgState), hipMemcpyDeviceToHost, s));
@@ -235,9 +285,12 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   hipStream_t s = pl->ctx->stream;
   const int64_t n = pl->n_dof;
   DevBuf<FcgState> S;
-  DevBuf<double> part;
+  DevBuf<double> part, ring;
+  const int R = net.fcg_window + 1;
   int rc;
-  if ((rc = S.alloc(1)) || (rc = part.alloc(2 * kBlocks))) return rc;
+  if ((rc = S.alloc(1)) || (rc = part.alloc((size_t)std::max(2, kMaxWin) * kBlocks)) ||
+      (rc = ring.alloc((size_t)2 * R * n)))
+    return rc;
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   PP_HIP(hipMemsetAsync(pl->x.p, 0, n * 8, s));
   PP_HIP(hipMemcpyAsync(pl->r.p, pl->F.p, n * 8, hipMemcpyDeviceToDevice, s));
@@ -251,7 +304,8 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     hs.converged = 1;
   } else {
     for (int k = 0; k <= 4; k++) {
-      if ((rc = fcg_run(pl, net, fnorm, o->tolerance, o->max_iterations - total, &hs, S, part)))
+      if ((rc = fcg_run(pl, net, fnorm, o->tolerance, o->max_iterations - total, &hs, S, part,
+                        ring.p, ring.p + (size_t)R * n)))
         return rc;
       if (sweeps < PP_MAX_SWEEPS) st->sweep_iterations[sweeps] = hs.it;
       total += hs.it;

@@ -67,7 +67,8 @@ TRAIN_SYNTHETIC, TRAIN_SOLVER = 0, 1
 class CnnConfig(C.Structure):
     _fields_ = [("channels", C.c_int32 * 3), ("bottleneck_dilation", C.c_int32),
                 ("rotation_scale", C.c_double), ("smoother_weight", C.c_double),
-                ("relu", C.c_int32), ("fixed_step", C.c_int32)]
+                ("relu", C.c_int32), ("fixed_step", C.c_int32),
+                ("fcg_window", C.c_int32)]
 
 
 class TrainOpts(C.Structure):

@@ -169,11 +169,12 @@ class Net:
     """The learned preconditioner of one structured plan (pp_cnn_*)."""
 
     def __init__(self, plan, seed=1, channels=(16, 32, 64), bottleneck_dilation=1,
-                 rotation_scale=0.3, smoother_weight=0.0, relu=False, fixed_step=False):
+                 rotation_scale=0.3, smoother_weight=0.0, relu=False, fixed_step=False,
+                 fcg_window=0):
         self.plan, self.lib = plan, plan.lib
         cfg = _lib.CnnConfig((C.c_int32 * 3)(*channels), int(bottleneck_dilation),
                              float(rotation_scale), float(smoother_weight), int(bool(relu)),
-                             int(bool(fixed_step)))
+                             int(bool(fixed_step)), int(fcg_window))
         h = C.c_void_p()
         check(self.lib.pp_cnn_create(plan.h, C.byref(cfg), int(seed), C.byref(h)))
         self.h = h

@@ -227,8 +227,14 @@ typedef struct {
                                   1: ReLU after every conv but the last, with biases */
   int32_t fixed_step;          /* 0: scale the network's correction by its energy-optimal
                                   step in every application, so an application is never
-                                  worse than the smoothing alone; 1: unit step (with
-                                  relu = 0 the preconditioner is then a fixed linear map) */
+                                  worse than the smoothing alone (the preconditioner is then
+                                  nonlinear: use fcg_window > 1); 1: unit step (with relu = 0
+                                  a fixed linear map, as in training, but a trained network
+                                  need not then be a convergent preconditioner) */
+  int32_t fcg_window;          /* directions flexible PCG K-orthogonalises against (FCG(m));
+                                  0 = 8, at most 16.  With the energy-optimal step, FCG(1)
+                                  stalls on large plates (1024^2, zero network: 5e-3 after
+                                  30,000 iterations) where FCG(8) converges (29,897) */
 } pp_cnn_config;
 
 #define PP_TRAIN_CURVE 64

@@ -111,14 +111,15 @@ def test_loss_and_gradient_match_oracle(ctx, relu):
 
 
 def test_fcg_with_the_zero_network_converges(ctx):
-    """With every parameter zero the preconditioner is three block-Jacobi
-    smoothing steps (a fixed polynomial preconditioner): flexible PCG must then
-    reach 1e-8 with the block-Jacobi answer in fewer iterations than PCG."""
+    """With every parameter zero the preconditioner is block-Jacobi smoothing
+    around a block-Jacobi correction taken with its energy-optimal step:
+    flexible PCG (FCG(8), the default window) must reach 1e-8 with the
+    block-Jacobi answer in fewer iterations than PCG."""
     n = 64
     mesh, mask, et, plan, op = plate(ctx, n, seed=8)
     F = plan.pressure_load(1e4, np.array([[3e4, 3.0, 6.0, 1.5]]))
     x_bj, _, st_bj = plan.solve(F, tolerance=1e-8, max_iterations=20000)
-    net = Net(plan, seed=2, fixed_step=True)
+    net = Net(plan, seed=2)
     assert 0.05 < net.omega < 1.5
     net.set_params(np.zeros(net.n_params, np.float32))
     net.attach()
@@ -136,8 +137,8 @@ def test_training_gains_inside_flexible_pcg(ctx):
     seeded pressure loads), the network lowers its energy-norm loss and then
     solves a new load case to 1e-8, with the FP64 block-Jacobi answer, in
     fewer iterations than block-Jacobi PCG, than the same preconditioner
-    without the network (zero parameters: the fixed smoothing polynomial)
-    and than the untrained network."""
+    without the network (zero parameters) and than the untrained network;
+    and FCG(8) beats FCG(1) with the trained network."""
     n = 64
     mesh, mask, et, plan, op = plate(ctx, n, seed=8)
     F = plan.pressure_load(1e4, np.array([[3e4, 3.0, 6.0, 1.5]]))
@@ -145,7 +146,7 @@ def test_training_gains_inside_flexible_pcg(ctx):
     assert st_bj["converged"]
     its = {}
     for tag in ("zero", "untrained", "trained"):
-        net = Net(plan, seed=2, fixed_step=(tag == "zero"))
+        net = Net(plan, seed=2)
         if tag == "zero":
             net.set_params(np.zeros(net.n_params, np.float32))
         if tag == "trained":
@@ -154,6 +155,7 @@ def test_training_gains_inside_flexible_pcg(ctx):
             c = tr["loss_curve"]
             print("train %.1fs" % tr["seconds"], ["%.3g" % v for v in c[::8]])
             assert np.mean(c[-8:]) < 0.95 * np.mean(c[:8]), c
+            trained = net.params()
         net.attach()
         x, _, st = plan.solve(F, tolerance=1e-8, max_iterations=5000, precond=LEARNED)
         its[tag] = st["iterations"] if st["converged"] else None
@@ -161,10 +163,18 @@ def test_training_gains_inside_flexible_pcg(ctx):
             assert st["converged"] and st["trueRelResidual"] <= 1e-8, (tag, st)
             assert op.true_rel_residual(x, F.astype(np.float64)) <= 1e-8
             assert O.max_nodal_w_error(x, x_bj) < 1e-6
+        if tag == "trained":
+            net.close()
+            net = Net(plan, seed=2, fcg_window=1)
+            net.set_params(trained)
+            net.attach()
+            _, _, s1 = plan.solve(F, tolerance=1e-8, max_iterations=5000, precond=LEARNED)
+            its["trained_fcg1"] = s1["iterations"] if s1["converged"] else None
         net.close()
     print("iterations: block-Jacobi PCG", st_bj["iterations"], its)
     assert its["trained"] < its["zero"] < st_bj["iterations"]
     assert its["untrained"] is None or its["trained"] < its["untrained"]
+    assert its["trained_fcg1"] is None or its["trained"] < its["trained_fcg1"]
 
 
 def test_learned_solve_edge_cases(ctx):
