@@ -21,6 +21,45 @@ struct Layer {
   int64_t wf_off, wt_off;           // packed BF16 forward / backward-data weights
 };
 
+// Galerkin geometric multigrid on the structured node grid, used as the
+// smoother around the network when the configuration asks for it
+// (pp_mg.hip).  Level 0 is the plan's own operator (matrix-free FP64 K, its
+// block-Jacobi inverse and constraint bits); level l+1 halves the element
+// counts of level l (vertex-centred: coarse node I is fine node 2I) with
+// bilinear prolongation P_l (index space, per DOF component, zero on the
+// excluded DOFs of level l) and the coarse operator A_{l+1} = P_l^T A_l P_l,
+// assembled from coarse element matrices (each the sum over its four child
+// elements of P_loc^T K_child P_loc, which is exact for bilinear P).  Coarse
+// levels keep A as an FP32 9-point block stencil; the coarsest is solved by
+// a dense FP64 inverse.
+struct MgLevel {
+  int nx = 0, ny = 0;    // elements per side
+  int64_t nn = 0;        // nodes (nx + 1) (ny + 1)
+  double omega = 0.5;    // block-Jacobi smoothing weight 4 / (3 lambda_max(B A))
+  DevBuf<float> st;      // level >= 1: [node][9 neighbours][3][3], neighbour (dj + 1) * 3 + di + 1
+  DevBuf<double> minv;   // level >= 1: 6 per node (symmetric 3x3 block inverse)
+  DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
+  DevBuf<double> b, x, t;  // right-hand side, iterate, work (level 0: t only)
+};
+
+struct Mg {
+  pp_plan* pl = nullptr;
+  hipStream_t stream = nullptr;
+  int nu = 2;  // block-Jacobi sweeps before and after each coarse correction
+  static constexpr int kMaxLevels = 16;
+  MgLevel lv[kMaxLevels];
+  int nl = 0;
+  int nd_coarse = 0;
+  DevBuf<double> cinv;  // dense inverse of the coarsest operator [nd][nd]
+  DevBuf<double> dotp;
+  int setup(pp_plan* plan, double omega0, int sweeps);
+  int vcycle(const double* b, double* x);  // x = V b on level 0 (device FP64 DOF vectors)
+  int apply_level(int l, const double* x, double* y);  // y = A_l x
+ private:
+  int cycle(int l);
+  int estimate_omega(int l);
+};
+
 struct Cnn {
   pp_plan* pl = nullptr;
   hipStream_t stream = nullptr;
@@ -36,6 +75,7 @@ struct Cnn {
   int linear = 1;       // no ReLU, no biases
   int line_search = 1;  // energy-optimal step on the network's correction (else unit step)
   int fcg_window = 8;   // directions kept by flexible PCG (FCG(m))
+  Mg* mg = nullptr;     // V-cycle smoother in place of w B (null: w B)
 
   DevBuf<float> params, grads, adam_m, adam_v;
   DevBuf<uint16_t> wf, wt;
@@ -43,13 +83,14 @@ struct Cnn {
   DevBuf<uint16_t> g_c0b, g_c0, g_cat0, g_d1, g_c1, g_cat1, g_bb, g_a2, g_r2, g_a1, g_r1, g_a0,
       g_out;
   DevBuf<float> out32, wpart, bpart, logt;
-  DevBuf<double> zbj, mout, dvec, kvec, evec, rvec, scal, dotp, bumps, z0v, z1v, r1v;
+  DevBuf<double> zbj, mout, dvec, kvec, evec, rvec, scal, dotp, bumps, z0v, z1v, r1v, mgz;
 
   const TV* mk(const TV& v) const { return linear ? nullptr : &v; }  // ReLU mask or none
   int setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed);
   int pack_weights();
   int apply(const double* r, double* out);         // network alone, device vectors
   int precondition(const double* r, double* out, bool line_search);  // smoothing + network + smoothing
+  int smooth(const double* in, const double* base, double* out);  // out = base + S in (base may be null)
   int estimate_omega();
   int backward();
   int loss_grad(const double* e, double* loss_dev);  // e on the device

@@ -818,6 +818,10 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   }
   if (cfg && cfg->smoother_weight > 0) omega = cfg->smoother_weight;
   else if ((rc = estimate_omega())) return rc;
+  if (cfg && cfg->multigrid) {
+    mg = new Mg();
+    if ((rc = mg->setup(pl, omega, cfg->mg_sweeps)) || (rc = mgz.alloc(pl->n_dof))) return rc;
+  }
   PP_HIP(hipStreamSynchronize(stream));
   return PP_OK;
 }
@@ -919,18 +923,36 @@ int Cnn::backward() {
   return PP_OK;
 }
 
-// The preconditioner: damped block-Jacobi smoothing around the network,
-//   z0 = w B r;  r1 = r - K z0;  u = N(r1);  z1 = z0 + g u;  z = z1 + w B (r1 - g K u)
-// (the network takes the smooth error the smoother leaves).  With
-// line_search the step g = u.r1 / u.K u minimises the energy error of z1
-// along u, so an application is never worse than the smoothing alone; in
-// training g = 1 (the loss sees the network's own output).
+// out = base + S in: S = w B (block Jacobi) or the multigrid V-cycle.
+int Cnn::smooth(const double* in, const double* base, double* out) {
+  if (!mg) {
+    const int64_t nn = pl->n_nodes;
+    hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, (const double*)pl->minv.p,
+                       in, omega, base, out);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+  }
+  if (!base) return mg->vcycle(in, out);
+  int rc;
+  if ((rc = mg->vcycle(in, mgz.p))) return rc;
+  const int64_t nd = pl->n_dof;
+  hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, base, 1.0,
+                     (const double*)mgz.p, out);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+// The preconditioner: smoothing S (w B, or a multigrid V-cycle) around the
+// network,
+//   z0 = S r;  r1 = r - K z0;  u = N(r1);  z1 = z0 + g u;  z = z1 + S (r1 - g K u)
+// (the network takes the error the smoother leaves).  With line_search the
+// step g = u.r1 / u.K u minimises the energy error of z1 along u, so an
+// application is never worse than the smoothing alone; in training g = 1
+// (the loss sees the network's own output).
 int Cnn::precondition(const double* r, double* out, bool line_search) {
   int rc;
-  const int64_t nd = pl->n_dof, nn = pl->n_nodes;
-  const double* B = pl->minv.p;
-  hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, B, r, omega,
-                     (const double*)nullptr, z0v.p);
+  const int64_t nd = pl->n_dof;
+  if ((rc = smooth(r, nullptr, z0v.p))) return rc;
   if ((rc = struct_apply_k(pl, z0v.p, kvec.p))) return rc;
   hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, r, -1.0,
                      (const double*)kvec.p, r1v.p);
@@ -950,19 +972,17 @@ int Cnn::precondition(const double* r, double* out, bool line_search) {
                      (const double*)g, 1.0, (const double*)mout.p, z1v.p);
   hipLaunchKernelGGL(k_axpy_dev, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)r1v.p,
                      (const double*)g, -1.0, (const double*)kvec.p, r1v.p);
-  hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, B, (const double*)r1v.p,
-                     omega, (const double*)z1v.p, out);
   PP_HIP(hipGetLastError());
-  return PP_OK;
+  return smooth(r1v.p, z1v.p, out);
 }
 
 // One pair: e (device, FP64), r = K e, z = precondition(r), e_f = e - z.
-// Loss e_f^T K e_f / e^T K e.  With S = I - w B K, e_f = S (e - z0 - u) for
-// the network output u, so dL/du = -2 S^T K e_f / e^T K e and
-// S^T K e_f = q1 - w K B q1 with q1 = K e_f.
+// Loss e_f^T K e_f / e^T K e.  With E = I - S K (S the symmetric smoother),
+// e_f = E (e - z0 - u) for the network output u, so dL/du = -2 E^T K e_f /
+// e^T K e and E^T K e_f = q1 - K S q1 with q1 = K e_f.
 int Cnn::loss_grad(const double* e, double* loss_dev) {
   int rc;
-  const int64_t nd = pl->n_dof, nn = pl->n_nodes;
+  const int64_t nd = pl->n_dof;
   if ((rc = struct_apply_k(pl, e, rvec.p))) return rc;
   if ((rc = precondition(rvec.p, dvec.p, false))) return rc;  // z (scal[0] = s of the network input)
   hipLaunchKernelGGL(k_sub, dim3(nblk(nd)), dim3(256), 0, stream, nd, e, (const double*)dvec.p,
@@ -974,11 +994,10 @@ int Cnn::loss_grad(const double* e, double* loss_dev) {
                      kDotBlocks, scal.p + 2);
   hipLaunchKernelGGL(k_loss, dim3(1), dim3(64), 0, stream, (const double*)scal.p + 2, loss_dev,
                      scal.p + 4);  // scal[4] = 2 / eKe
-  hipLaunchKernelGGL(k_bjw, dim3(nblk(nn)), dim3(256), 0, stream, nn, (const double*)pl->minv.p,
-                     (const double*)kvec.p, 1.0, (const double*)nullptr, z0v.p);  // B q1
-  if ((rc = struct_apply_k(pl, z0v.p, z1v.p))) return rc;                          // K B q1
+  if ((rc = smooth(kvec.p, nullptr, z0v.p))) return rc;     // S q1
+  if ((rc = struct_apply_k(pl, z0v.p, z1v.p))) return rc;  // K S q1
   hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)kvec.p,
-                     -omega, (const double*)z1v.p, r1v.p);  // S^T K e_f
+                     -1.0, (const double*)z1v.p, r1v.p);  // E^T K e_f
   // dL/do = -(2 / eKe) S^T K e_f * s / scale_c
   hipLaunchKernelGGL(k_lin, dim3(1), dim3(1), 0, stream, (int64_t)1, (const double*)scal.p + 15,
                      -1.0, (const double*)scal.p + 4, scal.p + 5);  // scal[5] = 0 - scal[4]
@@ -1072,6 +1091,8 @@ int Cnn::adam_step(float lr, float max_norm) {
 }
 
 Cnn::~Cnn() {
+  delete mg;
+  mg = nullptr;
   if (!pl) return;
   if (pl->learned == this) pl->learned = nullptr;
   (void)hipStreamSynchronize(stream);

@@ -398,6 +398,52 @@ int pp_cnn_apply(pp_cnn* c, const double* r, double* z) {
   return PP_OK;
 }
 
+int pp_cnn_mg_levels(const pp_cnn* c, int32_t* nx, int32_t* ny, double* omega, int32_t cap) {
+  if (!c) return fail(PP_ERR_INVALID, "NULL argument");
+  const Mg* mg = c->net.mg;
+  if (!mg) return 0;
+  for (int l = 0; l < mg->nl && l < cap; l++) {
+    if (nx) nx[l] = mg->lv[l].nx;
+    if (ny) ny[l] = mg->lv[l].ny;
+    if (omega) omega[l] = l + 1 < mg->nl ? mg->lv[l].omega : 0.0;
+  }
+  return mg->nl;
+}
+
+int pp_cnn_mg_apply(pp_cnn* c, int32_t level, const double* x, double* y) {
+  if (!c || !x || !y) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  Mg* mg = n.mg;
+  if (!mg) return fail(PP_ERR_UNSUPPORTED, "the network was created without multigrid");
+  if (level < 0 || level >= mg->nl - 1)
+    return fail(PP_ERR_INVALID, "level must be below the coarsest");
+  PP_HIP(hipSetDevice(n.pl->ctx->device));
+  MgLevel& L = mg->lv[level];
+  double* dx = level ? L.x.p : n.evec.p;
+  double* dy = level ? L.t.p : n.rvec.p;
+  const int64_t nd = 3 * L.nn;
+  PP_HIP(hipMemcpyAsync(dx, x, nd * 8, hipMemcpyHostToDevice, n.stream));
+  int rc = mg->apply_level(level, dx, dy);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(y, dy, nd * 8, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
+int pp_cnn_mg_cycle(pp_cnn* c, const double* b, double* x) {
+  if (!c || !b || !x) return fail(PP_ERR_INVALID, "NULL argument");
+  Cnn& n = c->net;
+  if (!n.mg) return fail(PP_ERR_UNSUPPORTED, "the network was created without multigrid");
+  pp_plan* pl = n.pl;
+  PP_HIP(hipSetDevice(pl->ctx->device));
+  PP_HIP(hipMemcpyAsync(n.evec.p, b, pl->n_dof * 8, hipMemcpyHostToDevice, n.stream));
+  int rc = n.mg->vcycle(n.evec.p, n.rvec.p);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(x, n.rvec.p, pl->n_dof * 8, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
 int pp_cnn_loss_grad(pp_cnn* c, const double* e, double* loss, float* grad) {
   if (!c || !e || !loss) return fail(PP_ERR_INVALID, "NULL argument");
   Cnn& n = c->net;

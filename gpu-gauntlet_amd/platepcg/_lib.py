@@ -68,7 +68,7 @@ class CnnConfig(C.Structure):
     _fields_ = [("channels", C.c_int32 * 3), ("bottleneck_dilation", C.c_int32),
                 ("rotation_scale", C.c_double), ("smoother_weight", C.c_double),
                 ("relu", C.c_int32), ("fixed_step", C.c_int32),
-                ("fcg_window", C.c_int32)]
+                ("fcg_window", C.c_int32), ("multigrid", C.c_int32), ("mg_sweeps", C.c_int32)]
 
 
 class TrainOpts(C.Structure):
@@ -126,6 +126,9 @@ SIGNATURES = {
     "pp_cnn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_cnn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_cnn_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pp_cnn_mg_levels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    "pp_cnn_mg_apply": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pp_cnn_mg_cycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_cnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
     "pp_cnn_gen_error": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "pp_cnn_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), C.POINTER(TrainStats)]),

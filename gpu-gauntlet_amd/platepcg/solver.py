@@ -170,11 +170,15 @@ class Net:
 
     def __init__(self, plan, seed=1, channels=(16, 32, 64), bottleneck_dilation=1,
                  rotation_scale=0.3, smoother_weight=0.0, relu=False, fixed_step=False,
-                 fcg_window=0):
+                 fcg_window=0, multigrid=False, mg_sweeps=0):
+        """multigrid: the smoother around the network is a Galerkin V-cycle
+        (mg_sweeps block-Jacobi sweeps per level before and after the coarse
+        correction, 0 = 2) instead of w B."""
         self.plan, self.lib = plan, plan.lib
         cfg = _lib.CnnConfig((C.c_int32 * 3)(*channels), int(bottleneck_dilation),
                              float(rotation_scale), float(smoother_weight), int(bool(relu)),
-                             int(bool(fixed_step)), int(fcg_window))
+                             int(bool(fixed_step)), int(fcg_window), int(bool(multigrid)),
+                             int(mg_sweeps))
         h = C.c_void_p()
         check(self.lib.pp_cnn_create(plan.h, C.byref(cfg), int(seed), C.byref(h)))
         self.h = h
@@ -207,6 +211,26 @@ class Net:
         z = np.empty_like(r)
         check(self.lib.pp_cnn_apply(self.h, _ptr(r), _ptr(z)))
         return z
+
+    def mg_levels(self):
+        """[(nx, ny, omega)] per multigrid level ([] without multigrid)."""
+        nx, ny = (C.c_int32 * 16)(), (C.c_int32 * 16)()
+        om = (C.c_double * 16)()
+        n = self.lib.pp_cnn_mg_levels(self.h, nx, ny, om, 16)
+        check(min(n, 0))
+        return [(nx[l], ny[l], om[l]) for l in range(n)]
+
+    def mg_apply(self, level, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        check(self.lib.pp_cnn_mg_apply(self.h, int(level), _ptr(x), _ptr(y)))
+        return y
+
+    def mg_cycle(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        check(self.lib.pp_cnn_mg_cycle(self.h, _ptr(b), _ptr(x)))
+        return x
 
     def loss_grad(self, e):
         e = np.ascontiguousarray(e, np.float64)

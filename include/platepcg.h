@@ -235,6 +235,13 @@ typedef struct {
                                   0 = 8, at most 16.  With the energy-optimal step, FCG(1)
                                   stalls on large plates (1024^2, zero network: 5e-3 after
                                   30,000 iterations) where FCG(8) converges (29,897) */
+  int32_t multigrid;           /* 0: the smoother S around the network is w B (block Jacobi);
+                                  1: S is a Galerkin geometric-multigrid V-cycle (bilinear
+                                  prolongation, damped block-Jacobi sweeps per level, dense
+                                  coarsest solve).  Needs the element counts per side to halve
+                                  down to a coarsest grid of at most 2,200 DOFs */
+  int32_t mg_sweeps;           /* block-Jacobi sweeps before and after each coarse correction;
+                                  0 = 2 */
 } pp_cnn_config;
 
 #define PP_TRAIN_CURVE 64
@@ -271,9 +278,21 @@ int pp_cnn_set_params(pp_cnn* net, const float* params);
 /* The smoothing weight w in use. */
 double pp_cnn_smoother_weight(const pp_cnn* net);
 /* z = M(r) for host vectors of 3 * node_count doubles: the preconditioner
- * z0 = w B r; z1 = z0 + N(r - K z0); z = z1 + w B (r - K z1), B the 3x3
- * block-Jacobi inverse, N the network (N(v) = B v + its learned correction). */
+ * z0 = S r; z1 = z0 + N(r - K z0); z = z1 + S (r - K z1), S the smoother
+ * (w B, B the 3x3 block-Jacobi inverse, or with multigrid = 1 one V-cycle),
+ * N the network (N(v) = B v + its learned correction). */
 int pp_cnn_apply(pp_cnn* net, const double* r, double* z);
+/* Multigrid smoother diagnostics (config multigrid = 1; no reference
+ * counterpart).  Levels: returns the level count (0 without multigrid) and
+ * fills up to cap entries of the element counts per side and the smoothing
+ * weight of each level (0 on the coarsest, which is solved exactly). */
+int pp_cnn_mg_levels(const pp_cnn* net, int32_t* nx, int32_t* ny, double* omega, int32_t cap);
+/* y = A_l x on level l (below the coarsest), host vectors of
+ * 3 (nx_l + 1) (ny_l + 1) doubles; level 0 is the plan's K with its
+ * constraint rows. */
+int pp_cnn_mg_apply(pp_cnn* net, int32_t level, const double* x, double* y);
+/* x = V b: one V-cycle on level 0 (host vectors of 3 * node_count doubles). */
+int pp_cnn_mg_cycle(pp_cnn* net, const double* b, double* x);
 /* Loss (M(K e) - e)^T K (M(K e) - e) / e^T K e and its gradient in the
  * parameters (grad may be NULL). */
 int pp_cnn_loss_grad(pp_cnn* net, const double* e, double* loss, float* grad);

@@ -202,31 +202,35 @@ class UNetOracle:
         """omega B v (block-Jacobi, per node)."""
         return omega * np.einsum("nij,nj->ni", self.blk, v.reshape(-1, 3)).reshape(-1)
 
-    def precondition(self, params, r, apply_k, omega):
-        """z0 = w B r; z1 = z0 + N(r - K z0); z = z1 + w B (r - K z1)."""
+    def precondition(self, params, r, apply_k, omega, smoother=None):
+        """z0 = S r; z1 = z0 + N(r - K z0); z = z1 + S (r - K z1), S = w B or
+        the given symmetric smoother (v -> S v, e.g. a multigrid V-cycle)."""
+        S = smoother if smoother is not None else (lambda v: self.smooth(v, omega))
         r = np.asarray(r, np.float64)
-        z0 = self.smooth(r, omega)
+        z0 = S(r)
         u = self.forward(params, r - apply_k(z0))
         z1 = z0 + u
-        return z1 + self.smooth(r - apply_k(z1), omega)
+        return z1 + S(r - apply_k(z1))
 
     # ---- loss and gradient
-    def loss_grad(self, params, e, apply_k, omega):
+    def loss_grad(self, params, e, apply_k, omega, smoother=None):
         """Loss e_f^T K e_f / e^T K e of the pair (e, r = K e), e_f = e - M(r)
         with the smoothed preconditioner above, and its gradient in the flat
-        parameters.  With S = I - w B K, e_f = S (e - z0 - u) for the network
-        output u, so dL/du = -2 S^T K e_f / e^T K e.  apply_k: the FP64 operator
-        (with the reference's BC rows)."""
+        parameters.  With E = I - S K (S = w B or the symmetric smoother
+        given), e_f = E (e - z0 - u) for the network output u, so dL/du =
+        -2 E^T K e_f / e^T K e, E^T K e_f = K e_f - K S K e_f.  apply_k: the
+        FP64 operator (with the reference's BC rows)."""
         q = self.q
+        S = smoother if smoother is not None else (lambda v: self.smooth(v, omega))
         e = np.asarray(e, np.float64)
         r = apply_k(e)
-        ef = e - self.precondition(params, r, apply_k, omega)
+        ef = e - self.precondition(params, r, apply_k, omega, smoother)
         q1 = apply_k(ef)
         eke = float(e @ r)
         loss = float(ef @ q1) / eke
         Wq, t, s, z = self.cache
         ny, nx = self.ny, self.nx
-        g = q1 - omega * apply_k(self.smooth(q1, 1.0))
+        g = q1 - apply_k(S(q1))
         gM = (-2.0 / eke) * g.reshape(ny + 1, nx + 1, 3)
         go = np.zeros((self.H0, self.W0, 4))
         go[:ny + 1, :nx + 1, :3] = np.where(self.free, gM * s / self.c, 0.0)
