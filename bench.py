@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--train-epochs", type=int, default=3, help="c5: training epochs")
     ap.add_argument("--learned-max-iterations", type=int, default=40000,
                     help="c5: iteration cap of the learned-preconditioner solve")
+    ap.add_argument("--learned-smoother", default="mg", choices=["mg", "bj"],
+                    help="c5: smoother around the network: Galerkin multigrid V-cycle (mg) or "
+                         "damped block Jacobi (bj)")
     ap.add_argument("--cpu-sample-iters", type=int, default=-1,
                     help="iterations of the CPU baseline sample (0 = skip; default 1200, "
                          "150 for c3)")
@@ -197,7 +200,8 @@ def bench_learned(a):
     w = P.workload("c5")
     ctx = Context(0)
     plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"], element_t=w["element_t"])
-    net = Net(plan, seed=5)
+    mg = a.learned_smoother == "mg"
+    net = Net(plan, seed=5, multigrid=mg)
     t0 = time.perf_counter()
     tr = None
     for k in range(a.train_loads):  # one load case per call: progress stays visible
@@ -213,7 +217,10 @@ def bench_learned(a):
                       precond=_lib.PRECOND_LEARNED)
     net.attach()
     res = {}
-    for tag, o in (("block_jacobi", bj), ("learned", lo)):
+    trained = net.params()
+    for tag, o in (("block_jacobi", bj), ("learned", lo), ("zero_network", lo)):
+        # zero_network: the same preconditioner with every parameter zero
+        net.set_params(np.zeros_like(trained) if tag == "zero_network" else trained)
         for _ in range(a.warmup):
             plan.solve_resident(o)
         t1, dev = time.perf_counter(), []
@@ -226,7 +233,8 @@ def bench_learned(a):
         print(f"{tag}: {st.iterations} iterations, {res[tag]['ms']:.0f} ms", file=sys.stderr,
               flush=True)
         res[tag]["x"] = plan.solution()[0]
-    err = O.max_nodal_w_error(res["learned"].pop("x"), res["block_jacobi"].pop("x"))
+    err = O.max_nodal_w_error(res["learned"].pop("x"), res["block_jacobi"]["x"])
+    err0 = O.max_nodal_w_error(res["zero_network"].pop("x"), res["block_jacobi"].pop("x"))
     n_dof = plan.n_dof
     it_bj = res["block_jacobi"]["iterations"]
     achieved = BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj / (res["block_jacobi"]["device_ms"] * 1e-3) / 1e9
@@ -238,15 +246,22 @@ def bench_learned(a):
         "scaling": "strong", "vs_baseline": None, "dtype": "f64+bf16", "data": "synthetic",
         "config": {"workload": "c5: 1024 x 1024 Q4 plate, t_e = 0.1 exp(0.5 g), g unit Gaussian "
                    "field with correlation length 0.5 m (seed 5), clamped, 10 kPa + 3 Gaussian "
-                   "patches", "dof": n_dof, "precond": "learned U-net (16/32/64, BF16 MFMA) + "
-                   "block-Jacobi smoothing, flexible PCG", "precision": "fp64 outer, bf16 network"},
+                   "patches", "dof": n_dof, "precond": "learned U-net (16/32/64, BF16 MFMA) + " +
+                   ("Galerkin multigrid V-cycle smoothing (7 levels, FP32 coarse stencils)" if mg
+                    else "block-Jacobi smoothing") + ", flexible PCG (FCG(8))",
+                   "precision": "fp64 outer, bf16 network"},
         "learned": res["learned"], "block_jacobi": res["block_jacobi"],
+        "zero_network": res["zero_network"],
+        "network_iteration_gain": (res["zero_network"]["iterations"] / res["learned"]["iterations"]
+                                   if res["learned"]["converged"] and res["zero_network"]["converged"]
+                                   else None),
         # only meaningful when the learned solve reached the tolerance
         "speedup_vs_block_jacobi": (res["block_jacobi"]["ms"] / res["learned"]["ms"]
                                     if res["learned"]["converged"] else None),
         "iteration_ratio": (it_bj / max(1, res["learned"]["iterations"])
                             if res["learned"]["converged"] else None),
         "max_nodal_w_error_learned_vs_bj": err,
+        "max_nodal_w_error_zero_network_vs_bj": err0,
         "training": {"pairs": a.train_loads * 8, "epochs": a.train_epochs, "seconds": train_s,
                      "loss_first": tr["loss_first"], "loss_last": tr["loss_last"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

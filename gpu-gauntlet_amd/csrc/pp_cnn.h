@@ -36,7 +36,7 @@ struct MgLevel {
   int nx = 0, ny = 0;    // elements per side
   int64_t nn = 0;        // nodes (nx + 1) (ny + 1)
   double omega = 0.5;    // block-Jacobi smoothing weight 4 / (3 lambda_max(B A))
-  DevBuf<float> st;      // level >= 1: [node][9 neighbours][3][3], neighbour (dj + 1) * 3 + di + 1
+  DevBuf<float> st;      // level >= 1: [9 neighbours][3][3][node], neighbour (dj + 1) * 3 + di + 1
   DevBuf<double> minv;   // level >= 1: 6 per node (symmetric 3x3 block inverse)
   DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
   DevBuf<double> b, x, t;  // right-hand side, iterate, work (level 0: t only)
@@ -64,6 +64,7 @@ struct Cnn {
   pp_plan* pl = nullptr;
   hipStream_t stream = nullptr;
   int cu = 0, nx = 0, ny = 0;
+  size_t lds_max = 65536;  // dynamic LDS per workgroup of the device
   int H[3] = {0, 0, 0}, W[3] = {0, 0, 0};
   double sw = 1.0, st = 0.3;  // input scales of the w and rotation channels
   double omega = 0.5;         // block-Jacobi smoothing weight around the network

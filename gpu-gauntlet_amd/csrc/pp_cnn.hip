@@ -59,6 +59,22 @@ int nblk(int64_t n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------ convolution
 
+// four BF16 channels as one 8-byte access
+__device__ __forceinline__ void ld4(const uint16_t* p, float (&v)[4]) {
+  const uint2 u = *(const uint2*)p;
+  v[0] = __uint_as_float(u.x << 16);
+  v[1] = __uint_as_float(u.x & 0xffff0000u);
+  v[2] = __uint_as_float(u.y << 16);
+  v[3] = __uint_as_float(u.y & 0xffff0000u);
+}
+__device__ __forceinline__ void st4(uint16_t* p, const float (&v)[4]) {
+  uint2 u;
+  u.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+  u.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+  *(uint2*)p = u;
+}
+
+
 struct ConvArgs {
   const uint16_t* in;
   int in_cs, in_c0, cin;  // cin % 4 == 0
@@ -74,53 +90,97 @@ struct ConvArgs {
   int mask_cs, mask_c0, mask_from;
 };
 
-template <int NB>
+// One workgroup per output tile of MB rows x 64 pixels.  The input tile
+// with its halo, (MB + 2 dil) x (64 + 2 dil) pixels x cin channels, is
+// staged in LDS by independent 8-byte loads (pixel stride cin + 4 channels,
+// which spreads a wave's 16 pixels over the LDS banks), next to the weights
+// [NB * 16][kp].  Wave w owns the 16-pixel blocks w MB .. w MB + MB - 1 of
+// the tile (row-major, 4 per row).  The weights are the MFMA's A operand and
+// the pixels its B operand, so D[channel 4 kq + i][pixel m] leaves each lane
+// four consecutive output channels of one pixel: one 8-byte store.
+template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
-  extern __shared__ uint16_t w_s[];
+  extern __shared__ uint16_t smem[];
+  const int d = A.dil, TW = 64 + 2 * d, TH = MB + 2 * d, CS = A.cin + 4;
+  uint16_t* w_s = smem;
+  uint16_t* in_s = smem + NB * 16 * A.kp;
+  const int tiles_x = (A.W + 63) / 64;
+  const int tx0 = (blockIdx.x % tiles_x) * 64, ty0 = (blockIdx.x / tiles_x) * MB;
   const int nw = NB * 16 * A.kp;
   for (int i = threadIdx.x * 4; i < nw; i += 1024) *(uint2*)&w_s[i] = *(const uint2*)&A.w[i];
+  const int cq = A.cin >> 2, nq = TH * TW * cq;
+  for (int q = threadIdx.x; q < nq; q += 256) {
+    const int c = q % cq, pix = q / cq, col = pix % TW, row = pix / TW;
+    const int gx = tx0 - d + col, gy = ty0 - d + row;
+    uint2 v = {0u, 0u};
+    if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+      v = *(const uint2*)&A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + 4 * c];
+    *(uint2*)&in_s[(row * TW + col) * CS + 4 * c] = v;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int npix = A.H * A.W, nmb = (npix + 15) / 16;
   const int m = lane & 15, kq = lane >> 4;
   const int k9 = 9 * A.cin;
-  for (int mb = blockIdx.x * 4 + wave; mb < nmb; mb += gridDim.x * 4) {
-    const int pix = mb * 16 + m;
-    const bool pv = pix < npix;
-    const int px = pv ? pix % A.W : 0, py = pv ? pix / A.W : 0;
-    f4 acc[NB];
+  int pbase[MB];  // LDS pixel of each owned block's lane-m pixel at tap (0, 0)
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < A.kp; k0 += 16) {
-      const int kb = k0 + 4 * kq;
-      s4 a = {0, 0, 0, 0};
-      if (pv && kb < k9) {
-        const int tap = kb / A.cin, ci = kb - tap * A.cin;
-        const int qx = px + (tap % 3 - 1) * A.dil, qy = py + (tap / 3 - 1) * A.dil;
-        if (qx >= 0 && qx < A.W && qy >= 0 && qy < A.H)
-          a = *(const s4*)&A.in[((size_t)qy * A.W + qx) * A.in_cs + A.in_c0 + ci];
-      }
+  for (int j = 0; j < MB; j++) {
+    const int mb = wave * MB + j;
+    pbase[j] = ((mb >> 2) * TW + (mb & 3) * 16 + m) * CS;
+  }
+  f4 acc[MB][NB];
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++) {
-        const s4 b = *(const s4*)&w_s[(nb * 16 + m) * A.kp + kb];
-        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
-      }
+  for (int j = 0; j < MB; j++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[j][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < A.kp; k0 += 16) {
+    const int kb = k0 + 4 * kq;
+    s4 b[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * A.kp + kb];
+    const bool kv = kb < k9;
+    int toff = 0;
+    if (kv) {
+      const int tap = kb / A.cin, ci = kb - tap * A.cin;
+      toff = ((tap / 3) * d * TW + (tap % 3) * d) * CS + ci;
     }
-    // D[row = pixel 4 kq + i][col = channel m] of each N block
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+      s4 a = {0, 0, 0, 0};
+      if (kv) a = *(const s4*)&in_s[pbase[j] + toff];
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++)
+        acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a, acc[j][nb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MB; j++) {
+    const int mb = wave * MB + j;
+    const int oy = ty0 + (mb >> 2), ox = tx0 + (mb & 3) * 16 + m;
+    if (oy >= A.H || ox >= A.W) continue;
+    const size_t op = (size_t)oy * A.W + ox;
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
-      const int co = nb * 16 + m;
-#pragma unroll
+      const int co = nb * 16 + 4 * kq;
+      if (co >= A.cout) continue;
+      float v[4];
       for (int i = 0; i < 4; i++) {
-        const int op = mb * 16 + 4 * kq + i;
-        if (op >= npix || co >= A.cout) continue;
-        float v = acc[nb][i] + (A.bias ? A.bias[co] : 0.0f);
-        if (A.relu) v = fmaxf(v, 0.0f);
-        if (A.mask && co >= A.mask_from &&
-            !(bf2f(A.mask[(size_t)op * A.mask_cs + A.mask_c0 + co]) > 0.0f))
-          v = 0.0f;
-        if (A.out32) A.out32[(size_t)op * A.out32_cs + co] = v;
-        else A.out[(size_t)op * A.out_cs + A.out_c0 + co] = f2bf(v);
+        v[i] = acc[j][nb][i] + (A.bias && co + i < A.cout ? A.bias[co + i] : 0.0f);
+        if (A.relu) v[i] = fmaxf(v[i], 0.0f);
+      }
+      if (A.mask && co >= A.mask_from) {
+        float q[4];
+        ld4(&A.mask[op * A.mask_cs + A.mask_c0 + co], q);
+        for (int i = 0; i < 4; i++)
+          if (!(q[i] > 0.0f)) v[i] = 0.0f;
+      }
+      if (co + 4 <= A.cout) {
+        if (A.out32) *(float4*)&A.out32[op * A.out32_cs + co] = make_float4(v[0], v[1], v[2], v[3]);
+        else st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
+      } else {
+        for (int i = 0; i < 4 && co + i < A.cout; i++) {
+          if (A.out32) A.out32[op * A.out32_cs + co + i] = v[i];
+          else A.out[op * A.out_cs + A.out_c0 + co + i] = f2bf(v[i]);
+        }
       }
     }
   }
@@ -266,18 +326,37 @@ __global__ void k_pack_w(const float* W, int cout, int cin, int coutP, int kp, i
 
 // ------------------------------------------------------- grid transfer
 
-// out[I,J][c] = scale * sum w(a) w(b) in[2I+a, 2J+b][c] / 16 (w = 1 2 1),
-// then *= (mask > 0) and += previous out when accumulate.
+// Epilogue shared by the resamplers: scale, then += previous out when
+// accumulate, then *= (mask > 0); four channels at a time.
+__device__ __forceinline__ void resample_store(float (&s)[4], float scale, uint16_t* o,
+                                               const uint16_t* mask, int accumulate) {
+  float v[4] = {scale * s[0], scale * s[1], scale * s[2], scale * s[3]};
+  if (accumulate) {
+    float q[4];
+    ld4(o, q);
+    for (int k = 0; k < 4; k++) v[k] += q[k];
+  }
+  if (mask) {
+    float q[4];
+    ld4(mask, q);
+    for (int k = 0; k < 4; k++)
+      if (!(q[k] > 0.0f)) v[k] = 0.0f;
+  }
+  st4(o, v);
+}
+
+// Full-weighting restriction, four channels per thread (C % 4 == 0):
+// out[X,Y][c] = scale * sum w_a w_b in[2X + a, 2Y + b][c] / 16, w = (1, 2, 1).
 __global__ void k_restrict(const uint16_t* in, int in_cs, int in_c0, int Hf, int Wf,
                            uint16_t* out, int out_cs, int out_c0, int Hc, int Wc, int C,
                            float scale, const uint16_t* mask, int mask_cs, int mask_c0,
                            int accumulate) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)Hc * Wc * C) return;
-  const int c = (int)(i % C);
-  const int64_t pix = i / C;
-  const int X = (int)(pix % Wc), Y = (int)(pix / Wc);
-  float s = 0.0f;
+  const int C4 = C >> 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Hc * Wc * C4) return;
+  const int c = (i % C4) * 4, pix = i / C4;
+  const int X = pix % Wc, Y = pix / Wc;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int b = -1; b <= 1; b++) {
     const int fy = 2 * Y + b;
     if (fy < 0 || fy >= Hf) continue;
@@ -285,39 +364,41 @@ __global__ void k_restrict(const uint16_t* in, int in_cs, int in_c0, int Hf, int
     for (int a = -1; a <= 1; a++) {
       const int fx = 2 * X + a;
       if (fx < 0 || fx >= Wf) continue;
-      const float wx = a == 0 ? 2.0f : 1.0f;
-      s += wx * wy * bf2f(in[((size_t)fy * Wf + fx) * in_cs + in_c0 + c]);
+      const float w = wy * (a == 0 ? 2.0f : 1.0f);
+      float v[4];
+      ld4(&in[((size_t)fy * Wf + fx) * in_cs + in_c0 + c], v);
+      for (int k = 0; k < 4; k++) s[k] += w * v[k];
     }
   }
-  float v = scale * s * (1.0f / 16.0f);
-  uint16_t* o = &out[(size_t)pix * out_cs + out_c0 + c];
-  if (accumulate) v += bf2f(*o);
-  if (mask && !(bf2f(mask[(size_t)pix * mask_cs + mask_c0 + c]) > 0.0f)) v = 0.0f;
-  *o = f2bf(v);
+  resample_store(s, scale * (1.0f / 16.0f), &out[(size_t)pix * out_cs + out_c0 + c],
+                 mask ? &mask[(size_t)pix * mask_cs + mask_c0 + c] : nullptr, accumulate);
 }
 
-// out[x,y][c] = scale * bilinear(in) (vertex-centred, Hf = 2 Hc - 1).
+// out[x,y][c] = scale * bilinear(in) (vertex-centred, Hf = 2 Hc - 1), four
+// channels per thread.
 __global__ void k_prolong(const uint16_t* in, int in_cs, int in_c0, int Hc, int Wc, uint16_t* out,
                           int out_cs, int out_c0, int Hf, int Wf, int C, float scale,
                           const uint16_t* mask, int mask_cs, int mask_c0, int accumulate) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)Hf * Wf * C) return;
-  const int c = (int)(i % C);
-  const int64_t pix = i / C;
-  const int x = (int)(pix % Wf), y = (int)(pix / Wf);
+  const int C4 = C >> 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Hf * Wf * C4) return;
+  const int c = (i % C4) * 4, pix = i / C4;
+  const int x = pix % Wf, y = pix / Wf;
   const int x0 = x >> 1, y0 = y >> 1;
   const int x1 = (x & 1) ? x0 + 1 : x0, y1 = (y & 1) ? y0 + 1 : y0;
-  const float wx = (x & 1) ? 0.5f : 1.0f, wy = (y & 1) ? 0.5f : 1.0f;
-  auto at = [&](int X, int Y) { return bf2f(in[((size_t)Y * Wc + X) * in_cs + in_c0 + c]); };
-  float s = wx * wy * at(x0, y0);
-  if (x & 1) s += wx * wy * at(x1, y0);
-  if (y & 1) s += wx * wy * at(x0, y1);
-  if ((x & 1) && (y & 1)) s += wx * wy * at(x1, y1);
-  float v = scale * s;
-  uint16_t* o = &out[(size_t)pix * out_cs + out_c0 + c];
-  if (accumulate) v += bf2f(*o);
-  if (mask && !(bf2f(mask[(size_t)pix * mask_cs + mask_c0 + c]) > 0.0f)) v = 0.0f;
-  *o = f2bf(v);
+  const float w = ((x & 1) ? 0.5f : 1.0f) * ((y & 1) ? 0.5f : 1.0f);
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  auto add = [&](int X, int Y) {
+    float v[4];
+    ld4(&in[((size_t)Y * Wc + X) * in_cs + in_c0 + c], v);
+    for (int k = 0; k < 4; k++) s[k] += w * v[k];
+  };
+  add(x0, y0);
+  if (x & 1) add(x1, y0);
+  if (y & 1) add(x0, y1);
+  if ((x & 1) && (y & 1)) add(x1, y1);
+  resample_store(s, scale, &out[(size_t)pix * out_cs + out_c0 + c],
+                 mask ? &mask[(size_t)pix * mask_cs + mask_c0 + c] : nullptr, accumulate);
 }
 
 // ------------------------------------------------------ input / output
@@ -634,18 +715,33 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
   if (mask) {
     A.mask = mask->p; A.mask_cs = mask->cs; A.mask_c0 = mask->c0; A.mask_from = mask_from;
   }
-  const int npix = H[lev] * W[lev], nmb = (npix + 15) / 16;
-  const int grid = std::max(1, std::min((nmb + 15) / 16, 4 * cu));
-  const size_t lds = (size_t)nb * 16 * A.kp * 2;
+  // rows per tile: 4 when weights and tile fit two workgroups per CU, else 2,
+  // else the largest that fits one
+  auto lds_of = [&](int mb) {
+    return (size_t)nb * 16 * A.kp * 2 + (size_t)(mb + 2 * A.dil) * (64 + 2 * A.dil) * (cin + 4) * 2;
+  };
+  int MB = 0;
+  for (size_t cap : {lds_max / 2, lds_max})
+    for (int mb : {4, 2})
+      if (!MB && lds_of(mb) <= cap) MB = mb;
+  if (!MB || cin % 4) return fail(PP_ERR_INVALID, "conv: layer too wide for LDS");
+  const size_t lds = lds_of(MB);
+  const int grid = ((W[lev] + 63) / 64) * ((H[lev] + MB - 1) / MB);
   hipStream_t s = stream;
+#define PP_CONV_CASE(N)                                                                     \
+  case N:                                                                                  \
+    if (MB == 4) hipLaunchKernelGGL((k_conv<N, 4>), dim3(grid), dim3(256), lds, s, A);    \
+    else hipLaunchKernelGGL((k_conv<N, 2>), dim3(grid), dim3(256), lds, s, A);            \
+    break;
   switch (nb) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(grid), dim3(256), lds, s, A); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(grid), dim3(256), lds, s, A); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(grid), dim3(256), lds, s, A); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(grid), dim3(256), lds, s, A); break;
-    case 6: hipLaunchKernelGGL(k_conv<6>, dim3(grid), dim3(256), lds, s, A); break;
+    PP_CONV_CASE(1)
+    PP_CONV_CASE(2)
+    PP_CONV_CASE(3)
+    PP_CONV_CASE(4)
+    PP_CONV_CASE(6)
     default: return fail(PP_ERR_INVALID, "conv: unsupported channel count");
   }
+#undef PP_CONV_CASE
   PP_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -673,7 +769,11 @@ int Cnn::conv_wgrad(const Layer& L, const TV& in, const TV& dout, int lev) {
 
 int Cnn::resample(bool restrict_op, const TV& in, int lin, const TV& out, int lout, int C,
                   float scale, const TV* mask, bool accumulate) {
-  const int64_t n = (int64_t)H[lout] * W[lout] * C;
+  if (C % 4 || in.cs % 4 || in.c0 % 4 || out.cs % 4 || out.c0 % 4 ||
+      (mask && (mask->cs % 4 || mask->c0 % 4)))
+    return fail(PP_ERR_INVALID, "resample: channel counts, strides and offsets must be multiples of 4");
+  const int64_t n = (int64_t)H[lout] * W[lout] * (C / 4);
+  if (n >= (1LL << 31)) return fail(PP_ERR_INVALID, "resample: grid too large");
   if (restrict_op)
     hipLaunchKernelGGL(k_restrict, dim3(nblk(n)), dim3(256), 0, stream, (const uint16_t*)in.p,
                        in.cs, in.c0, H[lin], W[lin], out.p, out.cs, out.c0, H[lout], W[lout], C,
@@ -752,12 +852,13 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   if ((rc = params.alloc(np)) || (rc = grads.alloc(np)) || (rc = adam_m.alloc(np)) ||
       (rc = adam_v.alloc(np)) || (rc = wf.alloc(nwf)) || (rc = wt.alloc(nwt)))
     return rc;
-  // He initialisation (head x 0.1), zero biases
+  // He initialisation, zero biases, and a zero head: the untrained network
+  // adds no correction, so training starts from the preconditioner without it
   std::vector<float> h(np, 0.0f);
   uint64_t s = seed;
-  for (size_t li = 0; li < layers.size(); li++) {
+  for (size_t li = 0; li + 1 < layers.size(); li++) {
     const Layer& L = layers[li];
-    const double sd = sqrt(2.0 / (9.0 * L.cin)) * (li + 1 == layers.size() ? 0.1 : 1.0);
+    const double sd = sqrt(2.0 / (9.0 * L.cin));
     for (int64_t k = 0; k < (int64_t)L.cout * 9 * L.cin; k++) h[L.w_off + k] = (float)(sd * gauss(s));
   }
   PP_HIP(hipMemcpyAsync(params.p, h.data(), np * 4, hipMemcpyHostToDevice, stream));
@@ -801,11 +902,18 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   hipLaunchKernelGGL(k_node_logt, dim3(nblk(pl->n_nodes)), dim3(256), 0, stream, G, logt.p);
   PP_HIP(hipGetLastError());
   // dynamic LDS above 64 KB for the widest layers
-  size_t mx = 0;
-  for (auto& L : layers) mx = std::max({mx, (size_t)L.coutP * L.kp * 2, (size_t)L.cinP * L.kpT * 2});
-  const void* ks[] = {(const void*)k_conv<1>, (const void*)k_conv<2>, (const void*)k_conv<3>,
-                      (const void*)k_conv<4>, (const void*)k_conv<6>};
-  for (auto k : ks) PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx));
+  {
+    int v = 0, o = 0;
+    PP_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, plan->ctx->device));
+    PP_HIP(hipDeviceGetAttribute(&o, hipDeviceAttributeSharedMemPerBlockOptin, plan->ctx->device));
+    lds_max = (size_t)std::max(v, o);
+  }
+  const void* ks[] = {(const void*)k_conv<1, 4>, (const void*)k_conv<2, 4>, (const void*)k_conv<3, 4>,
+                      (const void*)k_conv<4, 4>, (const void*)k_conv<6, 4>, (const void*)k_conv<1, 2>,
+                      (const void*)k_conv<2, 2>, (const void*)k_conv<3, 2>, (const void*)k_conv<4, 2>,
+                      (const void*)k_conv<6, 2>};
+  for (auto k : ks)
+    PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
   size_t mw = 0;
   for (auto& L : layers)
     mw = std::max(mw, ((size_t)L.cin * 3 * (64 + 2 * L.dil) + (size_t)L.coutP * 64) * 2);

@@ -17,7 +17,8 @@
 //                A DOF whose assembled diagonal is exactly zero (its basis
 //                function vanishes) is excluded on that level: identity row,
 //                masked in P.
-// Levels >= 1 keep A as an FP32 9-point block stencil (324 B per node) and
+// Levels >= 1 keep A as an FP32 9-point block stencil (324 B per node, stored
+// as 81 planes of one float per node so that a wave's loads coalesce) and
 // an FP64 block-Jacobi inverse; the coarsest level (at most kDenseMax DOFs)
 // is solved by its dense FP64 inverse, formed on the host by Cholesky.
 //
@@ -142,7 +143,8 @@ __global__ void k_mg_stencil(int nx, int ny, const double* ke, float* st, uint8_
       S[4 * 9 + 4 * d] = 1.0;
     }
   dead[n] = (uint8_t)bits;
-  for (int k = 0; k < 81; k++) st[n * 81 + k] = (float)S[k];
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
+  for (int k = 0; k < 81; k++) st[k * nn + n] = (float)S[k];
   double A[9];
   for (int k = 0; k < 9; k++) A[k] = S[4 * 9 + k];
   block_inverse_sym(A, bits, minv + 6 * n);
@@ -153,7 +155,8 @@ __global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, dou
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
   const int i = (int)(n % (nx + 1)), j = (int)(n / (nx + 1));
-  const float* S = st + n * 81;
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
+  const float* S = st + n;  // entry k at S[k * nn]
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
   for (int dj = -1; dj <= 1; dj++) {
     const int jj = j + dj;
@@ -164,10 +167,10 @@ __global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, dou
       const int nb = (dj + 1) * 3 + di + 1;
       const int64_t m = (int64_t)jj * (nx + 1) + ii;
       const double x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
-      const float* s = S + 9 * nb;
-      y0 += (double)s[0] * x0 + (double)s[1] * x1 + (double)s[2] * x2;
-      y1 += (double)s[3] * x0 + (double)s[4] * x1 + (double)s[5] * x2;
-      y2 += (double)s[6] * x0 + (double)s[7] * x1 + (double)s[8] * x2;
+      const float* s = S + 9 * nb * nn;
+      y0 += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
+      y1 += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
+      y2 += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
     }
   }
   y[3 * n] = y0;
