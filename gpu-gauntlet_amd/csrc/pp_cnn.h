@@ -39,7 +39,7 @@ struct MgLevel {
   DevBuf<float> st;      // level >= 1: [9 neighbours][3][3][node], neighbour (dj + 1) * 3 + di + 1
   DevBuf<double> minv;   // level >= 1: 6 per node (symmetric 3x3 block inverse)
   DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
-  DevBuf<double> b, x, t;  // right-hand side, iterate, work (level 0: t only)
+  DevBuf<double> b, x, x2, t;  // right-hand side, iterate and its sweep partner, work
 };
 
 struct Mg {
@@ -57,6 +57,7 @@ struct Mg {
   int apply_level(int l, const double* x, double* y);  // y = A_l x
  private:
   int cycle(int l);
+  int sweep(int l, int mode, const double* x, double* xo);
   int estimate_omega(int l);
 };
 

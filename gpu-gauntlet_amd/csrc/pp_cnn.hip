@@ -98,6 +98,8 @@ struct ConvArgs {
 // the tile (row-major, 4 per row).  The weights are the MFMA's A operand and
 // the pixels its B operand, so D[channel 4 kq + i][pixel m] leaves each lane
 // four consecutive output channels of one pixel: one 8-byte store.
+constexpr int kStage = 8;  // staging loads in flight per thread
+
 template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   extern __shared__ uint16_t smem[];
@@ -106,16 +108,43 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   uint16_t* in_s = smem + NB * 16 * A.kp;
   const int tiles_x = (A.W + 63) / 64;
   const int tx0 = (blockIdx.x % tiles_x) * 64, ty0 = (blockIdx.x / tiles_x) * MB;
-  const int nw = NB * 16 * A.kp;
-  for (int i = threadIdx.x * 4; i < nw; i += 1024) *(uint2*)&w_s[i] = *(const uint2*)&A.w[i];
+  // staging: kStage independent 8-byte loads in flight per thread, then
+  // their LDS stores
+  const int nw4 = NB * 4 * A.kp;  // weight quads
+  for (int i0 = threadIdx.x; i0 < nw4; i0 += 256 * kStage) {
+    uint2 v[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      if (i0 + 256 * u < nw4) v[u] = *(const uint2*)&A.w[4 * (i0 + 256 * u)];
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      if (i0 + 256 * u < nw4) *(uint2*)&w_s[4 * (i0 + 256 * u)] = v[u];
+  }
   const int cq = A.cin >> 2, nq = TH * TW * cq;
-  for (int q = threadIdx.x; q < nq; q += 256) {
-    const int c = q % cq, pix = q / cq, col = pix % TW, row = pix / TW;
-    const int gx = tx0 - d + col, gy = ty0 - d + row;
-    uint2 v = {0u, 0u};
-    if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
-      v = *(const uint2*)&A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + 4 * c];
-    *(uint2*)&in_s[(row * TW + col) * CS + 4 * c] = v;
+  // q / cq and pix / TW by float reciprocals, exact here: (q + 1/2) / cq is at
+  // least 1 / (2 cq) >= 1 / 48 from an integer (cq <= 24), and for q < 2^16
+  // its float error is below 1e-2 (the launch checks both bounds)
+  const float rcq = 1.0f / (float)cq, rtw = 1.0f / (float)TW;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += 256 * kStage) {
+    uint2 v[kStage];
+    int dst[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int q = q0 + 256 * u;
+      v[u] = uint2{0u, 0u};
+      dst[u] = -1;
+      if (q < nq) {
+        const int pix = (int)(((float)q + 0.5f) * rcq), c = q - pix * cq;
+        const int row = (int)(((float)pix + 0.5f) * rtw), col = pix - row * TW;
+        const int gx = tx0 - d + col, gy = ty0 - d + row;
+        dst[u] = (row * TW + col) * CS + 4 * c;
+        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+          v[u] = *(const uint2*)&A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + 4 * c];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      if (dst[u] >= 0) *(uint2*)&in_s[dst[u]] = v[u];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -132,25 +161,37 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   for (int j = 0; j < MB; j++)
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[j][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  // (tap, channel) of this lane's k = k0 + 4 kq, advanced by 16 per step
+  // without branches (at most 16 / cin <= 4 wraps).  Steps past 9 cin have
+  // zero weights, so their A reads may use any pixel of the tile: they take
+  // the lane's own (toff 0) and every read is unconditional.
+  int tap = 0, ci = 4 * kq;
+  auto wrap = [&]() {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int w = ci >= A.cin;
+      ci -= w * A.cin;
+      tap += w;
+    }
+  };
+  wrap();
   for (int k0 = 0; k0 < A.kp; k0 += 16) {
     const int kb = k0 + 4 * kq;
     s4 b[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * A.kp + kb];
-    const bool kv = kb < k9;
-    int toff = 0;
-    if (kv) {
-      const int tap = kb / A.cin, ci = kb - tap * A.cin;
-      toff = ((tap / 3) * d * TW + (tap % 3) * d) * CS + ci;
-    }
+    const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
+    const int toff = kb < k9 ? (ty * d * TW + (tap - 3 * ty) * d) * CS + ci : 0;
+    s4 a[MB];
 #pragma unroll
-    for (int j = 0; j < MB; j++) {
-      s4 a = {0, 0, 0, 0};
-      if (kv) a = *(const s4*)&in_s[pbase[j] + toff];
+    for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff];
+#pragma unroll
+    for (int j = 0; j < MB; j++)
 #pragma unroll
       for (int nb = 0; nb < NB; nb++)
-        acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a, acc[j][nb], 0, 0, 0);
-    }
+        acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+    ci += 16;
+    wrap();
   }
 #pragma unroll
   for (int j = 0; j < MB; j++) {
@@ -725,6 +766,8 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
     for (int mb : {4, 2})
       if (!MB && lds_of(mb) <= cap) MB = mb;
   if (!MB || cin % 4) return fail(PP_ERR_INVALID, "conv: layer too wide for LDS");
+  if (cin > 96 || (MB + 2 * A.dil) * (64 + 2 * A.dil) * (cin / 4) >= 65536)
+    return fail(PP_ERR_INVALID, "conv: input tile beyond the staging index range");
   const size_t lds = lds_of(MB);
   const int grid = ((W[lev] + 63) / 64) * ((H[lev] + MB - 1) / MB);
   hipStream_t s = stream;
@@ -804,7 +847,9 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   st = cfg && cfg->rotation_scale > 0 ? cfg->rotation_scale : 0.3;
   linear = cfg && cfg->relu ? 0 : 1;
   line_search = cfg && cfg->fixed_step ? 0 : 1;
-  fcg_window = cfg && cfg->fcg_window > 0 ? cfg->fcg_window : 8;
+  // FCG(8) by default; FCG(4) with the multigrid smoother, whose stronger
+  // steps need fewer kept directions (1024^2: 217 iterations against 242)
+  fcg_window = cfg && cfg->fcg_window > 0 ? cfg->fcg_window : cfg && cfg->multigrid ? 4 : 8;
   if (fcg_window > 16) return fail(PP_ERR_INVALID, "fcg_window must be at most 16");
   // padded level-0 grid: 4k + 1 nodes per side
   W[0] = ((nx + 3) / 4) * 4 + 1;

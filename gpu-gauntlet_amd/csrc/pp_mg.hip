@@ -150,32 +150,59 @@ __global__ void k_mg_stencil(int nx, int ny, const double* ke, float* st, uint8_
   block_inverse_sym(A, bits, minv + 6 * n);
 }
 
-// y = A x on a stencil level
-__global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, double* y) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
+// (A x)_n on a stencil level.  A neighbour outside the grid has exactly zero
+// stencil entries (no element contributes to them), so its index is clamped
+// into the grid instead of branching: every load is unconditional and a
+// thread's 108 loads are all in flight together.
+__device__ __forceinline__ void stencil_row(int nx, int ny, const float* st, const double* x,
+                                            int64_t n, double (&y)[3]) {
   const int i = (int)(n % (nx + 1)), j = (int)(n / (nx + 1));
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
   const float* S = st + n;  // entry k at S[k * nn]
-  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  y[0] = y[1] = y[2] = 0.0;
+#pragma unroll
   for (int dj = -1; dj <= 1; dj++) {
-    const int jj = j + dj;
-    if (jj < 0 || jj > ny) continue;
+    const int jj = min(max(j + dj, 0), ny);
+#pragma unroll
     for (int di = -1; di <= 1; di++) {
-      const int ii = i + di;
-      if (ii < 0 || ii > nx) continue;
+      const int ii = min(max(i + di, 0), nx);
       const int nb = (dj + 1) * 3 + di + 1;
       const int64_t m = (int64_t)jj * (nx + 1) + ii;
       const double x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
       const float* s = S + 9 * nb * nn;
-      y0 += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
-      y1 += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
-      y2 += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
+      y[0] += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
+      y[1] += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
+      y[2] += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
     }
   }
-  y[3 * n] = y0;
-  y[3 * n + 1] = y1;
-  y[3 * n + 2] = y2;
+}
+
+// y = A x on a stencil level
+__global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, double* y) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
+  double v[3];
+  stencil_row(nx, ny, st, x, n, v);
+  for (int d = 0; d < 3; d++) y[3 * n + d] = v[d];
+}
+
+// Fused sweep on a stencil level: MODE 1: xo = x + w B (b - A x) (xo must
+// not alias x); MODE 2: xo = b - A x.
+template <int MODE>
+__global__ void k_mg_sweep(int nx, int ny, const float* st, const double* minv, double omega,
+                           const double* x, const double* b, double* xo) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
+  double v[3], r[3];
+  stencil_row(nx, ny, st, x, n, v);
+  for (int d = 0; d < 3; d++) r[d] = b[3 * n + d] - v[d];
+  if (MODE == 2) {
+    for (int d = 0; d < 3; d++) xo[3 * n + d] = r[d];
+    return;
+  }
+  double z[3];
+  sym3_apply(minv + 6 * n, r, z);
+  for (int d = 0; d < 3; d++) xo[3 * n + d] = x[3 * n + d] + omega * z[d];
 }
 
 // mode 0: x = w B b;  mode 1: x += w B (b - t);  mode 2: t = b - t
@@ -351,6 +378,7 @@ int Mg::setup(pp_plan* plan, double omega0, int sweeps) {
     L.ny = dims[l].second;
     L.nn = (int64_t)(L.nx + 1) * (L.ny + 1);
     if ((rc = L.t.alloc(3 * L.nn))) return rc;
+    if (l < nl - 1 && (rc = L.x2.alloc(3 * L.nn))) return rc;
     if (l > 0 && ((rc = L.b.alloc(3 * L.nn)) || (rc = L.x.alloc(3 * L.nn)))) return rc;
     if (l > 0 && l < nl - 1 &&
         ((rc = L.st.alloc(81 * L.nn)) || (rc = L.minv.alloc(6 * L.nn)) || (rc = L.dead.alloc(L.nn))))
@@ -462,6 +490,23 @@ int Mg::estimate_omega(int l) {
   return PP_OK;
 }
 
+int Mg::sweep(int l, int mode, const double* x, double* xo) {
+  MgLevel& L = lv[l];
+  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, L.omega, xo);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_mg_sweep<1>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
+                       (const float*)L.st.p, (const double*)L.minv.p, L.omega, x,
+                       (const double*)L.b.p, xo);
+  else
+    hipLaunchKernelGGL(k_mg_sweep<2>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
+                       (const float*)L.st.p, (const double*)L.minv.p, L.omega, x,
+                       (const double*)L.b.p, xo);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+// The sweeps ping-pong between L.x and L.x2: 2 nu - 1 of them follow the
+// first (x = w B b), an odd count, so starting in x2 the cycle ends in L.x.
 int Mg::cycle(int l) {
   MgLevel& L = lv[l];
   if (l == nl - 1) {
@@ -474,27 +519,25 @@ int Mg::cycle(int l) {
   const uint8_t* mask = l == 0 ? (const uint8_t*)pl->cmask.p : (const uint8_t*)L.dead.p;
   const int g = mg_blocks(L.nn);
   int rc;
+  double* cur = L.x2.p;
+  auto other = [&](double* p) { return p == L.x.p ? L.x2.p : L.x.p; };
   hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 0, B, L.omega,
-                     (const double*)L.b.p, (double*)nullptr, L.x.p);
+                     (const double*)L.b.p, (double*)nullptr, cur);
   for (int s = 1; s < nu; s++) {
-    if ((rc = apply_level(l, L.x.p, L.t.p))) return rc;
-    hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 1, B, L.omega,
-                       (const double*)L.b.p, L.t.p, L.x.p);
+    if ((rc = sweep(l, 1, cur, other(cur)))) return rc;
+    cur = other(cur);
   }
-  if ((rc = apply_level(l, L.x.p, L.t.p))) return rc;
-  hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 2, B, L.omega,
-                     (const double*)L.b.p, L.t.p, (double*)nullptr);
+  if ((rc = sweep(l, 2, cur, L.t.p))) return rc;  // residual
   MgLevel& C = lv[l + 1];
   hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nn)), dim3(256), 0, stream, L.nx, L.ny, mask,
                      (const double*)L.t.p, C.nx, C.ny, C.b.p);
   PP_HIP(hipGetLastError());
   if ((rc = cycle(l + 1))) return rc;
   hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, stream, L.nx, L.ny, mask,
-                     (const double*)C.x.p, C.nx, L.x.p);
+                     (const double*)C.x.p, C.nx, cur);
   for (int s = 0; s < nu; s++) {
-    if ((rc = apply_level(l, L.x.p, L.t.p))) return rc;
-    hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 1, B, L.omega,
-                       (const double*)L.b.p, L.t.p, L.x.p);
+    if ((rc = sweep(l, 1, cur, other(cur)))) return rc;
+    cur = other(cur);
   }
   PP_HIP(hipGetLastError());
   return PP_OK;

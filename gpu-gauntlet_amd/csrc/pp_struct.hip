@@ -74,6 +74,44 @@ __global__ __launch_bounds__(CfgS::NT) void k_apply(KGrid G, const double* x, do
   }
 }
 
+// Fused multigrid sweep on the plan's operator: with (K x)_n as in k_apply,
+// MODE 1: xo = x + w B (b - K x) (one damped block-Jacobi sweep; xo must not
+// alias x, whose halo other workgroups read); MODE 2: xo = b - K x.
+template <int MODE>
+__global__ __launch_bounds__(CfgS::NT) void k_apply_sweep(KGrid G, const double* x,
+                                                          const double* b, const double* minv,
+                                                          double omega, double* xo) {
+  using C = CfgS;
+  __shared__ double p_s[3 * C::NP];
+  __shared__ double ye_s[12 * C::NE];
+  const int tix = blockIdx.x % tiles_x<C>(G), tiy = blockIdx.x / tiles_x<C>(G);
+  const int i0 = tix * C::TX, j0 = tiy * C::TY;
+  stage_tile<C, double, double>(G, i0, j0, x, nullptr, 0.0, p_s);
+  __syncthreads();
+  tile_elements<C, double>(G, i0, j0, p_s, ye_s);
+  __syncthreads();
+  if (threadIdx.x < C::TX * C::TY) {
+    const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
+    const int gi = i0 + tx, gj = j0 + ty;
+    if (gi <= G.nx && gj <= G.ny) {
+      double v[3];
+      node_sum<C, double>(tx, ty, ye_s, v);
+      const size_t n = (size_t)gj * (G.nx + 1) + gi;
+      const unsigned fix = G.cmask[n];
+      const int c = (ty + 1) * C::PW + tx + 1;
+      double r[3];
+      for (int d = 0; d < 3; d++) r[d] = b[3 * n + d] - ((fix >> d) & 1u ? p_s[d * C::NP + c] : v[d]);
+      if (MODE == 2) {
+        for (int d = 0; d < 3; d++) xo[3 * n + d] = r[d];
+      } else {
+        double z[3];
+        sym3_apply(minv + 6 * n, r, z);
+        for (int d = 0; d < 3; d++) xo[3 * n + d] = p_s[d * C::NP + c] + omega * z[d];
+      }
+    }
+  }
+}
+
 // ------------------------------------------- block-Jacobi set-up/apply
 
 // Diagonal 3x3 block of K at every node: each adjacent element contributes
@@ -910,6 +948,20 @@ int struct_apply_k(pp_plan* pl, const double* dx, double* dy) {
   const int ntiles =
       ((pl->nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((pl->ny + 1 + CfgS::TY - 1) / CfgS::TY);
   hipLaunchKernelGGL(k_apply, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, dx, dy);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int struct_sweep(pp_plan* pl, int mode, const double* x, const double* b, double omega, double* xo) {
+  KGrid G = make_kgrid(pl);
+  const int ntiles =
+      ((pl->nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((pl->ny + 1 + CfgS::TY - 1) / CfgS::TY);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_apply_sweep<1>, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, x, b,
+                       (const double*)pl->minv.p, omega, xo);
+  else
+    hipLaunchKernelGGL(k_apply_sweep<2>, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, x, b,
+                       (const double*)pl->minv.p, omega, xo);
   PP_HIP(hipGetLastError());
   return PP_OK;
 }

@@ -232,7 +232,7 @@ typedef struct {
                                   a fixed linear map, as in training, but a trained network
                                   need not then be a convergent preconditioner) */
   int32_t fcg_window;          /* directions flexible PCG K-orthogonalises against (FCG(m));
-                                  0 = 8, at most 16.  With the energy-optimal step, FCG(1)
+                                  0 = 8 (4 with multigrid = 1), at most 16.  With the energy-optimal step, FCG(1)
                                   stalls on large plates (1024^2, zero network: 5e-3 after
                                   30,000 iterations) where FCG(8) converges (29,897) */
   int32_t multigrid;           /* 0: the smoother S around the network is w B (block Jacobi);
