@@ -248,7 +248,8 @@ def bench_learned(a):
                    "field with correlation length 0.5 m (seed 5), clamped, 10 kPa + 3 Gaussian "
                    "patches", "dof": n_dof, "precond": "learned U-net (16/32/64, BF16 MFMA) + " +
                    ("Galerkin multigrid V-cycle smoothing (7 levels, FP32 coarse stencils)" if mg
-                    else "block-Jacobi smoothing") + ", flexible PCG (FCG(8))",
+                    else "block-Jacobi smoothing") + (", flexible PCG (FCG(4))" if mg
+                                                      else ", flexible PCG (FCG(8))"),
                    "precision": "fp64 outer, bf16 network"},
         "learned": res["learned"], "block_jacobi": res["block_jacobi"],
         "zero_network": res["zero_network"],
