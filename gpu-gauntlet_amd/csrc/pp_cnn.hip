@@ -84,6 +84,7 @@ struct ConvArgs {
   int out32_cs;
   const uint16_t* w;  // [NB*16][kp] bf16, k = tap * cin + ci
   int kp;             // multiple of 16, >= 9 cin
+  int cc;             // input channels per LDS chunk (divides cin, multiple of 4)
   const float* bias;  // [cout] or null
   int H, W, dil, relu;
   const uint16_t* mask;  // out[c] *= (mask[pixel][c] > 0) for c >= mask_from
@@ -103,53 +104,17 @@ constexpr int kStage = 8;  // staging loads in flight per thread
 template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   extern __shared__ uint16_t smem[];
-  const int d = A.dil, TW = 64 + 2 * d, TH = MB + 2 * d, CS = A.cin + 4;
+  // input channels in chunks of CC: per chunk the tile holds CC channels and
+  // the weights their K slice [NB * 16][KC], KC = 9 CC rounded up to 16
+  const int CC = A.cc, KC = (9 * CC + 15) / 16 * 16;
+  const int d = A.dil, TW = 64 + 2 * d, TH = MB + 2 * d, CS = CC + 4;
   uint16_t* w_s = smem;
-  uint16_t* in_s = smem + NB * 16 * A.kp;
+  uint16_t* in_s = smem + NB * 16 * KC;
   const int tiles_x = (A.W + 63) / 64;
   const int tx0 = (blockIdx.x % tiles_x) * 64, ty0 = (blockIdx.x / tiles_x) * MB;
-  // staging: kStage independent 8-byte loads in flight per thread, then
-  // their LDS stores
-  const int nw4 = NB * 4 * A.kp;  // weight quads
-  for (int i0 = threadIdx.x; i0 < nw4; i0 += 256 * kStage) {
-    uint2 v[kStage];
-#pragma unroll
-    for (int u = 0; u < kStage; u++)
-      if (i0 + 256 * u < nw4) v[u] = *(const uint2*)&A.w[4 * (i0 + 256 * u)];
-#pragma unroll
-    for (int u = 0; u < kStage; u++)
-      if (i0 + 256 * u < nw4) *(uint2*)&w_s[4 * (i0 + 256 * u)] = v[u];
-  }
-  const int cq = A.cin >> 2, nq = TH * TW * cq;
-  // q / cq and pix / TW by float reciprocals, exact here: (q + 1/2) / cq is at
-  // least 1 / (2 cq) >= 1 / 48 from an integer (cq <= 24), and for q < 2^16
-  // its float error is below 1e-2 (the launch checks both bounds)
-  const float rcq = 1.0f / (float)cq, rtw = 1.0f / (float)TW;
-  for (int q0 = threadIdx.x; q0 < nq; q0 += 256 * kStage) {
-    uint2 v[kStage];
-    int dst[kStage];
-#pragma unroll
-    for (int u = 0; u < kStage; u++) {
-      const int q = q0 + 256 * u;
-      v[u] = uint2{0u, 0u};
-      dst[u] = -1;
-      if (q < nq) {
-        const int pix = (int)(((float)q + 0.5f) * rcq), c = q - pix * cq;
-        const int row = (int)(((float)pix + 0.5f) * rtw), col = pix - row * TW;
-        const int gx = tx0 - d + col, gy = ty0 - d + row;
-        dst[u] = (row * TW + col) * CS + 4 * c;
-        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
-          v[u] = *(const uint2*)&A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + 4 * c];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kStage; u++)
-      if (dst[u] >= 0) *(uint2*)&in_s[dst[u]] = v[u];
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  const int k9 = 9 * A.cin;
+  const int k9 = 9 * CC;
   int pbase[MB];  // LDS pixel of each owned block's lane-m pixel at tap (0, 0)
 #pragma unroll
   for (int j = 0; j < MB; j++) {
@@ -161,37 +126,89 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   for (int j = 0; j < MB; j++)
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[j][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  // (tap, channel) of this lane's k = k0 + 4 kq, advanced by 16 per step
-  // without branches (at most 16 / cin <= 4 wraps).  Steps past 9 cin have
-  // zero weights, so their A reads may use any pixel of the tile: they take
-  // the lane's own (toff 0) and every read is unconditional.
-  int tap = 0, ci = 4 * kq;
-  auto wrap = [&]() {
+  // Index splits by float reciprocals, exact here: (q + 1/2) / D is at least
+  // 1 / (2 D) >= 1 / 144 from an integer for every divisor D <= 72 used
+  // below, and for q < 2^16 its float error is below 1e-2 / 2 (the launch
+  // checks both bounds)
+  const int cq = CC >> 2, nq = TH * TW * cq, kq4 = KC >> 2, nw4 = NB * 16 * kq4;
+  const float rcq = 1.0f / (float)cq, rtw = 1.0f / (float)TW, rkq = 1.0f / (float)kq4;
+  for (int c0 = 0; c0 < A.cin; c0 += CC) {
+    if (c0) __syncthreads();  // the previous chunk's reads are done
+    // staging: kStage independent 8-byte loads in flight per thread, then
+    // their LDS stores.  Weights: quad (co, k = 4 i) of the slice is
+    // w[co][tap * cin + c0 + ci] for k = tap * CC + ci < 9 CC, else 0.
+    for (int i0 = threadIdx.x; i0 < nw4; i0 += 256 * kStage) {
+      uint2 v[kStage];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int w = ci >= A.cin;
-      ci -= w * A.cin;
-      tap += w;
+      for (int u = 0; u < kStage; u++) {
+        const int i = i0 + 256 * u;
+        v[u] = uint2{0u, 0u};
+        if (i < nw4) {
+          const int co = (int)(((float)i + 0.5f) * rkq), k = 4 * (i - co * kq4);
+          if (k < k9) {
+            const int tap = (int)(((float)k + 0.5f) / (float)CC), ci = k - tap * CC;
+            v[u] = *(const uint2*)&A.w[(size_t)co * A.kp + tap * A.cin + c0 + ci];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; u++)
+        if (i0 + 256 * u < nw4) *(uint2*)&w_s[4 * (i0 + 256 * u)] = v[u];
     }
-  };
-  wrap();
-  for (int k0 = 0; k0 < A.kp; k0 += 16) {
-    const int kb = k0 + 4 * kq;
-    s4 b[NB];
+    for (int q0 = threadIdx.x; q0 < nq; q0 += 256 * kStage) {
+      uint2 v[kStage];
+      int dst[kStage];
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * A.kp + kb];
-    const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
-    const int toff = kb < k9 ? (ty * d * TW + (tap - 3 * ty) * d) * CS + ci : 0;
-    s4 a[MB];
+      for (int u = 0; u < kStage; u++) {
+        const int q = q0 + 256 * u;
+        v[u] = uint2{0u, 0u};
+        dst[u] = -1;
+        if (q < nq) {
+          const int pix = (int)(((float)q + 0.5f) * rcq), c = q - pix * cq;
+          const int row = (int)(((float)pix + 0.5f) * rtw), col = pix - row * TW;
+          const int gx = tx0 - d + col, gy = ty0 - d + row;
+          dst[u] = (row * TW + col) * CS + 4 * c;
+          if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+            v[u] = *(const uint2*)&A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + c0 + 4 * c];
+        }
+      }
 #pragma unroll
-    for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff];
+      for (int u = 0; u < kStage; u++)
+        if (dst[u] >= 0) *(uint2*)&in_s[dst[u]] = v[u];
+    }
+    __syncthreads();
+    // (tap, channel) of this lane's k = k0 + 4 kq, advanced by 16 per step
+    // without branches (at most 16 / CC <= 4 wraps).  Steps past 9 CC have
+    // zero weights, so their A reads may use any pixel of the tile: they take
+    // the lane's own (toff 0) and every read is unconditional.
+    int tap = 0, ci = 4 * kq;
+    auto wrap = [&]() {
 #pragma unroll
-    for (int j = 0; j < MB; j++)
-#pragma unroll
-      for (int nb = 0; nb < NB; nb++)
-        acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
-    ci += 16;
+      for (int r = 0; r < 4; r++) {
+        const int w = ci >= CC;
+        ci -= w * CC;
+        tap += w;
+      }
+    };
     wrap();
+    for (int k0 = 0; k0 < KC; k0 += 16) {
+      const int kb = k0 + 4 * kq;
+      s4 b[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KC + kb];
+      const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
+      const int toff = kb < k9 ? (ty * d * TW + (tap - 3 * ty) * d) * CS + ci : 0;
+      s4 a[MB];
+#pragma unroll
+      for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff];
+#pragma unroll
+      for (int j = 0; j < MB; j++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+          acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+      ci += 16;
+      wrap();
+    }
   }
 #pragma unroll
   for (int j = 0; j < MB; j++) {
@@ -756,17 +773,21 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
   if (mask) {
     A.mask = mask->p; A.mask_cs = mask->cs; A.mask_c0 = mask->c0; A.mask_from = mask_from;
   }
+  // input channels per chunk: all of them up to 32, else 32 or 16
+  A.cc = cin <= 32 ? cin : cin % 32 == 0 ? 32 : cin % 16 == 0 ? 16 : 4;
+  const int kc = (9 * A.cc + 15) / 16 * 16;
   // rows per tile: 4 when weights and tile fit two workgroups per CU, else 2,
   // else the largest that fits one
   auto lds_of = [&](int mb) {
-    return (size_t)nb * 16 * A.kp * 2 + (size_t)(mb + 2 * A.dil) * (64 + 2 * A.dil) * (cin + 4) * 2;
+    return (size_t)nb * 16 * kc * 2 + (size_t)(mb + 2 * A.dil) * (64 + 2 * A.dil) * (A.cc + 4) * 2;
   };
   int MB = 0;
   for (size_t cap : {lds_max / 2, lds_max})
     for (int mb : {4, 2})
       if (!MB && lds_of(mb) <= cap) MB = mb;
   if (!MB || cin % 4) return fail(PP_ERR_INVALID, "conv: layer too wide for LDS");
-  if (cin > 96 || (MB + 2 * A.dil) * (64 + 2 * A.dil) * (cin / 4) >= 65536)
+  if (64 + 2 * A.dil > 72 || (MB + 2 * A.dil) * (64 + 2 * A.dil) * (A.cc / 4) >= 65536 ||
+      nb * 16 * (kc / 4) >= 65536 || kc / 4 > 72)
     return fail(PP_ERR_INVALID, "conv: input tile beyond the staging index range");
   const size_t lds = lds_of(MB);
   const int grid = ((W[lev] + 63) / 64) * ((H[lev] + MB - 1) / MB);
