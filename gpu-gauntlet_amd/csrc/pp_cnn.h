@@ -45,14 +45,15 @@ struct MgLevel {
 struct Mg {
   pp_plan* pl = nullptr;
   hipStream_t stream = nullptr;
-  int nu = 2;  // block-Jacobi sweeps before and after each coarse correction
+  int nu = 3;   // block-Jacobi sweeps before and after each coarse correction, level 0
+  int nuc = 1;  // the same on levels >= 1
   static constexpr int kMaxLevels = 16;
   MgLevel lv[kMaxLevels];
   int nl = 0;
   int nd_coarse = 0;
   DevBuf<double> cinv;  // dense inverse of the coarsest operator [nd][nd]
   DevBuf<double> dotp;
-  int setup(pp_plan* plan, double omega0, int sweeps);
+  int setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps);
   int vcycle(const double* b, double* x);  // x = V b on level 0 (device FP64 DOF vectors)
   int apply_level(int l, const double* x, double* y);  // y = A_l x
  private:

@@ -994,7 +994,9 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   else if ((rc = estimate_omega())) return rc;
   if (cfg && cfg->multigrid) {
     mg = new Mg();
-    if ((rc = mg->setup(pl, omega, cfg->mg_sweeps)) || (rc = mgz.alloc(pl->n_dof))) return rc;
+    if ((rc = mg->setup(pl, omega, cfg->mg_sweeps, cfg->mg_coarse_sweeps)) ||
+        (rc = mgz.alloc(pl->n_dof)))
+      return rc;
   }
   PP_HIP(hipStreamSynchronize(stream));
   return PP_OK;

@@ -350,11 +350,12 @@ bool spd_inverse(std::vector<double>& A, int n) {
 
 }  // namespace
 
-int Mg::setup(pp_plan* plan, double omega0, int sweeps) {
+int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps) {
   pl = plan;
   stream = plan->ctx->stream;
-  nu = sweeps > 0 ? sweeps : 2;
-  if (nu > 8) return fail(PP_ERR_INVALID, "mg_sweeps must be at most 8");
+  nu = sweeps > 0 ? sweeps : 3;
+  nuc = coarse_sweeps > 0 ? coarse_sweeps : 1;
+  if (nu > 8 || nuc > 8) return fail(PP_ERR_INVALID, "mg_sweeps must be at most 8");
   // hierarchy shape
   std::vector<std::pair<int, int>> dims{{pl->nx, pl->ny}};
   while (true) {
@@ -505,7 +506,7 @@ int Mg::sweep(int l, int mode, const double* x, double* xo) {
   return PP_OK;
 }
 
-// The sweeps ping-pong between L.x and L.x2: 2 nu - 1 of them follow the
+// The sweeps ping-pong between L.x and L.x2: 2 ns - 1 of them follow the
 // first (x = w B b), an odd count, so starting in x2 the cycle ends in L.x.
 int Mg::cycle(int l) {
   MgLevel& L = lv[l];
@@ -519,11 +520,12 @@ int Mg::cycle(int l) {
   const uint8_t* mask = l == 0 ? (const uint8_t*)pl->cmask.p : (const uint8_t*)L.dead.p;
   const int g = mg_blocks(L.nn);
   int rc;
+  const int ns = l == 0 ? nu : nuc;
   double* cur = L.x2.p;
   auto other = [&](double* p) { return p == L.x.p ? L.x2.p : L.x.p; };
   hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 0, B, L.omega,
                      (const double*)L.b.p, (double*)nullptr, cur);
-  for (int s = 1; s < nu; s++) {
+  for (int s = 1; s < ns; s++) {
     if ((rc = sweep(l, 1, cur, other(cur)))) return rc;
     cur = other(cur);
   }
@@ -535,7 +537,7 @@ int Mg::cycle(int l) {
   if ((rc = cycle(l + 1))) return rc;
   hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, stream, L.nx, L.ny, mask,
                      (const double*)C.x.p, C.nx, cur);
-  for (int s = 0; s < nu; s++) {
+  for (int s = 0; s < ns; s++) {
     if ((rc = sweep(l, 1, cur, other(cur)))) return rc;
     cur = other(cur);
   }

@@ -240,8 +240,11 @@ typedef struct {
                                   prolongation, damped block-Jacobi sweeps per level, dense
                                   coarsest solve).  Needs the element counts per side to halve
                                   down to a coarsest grid of at most 2,200 DOFs */
-  int32_t mg_sweeps;           /* block-Jacobi sweeps before and after each coarse correction;
-                                  0 = 2 */
+  int32_t mg_sweeps;           /* block-Jacobi sweeps before and after each coarse correction
+                                  on the finest level; 0 = 3 */
+  int32_t mg_coarse_sweeps;    /* the same on the coarser stencil levels; 0 = 1 (1024^2:
+                                  3/1 sweeps converge in 218 iterations, 0.37 s; 2/2 in 217,
+                                  0.40 s; 1/1 in 331, 0.50 s) */
 } pp_cnn_config;
 
 #define PP_TRAIN_CURVE 64

@@ -109,8 +109,10 @@ class MgOracle:
     4 / (3 * 1.05 * lambda_max(B A)) from the exact eigenvalue); factor:
     factor the coarsest level for the cycle."""
 
-    def __init__(self, A0, nx, ny, keep0, sweeps=2, omegas=None, blk0=None, factor=True):
-        self.nu = sweeps
+    def __init__(self, A0, nx, ny, keep0, sweeps=2, omegas=None, blk0=None, factor=True,
+                 coarse_sweeps=None):
+        self.nu = sweeps  # on level 0; coarse_sweeps (None: the same) on the others
+        self.nuc = sweeps if coarse_sweeps is None else coarse_sweeps
         self.dims = hierarchy_dims(nx, ny)
         self.A, self.P, self.B, self.keep = [A0], [], [], [keep0]
         for l in range(len(self.dims) - 1):
@@ -147,11 +149,12 @@ class MgOracle:
         if l == len(self.A) - 1:
             return sla.cho_solve(self.coarse, b)
         A, w = self.A[l], self.omega[l]
+        nu = self.nu if l == 0 else self.nuc
         x = w * self.bj(l, b)
-        for _ in range(self.nu - 1):
+        for _ in range(nu - 1):
             x = x + w * self.bj(l, b - A @ x)
         bc = self.P[l].T @ (b - A @ x)
         x = x + self.P[l] @ self.cycle(bc, l + 1)
-        for _ in range(self.nu):
+        for _ in range(nu):
             x = x + w * self.bj(l, b - A @ x)
         return x
