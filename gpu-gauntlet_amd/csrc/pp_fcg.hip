@@ -44,7 +44,7 @@ struct Win {
   int w;
 };
 
-constexpr int kBlocks = 256;
+constexpr int kBlocks = 1024;  // partial sums per reduction: 16 waves per CU stream the vectors
 constexpr int kCheckEvery = 4;
 
 int nb(int64_t n) { return (int)((n + 255) / 256); }
@@ -75,8 +75,8 @@ __global__ void k_dots(int64_t n, const double* a, const double* b, const double
   }
 }
 
-// Fixed-order sum of the kBlocks pairs of partials by one 256-thread block
-// (every thread gets the result).
+// Fixed-order sum of the kBlocks pairs of partials by one kBlocks-thread
+// block (every thread gets the result).
 __device__ void sum2(const double* part, double& a, double& b) {
   __shared__ double sa[kBlocks], sb[kBlocks];
   const int t = threadIdx.x;
@@ -472,7 +472,8 @@ int pp_cnn_gen_error(pp_cnn* c, uint64_t seed, double* e) {
 int pp_cnn_train(pp_cnn* c, const pp_train_opts* o, pp_train_stats* st) {
   if (!c || !o || !st) return fail(PP_ERR_INVALID, "NULL argument");
   if (o->pairs <= 0 || o->epochs <= 0 || !(o->lr > 0)) return fail(PP_ERR_INVALID, "bad training options");
-  if (o->source != PP_TRAIN_SYNTHETIC && o->source != PP_TRAIN_SOLVER)
+  if (o->source != PP_TRAIN_SYNTHETIC && o->source != PP_TRAIN_SOLVER &&
+      o->source != PP_TRAIN_LEARNED)
     return fail(PP_ERR_INVALID, "unknown training source");
   Cnn& n = c->net;
   pp_plan* pl = n.pl;
@@ -481,10 +482,19 @@ int pp_cnn_train(pp_cnn* c, const pp_train_opts* o, pp_train_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t nd = pl->n_dof;
   int rc;
-  // solver pairs: per load case the FP64 block-Jacobi solution x* (to 1e-12)
-  // and the errors x* - x_k of its iterates k = 0, then geometric up to the
-  // iteration count, kept on the device in FP32
-  const bool solver = o->source == PP_TRAIN_SOLVER;
+  // solver pairs: per load case the FP64 solution x* (to 1e-12) and the
+  // errors x* - x_k of its iterates k = 0, then geometric up to the iteration
+  // count, kept on the device in FP32; block-Jacobi PCG (PP_TRAIN_SOLVER) or
+  // flexible PCG with this network's current preconditioner (PP_TRAIN_LEARNED)
+  const bool solver = o->source == PP_TRAIN_SOLVER || o->source == PP_TRAIN_LEARNED;
+  Cnn* const attached = pl->learned;
+  auto solve = [&](pp_solve_opts* so, pp_solve_stats* ss) {
+    if (o->source == PP_TRAIN_SOLVER) return struct_solve_fp64(pl, so, ss);
+    pl->learned = &n;
+    const int r = learned_solve(pl, so, ss);
+    pl->learned = attached;
+    return r;
+  };
   const int snaps = solver ? std::max(1, o->snapshots) : 1;
   const int per_epoch = o->pairs * snaps;
   DevBuf<float> bank;
@@ -512,7 +522,7 @@ int pp_cnn_train(pp_cnn* c, const pp_train_opts* o, pp_train_stats* st) {
       if ((rc = pp_pressure_load(pl, 1e4, pt.data(), np, F.data()))) return rc;
       if ((rc = pp_set_rhs(pl, F.data()))) return rc;
       so.max_iterations = 1000000;
-      if ((rc = struct_solve_fp64(pl, &so, &ss))) return rc;
+      if ((rc = solve(&so, &ss))) return rc;
       const int kstar = std::max(1, ss.iterations);
       PP_HIP(hipMemcpyAsync(n.evec.p, pl->x.p, nd * 8, hipMemcpyDeviceToDevice, n.stream));
       for (int j = 0; j < snaps; j++) {
@@ -523,7 +533,7 @@ int pp_cnn_train(pp_cnn* c, const pp_train_opts* o, pp_train_stats* st) {
                              (const double*)n.evec.p, (const double*)nullptr, dst);
         } else {
           so.max_iterations = k;
-          if ((rc = struct_solve_fp64(pl, &so, &ss))) return rc;
+          if ((rc = solve(&so, &ss))) return rc;
           hipLaunchKernelGGL(k_err32, dim3(nb(nd)), dim3(256), 0, n.stream, nd,
                              (const double*)n.evec.p, (const double*)pl->x.p, dst);
         }

@@ -61,7 +61,7 @@ class SolveStats(C.Structure):
 
 
 TRAIN_CURVE = 64
-TRAIN_SYNTHETIC, TRAIN_SOLVER = 0, 1
+TRAIN_SYNTHETIC, TRAIN_SOLVER, TRAIN_LEARNED = 0, 1, 2
 
 
 class CnnConfig(C.Structure):

@@ -251,6 +251,9 @@ typedef struct {
 
 #define PP_TRAIN_SYNTHETIC 0 /* e: seeded multi-scale smooth fields, r = K e */
 #define PP_TRAIN_SOLVER 1    /* e = x* - x_k: block-Jacobi PCG iterates on seeded pressure loads */
+#define PP_TRAIN_LEARNED 2   /* e = x* - x_k: iterates of flexible PCG with the network's current
+                                preconditioner (x* by the same solve to 1e-12), so the pairs are
+                                the errors that preconditioner leaves */
 
 typedef struct {
   int32_t pairs;        /* synthetic: pairs (seeds seed0 ..); solver: load cases (seeds seed0 ..) */

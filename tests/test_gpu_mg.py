@@ -174,3 +174,25 @@ def test_learned_multigrid_solve(ctx):
     assert max(its.values()) < 0.1 * st_bj["iterations"]
     net.close()
     plan.close()
+
+
+def test_training_on_learned_iterates(ctx):
+    """PP_TRAIN_LEARNED: pairs from flexible-PCG iterates with the network's
+    own preconditioner.  The energy-norm loss falls over training and the
+    trained preconditioner still solves to 1e-8 with the FP64 answer."""
+    s = setup(ctx, 64, seed=8)
+    plan, net, op = s["plan"], s["net"], s["op"]
+    F = plan.pressure_load(1e4, np.array([[3e4, 3.0, 6.0, 1.5]]))
+    x_bj, _, _ = plan.solve(F, tolerance=1e-8, max_iterations=20000)
+    tr = net.train(pairs=4, snapshots=6, epochs=4, seed0=2000, lr=1e-3, lr_final=1e-5,
+                   max_grad_norm=1.0, source=_lib.TRAIN_LEARNED)
+    c = np.array(tr["loss_curve"])
+    q = len(c) // 4
+    print("loss %.4f -> %.4f" % (c[:q].mean(), c[-q:].mean()))
+    assert np.isfinite(c).all() and c[-q:].mean() < c[:q].mean()
+    net.attach()
+    x, _, st = plan.solve(F, tolerance=1e-8, max_iterations=2000, precond=LEARNED)
+    assert st["converged"] and st["trueRelResidual"] <= 1e-8, st
+    assert O.max_nodal_w_error(x, x_bj) < 1e-4
+    net.close()
+    plan.close()
