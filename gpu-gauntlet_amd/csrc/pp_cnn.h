@@ -36,8 +36,8 @@ struct MgLevel {
   int nx = 0, ny = 0;    // elements per side
   int64_t nn = 0;        // nodes (nx + 1) (ny + 1)
   double omega = 0.5;    // block-Jacobi smoothing weight 4 / (3 lambda_max(B A))
-  DevBuf<float> st;      // level >= 1: [9 neighbours][3][3][node], neighbour (dj + 1) * 3 + di + 1
-  DevBuf<double> minv;   // level >= 1: 6 per node (symmetric 3x3 block inverse)
+  DevBuf<float> st;      // level >= 1: [5 blocks][3][3][node]: centre, E, NW, N, NE
+  DevBuf<float> minv;    // 6 per node (symmetric 3x3 block inverse, FP32 for smoothing)
   DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
   DevBuf<double> b, x, x2, t;  // right-hand side, iterate and its sweep partner, work
 };

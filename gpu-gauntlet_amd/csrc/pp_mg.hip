@@ -17,10 +17,12 @@
 //                A DOF whose assembled diagonal is exactly zero (its basis
 //                function vanishes) is excluded on that level: identity row,
 //                masked in P.
-// Levels >= 1 keep A as an FP32 9-point block stencil (324 B per node, stored
-// as 81 planes of one float per node so that a wave's loads coalesce) and
-// an FP64 block-Jacobi inverse; the coarsest level (at most kDenseMax DOFs)
-// is solved by its dense FP64 inverse, formed on the host by Cholesky.
+// Levels >= 1 keep A as an FP32 9-point block stencil: by symmetry only the
+// centre and the four forward neighbour blocks (180 B per node, as 45 planes
+// of one float per node so that a wave's loads coalesce).  Every level but
+// the coarsest smooths with an FP32 copy of its block-Jacobi inverse; the
+// coarsest (at most kDenseMax DOFs) is solved by its dense FP64 inverse,
+// formed on the host by Cholesky.
 //
 // V-cycle on level l for right-hand side b (x starts at 0):
 //   x = w B b;  nu - 1 times x += w B (b - A x);   (pre-smoothing, nu sweeps)
@@ -115,7 +117,7 @@ __global__ void k_mg_coarse_elems(int ncx, int ncy, int fnx, const double* kf, K
 // element matrices: node (i, j) sums its elements SW, SE, NE, NW of it in
 // that fixed order.
 __global__ void k_mg_stencil(int nx, int ny, const double* ke, float* st, uint8_t* dead,
-                             double* minv) {
+                             float* minv) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
   const int i = (int)(n % (nx + 1)), j = (int)(n / (nx + 1));
@@ -143,37 +145,49 @@ __global__ void k_mg_stencil(int nx, int ny, const double* ke, float* st, uint8_
       S[4 * 9 + 4 * d] = 1.0;
     }
   dead[n] = (uint8_t)bits;
+  // A is symmetric: keep the centre block and the forward neighbours E, NW,
+  // N, NE (S blocks 4..8); a backward block is the transpose of that
+  // neighbour's forward block
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
-  for (int k = 0; k < 81; k++) st[k * nn + n] = (float)S[k];
-  double A[9];
+  for (int k = 0; k < 45; k++) st[k * nn + n] = (float)S[36 + k];
+  double A[9], m6[6];
   for (int k = 0; k < 9; k++) A[k] = S[4 * 9 + k];
-  block_inverse_sym(A, bits, minv + 6 * n);
+  block_inverse_sym(A, bits, m6);
+  for (int k = 0; k < 6; k++) minv[6 * n + k] = (float)m6[k];
 }
 
-// (A x)_n on a stencil level.  A neighbour outside the grid has exactly zero
-// stencil entries (no element contributes to them), so its index is clamped
-// into the grid instead of branching: every load is unconditional and a
-// thread's 108 loads are all in flight together.
+// (A x)_n on a stencil level.  Stored blocks (planes of one float per
+// node): 0 centre, 1 E (0, +1), 2 NW (+1, -1), 3 N (+1, 0), 4 NE (+1, +1)
+// as (dj, di).  A forward neighbour outside the grid has exactly zero entries
+// (no element contributes to them), so its index is clamped into the grid
+// instead of branching; a backward block A(n, m) = A(m, n)^T is read from
+// neighbour m's forward block and masked to zero outside the grid.  Every
+// load is unconditional, so a thread's loads are all in flight together.
 __device__ __forceinline__ void stencil_row(int nx, int ny, const float* st, const double* x,
                                             int64_t n, double (&y)[3]) {
   const int i = (int)(n % (nx + 1)), j = (int)(n / (nx + 1));
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
-  const float* S = st + n;  // entry k at S[k * nn]
+  const int fdj[5] = {0, 0, 1, 1, 1}, fdi[5] = {0, 1, -1, 0, 1};
   y[0] = y[1] = y[2] = 0.0;
 #pragma unroll
-  for (int dj = -1; dj <= 1; dj++) {
-    const int jj = min(max(j + dj, 0), ny);
+  for (int sb = 0; sb < 5; sb++) {
+    const int64_t m = (int64_t)min(j + fdj[sb], ny) * (nx + 1) + min(max(i + fdi[sb], 0), nx);
+    const double x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
+    const float* s = st + (int64_t)sb * 9 * nn + n;
+    y[0] += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
+    y[1] += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
+    y[2] += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
+  }
 #pragma unroll
-    for (int di = -1; di <= 1; di++) {
-      const int ii = min(max(i + di, 0), nx);
-      const int nb = (dj + 1) * 3 + di + 1;
-      const int64_t m = (int64_t)jj * (nx + 1) + ii;
-      const double x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
-      const float* s = S + 9 * nb * nn;
-      y[0] += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
-      y[1] += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
-      y[2] += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
-    }
+  for (int sb = 1; sb < 5; sb++) {
+    const int jj = j - fdj[sb], ii = i - fdi[sb];
+    const double v = (jj >= 0 && ii >= 0 && ii <= nx) ? 1.0 : 0.0;
+    const int64_t m = (int64_t)max(jj, 0) * (nx + 1) + min(max(ii, 0), nx);
+    const double x0 = v * x[3 * m], x1 = v * x[3 * m + 1], x2 = v * x[3 * m + 2];
+    const float* s = st + (int64_t)sb * 9 * nn + m;  // A(m, n) = A(n, m)^T
+    y[0] += (double)s[0] * x0 + (double)s[3 * nn] * x1 + (double)s[6 * nn] * x2;
+    y[1] += (double)s[nn] * x0 + (double)s[4 * nn] * x1 + (double)s[7 * nn] * x2;
+    y[2] += (double)s[2 * nn] * x0 + (double)s[5 * nn] * x1 + (double)s[8 * nn] * x2;
   }
 }
 
@@ -189,7 +203,7 @@ __global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, dou
 // Fused sweep on a stencil level: MODE 1: xo = x + w B (b - A x) (xo must
 // not alias x); MODE 2: xo = b - A x.
 template <int MODE>
-__global__ void k_mg_sweep(int nx, int ny, const float* st, const double* minv, double omega,
+__global__ void k_mg_sweep(int nx, int ny, const float* st, const float* minv, double omega,
                            const double* x, const double* b, double* xo) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
@@ -206,7 +220,7 @@ __global__ void k_mg_sweep(int nx, int ny, const float* st, const double* minv, 
 }
 
 // mode 0: x = w B b;  mode 1: x += w B (b - t);  mode 2: t = b - t
-__global__ void k_mg_point(int64_t nn, int mode, const double* minv, double omega, const double* b,
+__global__ void k_mg_point(int64_t nn, int mode, const float* minv, double omega, const double* b,
                            double* t, double* x) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
@@ -306,6 +320,11 @@ __global__ void k_mg_dot2(int64_t n, const double* a, const double* b, double* p
   }
 }
 
+__global__ void k_mg_f32(int64_t n, const double* a, float* b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
 __global__ void k_mg_scale(int64_t n, double* v, double f) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] *= f;
@@ -379,13 +398,17 @@ int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps) {
     L.ny = dims[l].second;
     L.nn = (int64_t)(L.nx + 1) * (L.ny + 1);
     if ((rc = L.t.alloc(3 * L.nn))) return rc;
-    if (l < nl - 1 && (rc = L.x2.alloc(3 * L.nn))) return rc;
+    if (l < nl - 1 && ((rc = L.x2.alloc(3 * L.nn)) || (rc = L.minv.alloc(6 * L.nn)))) return rc;
     if (l > 0 && ((rc = L.b.alloc(3 * L.nn)) || (rc = L.x.alloc(3 * L.nn)))) return rc;
     if (l > 0 && l < nl - 1 &&
-        ((rc = L.st.alloc(81 * L.nn)) || (rc = L.minv.alloc(6 * L.nn)) || (rc = L.dead.alloc(L.nn))))
+        ((rc = L.st.alloc(45 * L.nn)) || (rc = L.dead.alloc(L.nn))))
       return rc;
   }
   lv[0].omega = omega0;
+  // level 0 smooths with an FP32 copy of the plan's block-Jacobi inverse
+  hipLaunchKernelGGL(k_mg_f32, dim3(mg_blocks(6 * lv[0].nn)), dim3(256), 0, stream, 6 * lv[0].nn,
+                     (const double*)pl->minv.p, lv[0].minv.p);
+  PP_HIP(hipGetLastError());
   if ((rc = dotp.alloc(2 * kMgDotBlocks))) return rc;
   // coarse element matrices level by level (two alive at a time)
   KGrid G = make_kgrid(pl);
@@ -469,7 +492,7 @@ int Mg::estimate_omega(int l) {
     int rc;
     if ((rc = apply_level(l, L.x.p, L.t.p))) return rc;
     hipLaunchKernelGGL(k_mg_point, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nn, 0,
-                       (const double*)L.minv.p, 1.0, (const double*)L.t.p, (double*)nullptr, L.b.p);
+                       (const float*)L.minv.p, 1.0, (const double*)L.t.p, (double*)nullptr, L.b.p);
     hipLaunchKernelGGL(k_mg_dot2, dim3(kMgDotBlocks), dim3(256), 0, stream, nd,
                        (const double*)L.b.p, (const double*)L.x.p, dotp.p);
     double hp[2 * kMgDotBlocks];
@@ -493,14 +516,14 @@ int Mg::estimate_omega(int l) {
 
 int Mg::sweep(int l, int mode, const double* x, double* xo) {
   MgLevel& L = lv[l];
-  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, L.omega, xo);
+  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, (const float*)L.minv.p, L.omega, xo);
   if (mode == 1)
     hipLaunchKernelGGL(k_mg_sweep<1>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
-                       (const float*)L.st.p, (const double*)L.minv.p, L.omega, x,
+                       (const float*)L.st.p, (const float*)L.minv.p, L.omega, x,
                        (const double*)L.b.p, xo);
   else
     hipLaunchKernelGGL(k_mg_sweep<2>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
-                       (const float*)L.st.p, (const double*)L.minv.p, L.omega, x,
+                       (const float*)L.st.p, (const float*)L.minv.p, L.omega, x,
                        (const double*)L.b.p, xo);
   PP_HIP(hipGetLastError());
   return PP_OK;
@@ -516,7 +539,7 @@ int Mg::cycle(int l) {
     PP_HIP(hipGetLastError());
     return PP_OK;
   }
-  const double* B = l == 0 ? (const double*)pl->minv.p : (const double*)L.minv.p;
+  const float* B = L.minv.p;
   const uint8_t* mask = l == 0 ? (const uint8_t*)pl->cmask.p : (const uint8_t*)L.dead.p;
   const int g = mg_blocks(L.nn);
   int rc;

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(CfgS::NT) void k_apply(KGrid G, const double* x, do
 // alias x, whose halo other workgroups read); MODE 2: xo = b - K x.
 template <int MODE>
 __global__ __launch_bounds__(CfgS::NT) void k_apply_sweep(KGrid G, const double* x,
-                                                          const double* b, const double* minv,
+                                                          const double* b, const float* minv,
                                                           double omega, double* xo) {
   using C = CfgS;
   __shared__ double p_s[3 * C::NP];
@@ -952,16 +952,17 @@ int struct_apply_k(pp_plan* pl, const double* dx, double* dy) {
   return PP_OK;
 }
 
-int struct_sweep(pp_plan* pl, int mode, const double* x, const double* b, double omega, double* xo) {
+int struct_sweep(pp_plan* pl, int mode, const double* x, const double* b, const float* minv,
+                 double omega, double* xo) {
   KGrid G = make_kgrid(pl);
   const int ntiles =
       ((pl->nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((pl->ny + 1 + CfgS::TY - 1) / CfgS::TY);
   if (mode == 1)
     hipLaunchKernelGGL(k_apply_sweep<1>, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, x, b,
-                       (const double*)pl->minv.p, omega, xo);
+                       minv, omega, xo);
   else
     hipLaunchKernelGGL(k_apply_sweep<2>, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, x, b,
-                       (const double*)pl->minv.p, omega, xo);
+                       minv, omega, xo);
   PP_HIP(hipGetLastError());
   return PP_OK;
 }
