@@ -47,13 +47,8 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-// round to nearest even (NaN kept quiet)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// round to nearest even (NaN kept quiet): one v_cvt_pk_bf16_f32 on gfx950
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 int nblk(int64_t n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 
@@ -169,7 +164,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
           const int gx = tx0 - d + col, gy = ty0 - d + row;
           dst[u] = (row * TW + col) * CS + 4 * c;
           if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
-            v[u] = *(const uint2*)&A.in[((size_t)gy * A.W + gx) * A.in_cs + A.in_c0 + c0 + 4 * c];
+            v[u] = *(const uint2*)&A.in[(gy * A.W + gx) * A.in_cs + A.in_c0 + c0 + 4 * c];
         }
       }
 #pragma unroll
@@ -177,37 +172,62 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
         if (dst[u] >= 0) *(uint2*)&in_s[dst[u]] = v[u];
     }
     __syncthreads();
-    // (tap, channel) of this lane's k = k0 + 4 kq, advanced by 16 per step
-    // without branches (at most 16 / CC <= 4 wraps).  Steps past 9 CC have
-    // zero weights, so their A reads may use any pixel of the tile: they take
-    // the lane's own (toff 0) and every read is unconditional.
-    int tap = 0, ci = 4 * kq;
-    auto wrap = [&]() {
+    if (CC % 16 == 0) {
+      // every 16-wide K step lies within one tap: the tap's LDS offset is
+      // uniform over the wave, the lane adds its pixel and channel quad
+#pragma unroll 1
+      for (int tap = 0; tap < 9; tap++) {
+        const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
+        const int toff = (ty * d * TW + (tap - 3 * ty) * d) * CS + 4 * kq;
+        for (int c16 = 0; c16 < CC; c16 += 16) {
+          const int kb = tap * CC + c16 + 4 * kq;
+          s4 b[NB];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int w = ci >= CC;
-        ci -= w * CC;
-        tap += w;
+          for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KC + kb];
+          s4 a[MB];
+#pragma unroll
+          for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff + c16];
+#pragma unroll
+          for (int j = 0; j < MB; j++)
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+              acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+        }
       }
-    };
-    wrap();
-    for (int k0 = 0; k0 < KC; k0 += 16) {
-      const int kb = k0 + 4 * kq;
-      s4 b[NB];
+    } else {
+      // CC = 4, 8 or 12: (tap, channel) of this lane's k = k0 + 4 kq, advanced
+      // by 16 per step without branches (at most 16 / CC <= 4 wraps).  Steps
+      // past 9 CC have zero weights, so their A reads may use any pixel of
+      // the tile: they take the lane's own (toff 0) and every read is
+      // unconditional.
+      int tap = 0, ci = 4 * kq;
+      auto wrap = [&]() {
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KC + kb];
-      const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
-      const int toff = kb < k9 ? (ty * d * TW + (tap - 3 * ty) * d) * CS + ci : 0;
-      s4 a[MB];
-#pragma unroll
-      for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff];
-#pragma unroll
-      for (int j = 0; j < MB; j++)
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-          acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
-      ci += 16;
+        for (int r = 0; r < 4; r++) {
+          const int w = ci >= CC;
+          ci -= w * CC;
+          tap += w;
+        }
+      };
       wrap();
+      for (int k0 = 0; k0 < KC; k0 += 16) {
+        const int kb = k0 + 4 * kq;
+        s4 b[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KC + kb];
+        const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
+        const int toff = kb < k9 ? (ty * d * TW + (tap - 3 * ty) * d) * CS + ci : 0;
+        s4 a[MB];
+#pragma unroll
+        for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff];
+#pragma unroll
+        for (int j = 0; j < MB; j++)
+#pragma unroll
+          for (int nb = 0; nb < NB; nb++)
+            acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+        ci += 16;
+        wrap();
+      }
     }
   }
 #pragma unroll
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
     const int mb = wave * MB + j;
     const int oy = ty0 + (mb >> 2), ox = tx0 + (mb & 3) * 16 + m;
     if (oy >= A.H || ox >= A.W) continue;
-    const size_t op = (size_t)oy * A.W + ox;
+    const int op = oy * A.W + ox;  // 32-bit offsets: the launch checks they fit
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
       const int co = nb * 16 + 4 * kq;
@@ -786,6 +806,10 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
     for (int mb : {4, 2})
       if (!MB && lds_of(mb) <= cap) MB = mb;
   if (!MB || cin % 4) return fail(PP_ERR_INVALID, "conv: layer too wide for LDS");
+  const int64_t cs_max = std::max({(int64_t)A.in_cs, (int64_t)A.out_cs, (int64_t)A.out32_cs,
+                                    (int64_t)A.mask_cs});
+  if ((int64_t)H[lev] * W[lev] * cs_max + cs_max >= (1LL << 31))
+    return fail(PP_ERR_INVALID, "conv: tensor too large for 32-bit offsets");
   if (64 + 2 * A.dil > 72 || (MB + 2 * A.dil) * (64 + 2 * A.dil) * (A.cc / 4) >= 65536 ||
       nb * 16 * (kc / 4) >= 65536 || kc / 4 > 72)
     return fail(PP_ERR_INVALID, "conv: input tile beyond the staging index range");
