@@ -69,7 +69,7 @@ struct Cnn {
   size_t lds_max = 65536;  // dynamic LDS per workgroup of the device
   int H[3] = {0, 0, 0}, W[3] = {0, 0, 0};
   double sw = 1.0, st = 0.3;  // input scales of the w and rotation channels
-  double omega = 0.5;         // block-Jacobi smoothing weight around the network
+  double omega = 0.5;         // block-Jacobi smoothing weight (S = w B, and level 0 of the V-cycle)
   int cat0C = 0, cat1C = 0;
   std::vector<Layer> layers;
   int64_t n_params = 0;

@@ -16,17 +16,23 @@
 // every conv 3x3; with ReLU and biases (all but the head) or, by default,
 // without either, so the network is a fixed linear operator.  Inputs: the
 // block-Jacobi preconditioned residual z = M^-1 r (3 channels, scaled per
-// channel and by its RMS s) and log(t / t0) times the first of them.  Output: M(r) = z + s * o / cin_scale
-// (the network learns the correction beyond block Jacobi).
+// channel and by its RMS s) and log(t / t0) times the first of them.
+// Output: N(r) = z + s * o / cin_scale (the network learns the correction
+// beyond block Jacobi).
+//
+// The preconditioner wraps the network in a smoother S (damped block Jacobi,
+// or a Galerkin multigrid V-cycle from pp_mg.hip): z0 = S r, a correction
+// from the network on r - K z0 with its energy-optimal step, then S again.
 //
 // Training pairs: e from a seed (a windowed sum of Gaussian bumps in w with
-// Kirchhoff-consistent rotations, zero on constrained DOFs), r = K e with the
-// solver's own FP64 operator.  Loss: the energy norm of the error,
-// (M(r) - e)^T K (M(r) - e) / e^T K e, whose gradient in M(r) is one more
-// K apply.  Adam on FP32 master weights.
+// Kirchhoff-consistent rotations, zero on constrained DOFs) or from solver
+// iterates, r = K e with the solver's own FP64 operator.  Loss: the energy
+// norm of the error of the whole preconditioner, (M(r) - e)^T K (M(r) - e) /
+// e^T K e.  Adam on FP32 master weights.
 //
-// Convolution as implicit GEMM on v_mfma_f32_16x16x16bf16_1k: M = 16 pixels,
-// N = 16 output channels, K = (tap, input channel) in chunks of 16.  The
+// Convolution as implicit GEMM on v_mfma_f32_16x16x16bf16_1k: M = 16 output
+// channels (the weights are the A operand), N = 16 pixels, K = (tap, input
+// channel) in steps of 16, input channels staged through LDS in chunks.  The
 // backward-data pass is the same kernel with the flipped, transposed weights;
 // the weight gradient is a GEMM over pixels with LDS-staged transposed tiles
 // and deterministic partial sums.
