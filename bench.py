@@ -270,6 +270,18 @@ def bench_learned(a):
                      "kernel": "k_pcg (block-Jacobi FP64 PCG, tiles strided; the comparison leg)",
                      "algorithmic_bytes_per_launch": BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj},
     }
+    if a.cpu_sample_iters != 0:
+        try:
+            per, st_js = js_ms_per_dof_iteration()
+            out["cpu_baseline"] = {
+                "value": per * n_dof * it_bj, "unit": "ms/solve", "cores": 1, "kind": "port",
+                "sample": (f"float32 solver.ts restatement in Node {st_js['node']} on "
+                           f"{st_js['cpuModel']}: 300 PCG iterations of a 64 x 64 Q4 plate timed, "
+                           f"{per * 1e6:.3f} ns per DOF-iteration, extrapolated x {n_dof} DOF x "
+                           f"{it_bj} block-Jacobi iterations (the reference takes one thickness "
+                           f"per plate, so it cannot run this plate itself; extrapolation)")}
+        except (OSError, subprocess.CalledProcessError) as e:
+            out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))
     net.close()
     plan.close()
