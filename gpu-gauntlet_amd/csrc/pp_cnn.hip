@@ -92,7 +92,7 @@ struct ConvArgs {
   int mask_cs, mask_c0, mask_from;
 };
 
-// One workgroup per output tile of MB rows x 64 pixels.  The input tile
+// One workgroup per output tile of MB rows x 64 pixels (MB = 2 or 4).  The input tile
 // with its halo, (MB + 2 dil) x (64 + 2 dil) pixels x cin channels, is
 // staged in LDS by independent 8-byte loads (pixel stride cin + 4 channels,
 // which spreads a wave's 16 pixels over the LDS banks), next to the weights
@@ -102,7 +102,7 @@ struct ConvArgs {
 // four consecutive output channels of one pixel: one 8-byte store.
 constexpr int kStage = 8;  // staging loads in flight per thread
 
-template <int NB, int MB>
+template <int NB, int MB, bool U16>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   extern __shared__ uint16_t smem[];
   // input channels in chunks of CC: per chunk the tile holds CC channels and
@@ -178,9 +178,9 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
         if (dst[u] >= 0) *(uint2*)&in_s[dst[u]] = v[u];
     }
     __syncthreads();
-    if (CC % 16 == 0) {
-      // every 16-wide K step lies within one tap: the tap's LDS offset is
-      // uniform over the wave, the lane adds its pixel and channel quad
+    if constexpr (U16) {
+      // CC % 16 == 0: every 16-wide K step lies within one tap, whose LDS
+      // offset is uniform over the wave; the lane adds its pixel and quad
 #pragma unroll 1
       for (int tap = 0; tap < 9; tap++) {
         const int ty = tap >= 6 ? 2 : tap >= 3 ? 1 : 0;
@@ -822,20 +822,29 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
   const size_t lds = lds_of(MB);
   const int grid = ((W[lev] + 63) / 64) * ((H[lev] + MB - 1) / MB);
   hipStream_t s = stream;
-#define PP_CONV_CASE(N)                                                                     \
-  case N:                                                                                  \
-    if (MB == 4) hipLaunchKernelGGL((k_conv<N, 4>), dim3(grid), dim3(256), lds, s, A);    \
-    else hipLaunchKernelGGL((k_conv<N, 2>), dim3(grid), dim3(256), lds, s, A);            \
+  const bool u16 = A.cc % 16 == 0;
+  if (!u16 && nb != 1) return fail(PP_ERR_INVALID, "conv: narrow input chunks need one N block");
+#define PP_CONV(N, R, U) hipLaunchKernelGGL((k_conv<N, R, U>), dim3(grid), dim3(256), lds, s, A)
+#define PP_CONV_CASE(N)                  \
+  case N:                                \
+    if (MB == 4) PP_CONV(N, 4, true);    \
+    else PP_CONV(N, 2, true);            \
     break;
-  switch (nb) {
-    PP_CONV_CASE(1)
-    PP_CONV_CASE(2)
-    PP_CONV_CASE(3)
-    PP_CONV_CASE(4)
-    PP_CONV_CASE(6)
-    default: return fail(PP_ERR_INVALID, "conv: unsupported channel count");
+  if (!u16) {
+    if (MB == 4) PP_CONV(1, 4, false);
+    else PP_CONV(1, 2, false);
+  } else {
+    switch (nb) {
+      PP_CONV_CASE(1)
+      PP_CONV_CASE(2)
+      PP_CONV_CASE(3)
+      PP_CONV_CASE(4)
+      PP_CONV_CASE(6)
+      default: return fail(PP_ERR_INVALID, "conv: unsupported channel count");
+    }
   }
 #undef PP_CONV_CASE
+#undef PP_CONV
   PP_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -1004,10 +1013,12 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
     PP_HIP(hipDeviceGetAttribute(&o, hipDeviceAttributeSharedMemPerBlockOptin, plan->ctx->device));
     lds_max = (size_t)std::max(v, o);
   }
-  const void* ks[] = {(const void*)k_conv<1, 4>, (const void*)k_conv<2, 4>, (const void*)k_conv<3, 4>,
-                      (const void*)k_conv<4, 4>, (const void*)k_conv<6, 4>, (const void*)k_conv<1, 2>,
-                      (const void*)k_conv<2, 2>, (const void*)k_conv<3, 2>, (const void*)k_conv<4, 2>,
-                      (const void*)k_conv<6, 2>};
+  const void* ks[] = {(const void*)k_conv<1, 4, true>,  (const void*)k_conv<2, 4, true>,
+                      (const void*)k_conv<3, 4, true>,  (const void*)k_conv<4, 4, true>,
+                      (const void*)k_conv<6, 4, true>,  (const void*)k_conv<1, 2, true>,
+                      (const void*)k_conv<2, 2, true>,  (const void*)k_conv<3, 2, true>,
+                      (const void*)k_conv<4, 2, true>,  (const void*)k_conv<6, 2, true>,
+                      (const void*)k_conv<1, 4, false>, (const void*)k_conv<1, 2, false>};
   for (auto k : ks)
     PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
   size_t mw = 0;
