@@ -318,7 +318,8 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   if (!pl || !o || !st) return fail(PP_ERR_INVALID, "NULL argument");
   if (!pl->rhs_set) return fail(PP_ERR_INVALID, "no right-hand side set");
   if (!(o->tolerance > 0) || o->max_iterations < 0) return fail(PP_ERR_INVALID, "bad tolerance/max_iterations");
-  if (o->precond != PP_PRECOND_BLOCK_JACOBI && o->precond != PP_PRECOND_LEARNED)
+  if (o->precond != PP_PRECOND_BLOCK_JACOBI && o->precond != PP_PRECOND_LEARNED &&
+      o->precond != PP_PRECOND_CHEBYSHEV)
     return fail(PP_ERR_INVALID, "unknown preconditioner");
   memset(st, 0, sizeof(*st));
   PP_HIP(hipSetDevice(pl->ctx->device));
@@ -331,6 +332,10 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     if (o->precision != PP_PRECISION_FP64)
       return fail(PP_ERR_UNSUPPORTED, "the learned preconditioner runs with FP64 flexible PCG");
     rc = learned_solve(pl, o, st);
+  } else if (o->precond == PP_PRECOND_CHEBYSHEV) {
+    if (!structured || o->precision != PP_PRECISION_FP64)
+      return fail(PP_ERR_UNSUPPORTED, "the Chebyshev preconditioner runs on the structured FP64 path");
+    rc = struct_solve_fp64(pl, o, st);
   } else if (o->precision == PP_PRECISION_FP64) {
     rc = structured ? struct_solve_fp64(pl, o, st) : general_solve_fp64(pl, o, st);
   } else if (o->precision == PP_PRECISION_MIXED) {

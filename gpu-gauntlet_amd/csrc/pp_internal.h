@@ -111,6 +111,8 @@ struct pp_plan {
   pp::DevBuf<uint8_t> cmask;          // per-node PP_FIX bits
   pp::DevBuf<uint8_t> cmask_dof;      // per-DOF 0/1 (coloured path)
   pp::DevBuf<double> minv;            // 6 per node (symmetric 3x3 inverse)
+  double cheb_lmax = 0.0;             // lambda_max(B K) for PP_PRECOND_CHEBYSHEV (0: not yet)
+  int cheb_degree = 8;                // degree of the solve being launched
   pp::DevBuf<double> ke;              // coloured path: per-element FP64 Ke
   pp::DevBuf<uint32_t> elems;         // coloured path: connectivity
   pp::DevBuf<uint32_t> by_colour;     // coloured path: elements by colour

@@ -800,6 +800,221 @@ __global__ __launch_bounds__(C::NT) void k_pcg_cg(PcgArgs A) {
   }
 }
 
+// ------------------------- Chebyshev-preconditioned single-reduction PCG
+//
+// The recurrence of k_pcg_cg with M = q_d(B K) B: u = M r is d steps of the
+// Chebyshev semi-iteration for B K z = B r from z = 0 on [lmin, lmax]
+// (theta = (lmax + lmin) / 2, delta = (lmax - lmin) / 2, sigma = theta /
+// delta):
+//   z1 = B r / theta;  rho_1 = 1 / sigma
+//   z_{k+1} = z_k + dz_k,  dz_k = rho_k rho_{k-1} dz_{k-1} + (2 rho_k / delta) B (r - K z_k),
+//   rho_k = 1 / (2 sigma - rho_{k-1}),
+// i.e. q_d(l) = (1 - T_d((theta - l) / delta) / T_d(sigma)) / l > 0 on (0, lmax]: M is
+// symmetric positive definite, and CG needs about d times fewer iterations
+// (and all-reduces) for about the same number of K applies.  Each K apply
+// needs the halo of its input: the owners publish their tile edges as
+// tagged words (the exchange step's low 8 bits replace the mantissa's, as in
+// grid_allreduce; relative change 2^-44), double-buffered by step parity, and
+// each halo-ring thread polls its node's three words until the tags match
+// (bounded spin; the abort word ends every waiter).  No flag and no store
+// fence: the tag travels with the data.  A neighbour can be at most one step
+// ahead: to write step s + 2 into the same buffer it must first have read
+// this workgroup's step s + 1, written only after this workgroup read step s.
+struct ChebArgs {
+  double theta, delta;
+  int degree;
+};
+
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_pcg_cheb(PcgArgs A, ChebArgs Q) {
+  constexpr int NRING = 2 * C::PW + 2 * C::TY;
+  static_assert(NRING <= C::NT, "one ring node per thread");
+  __shared__ double u_s[3 * C::NP];
+  __shared__ double ye_s[12 * C::NE];
+  __shared__ double red[80];
+  const unsigned me = blockIdx.x, nwg = gridDim.x;
+  const KGrid& G = A.G;
+  const int ntx = tiles_x<C>(G);
+  const int i0 = (int)(me % ntx) * C::TX, j0 = (int)(me / ntx) * C::TY;
+  const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
+  const int gi = i0 + tx, gj = j0 + ty;
+  const bool mine = threadIdx.x < C::TX * C::TY && gi <= G.nx && gj <= G.ny;
+  const int64_t n = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
+  const bool edge = mine && (tx == 0 || ty == 0 || tx == C::TX - 1 || ty == C::TY - 1 ||
+                             gi == G.nx || gj == G.ny);
+  const int c = (ty + 1) * C::PW + tx + 1;
+  int rpos = -1;
+  int64_t rnode = -1;
+  if (threadIdx.x < NRING) {
+    int lx, ly;
+    ring_pos<C>(threadIdx.x, lx, ly);
+    rpos = ly * C::PW + lx;
+    const int hi = i0 - 1 + lx, hj = j0 - 1 + ly;
+    if (hi >= 0 && hi <= G.nx && hj >= 0 && hj <= G.ny) rnode = (int64_t)hj * (G.nx + 1) + hi;
+  }
+  unsigned epoch = 0, step = 0;
+
+  double x[3] = {0, 0, 0}, r[3] = {0, 0, 0}, u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+  double p[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+  const unsigned fix = mine ? G.cmask[n] : 0u;
+  const bool restart = A.r0 != nullptr;
+  const double* src = restart ? A.r0 : A.F;
+  const double* mown = A.minv + 6 * n;
+  if (mine)
+    for (int d = 0; d < 3; d++) {
+      r[d] = src[3 * n + d];
+      x[d] = restart ? A.x[3 * n + d] : 0.0;
+    }
+
+  // stage v on the tile and its halo ring in u_s (see above); false on abort
+  auto exchange = [&](const double (&v)[3]) -> bool {
+    step++;
+    const unsigned tag = (step - 1u) % 255u + 1u;  // never 0: the buffers start zeroed
+    unsigned long long* pub = (unsigned long long*)((step & 1u) ? A.pb1 : A.pb0);
+    if (edge)
+      for (int d = 0; d < 3; d++)
+        __hip_atomic_store(&pub[3 * n + d], tag_word(v[d], tag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    if (mine) {
+      for (int d = 0; d < 3; d++) u_s[d * C::NP + c] = v[d];
+    } else if (threadIdx.x < C::TX * C::TY) {
+      for (int d = 0; d < 3; d++) u_s[d * C::NP + c] = 0.0;
+    }
+    int bad = 0;
+    if (rpos >= 0) {
+      double hv[3] = {0.0, 0.0, 0.0};
+      if (rnode >= 0) {
+        unsigned spins = 0;
+        for (;;) {
+          unsigned long long q[3];
+          for (int d = 0; d < 3; d++)
+            q[d] = __hip_atomic_load(&pub[3 * rnode + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((unsigned)(q[0] & 0xffu) == tag && (unsigned)(q[1] & 0xffu) == tag &&
+              (unsigned)(q[2] & 0xffu) == tag) {
+            for (int d = 0; d < 3; d++) hv[d] = untag_word(q[d]);
+            break;
+          }
+          if (++spins > kSpinLimit ||
+              __hip_atomic_load(&A.gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+            __hip_atomic_store(&A.gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bad = 1;
+            break;
+          }
+        }
+      }
+      for (int d = 0; d < 3; d++) u_s[d * C::NP + rpos] = hv[d];
+    }
+    return !__syncthreads_or(bad);
+  };
+  // K v on the own nodes from the staged u_s, with the constraint rows
+  auto apply = [&](const double (&v)[3], double (&kv)[3]) {
+    tile_elements<C, double>(G, i0, j0, u_s, ye_s);
+    __syncthreads();
+    if (mine) {
+      double t[3];
+      node_sum<C, double>(tx, ty, ye_s, t);
+      for (int d = 0; d < 3; d++) kv[d] = ((fix >> d) & 1u) ? v[d] : t[d];
+    }
+    __syncthreads();
+  };
+  // u = q_d(B K) B r
+  auto precond = [&]() -> bool {
+    const double sigma = Q.theta / Q.delta;
+    double rho = 1.0 / sigma;
+    double z[3] = {0, 0, 0}, dz[3] = {0, 0, 0};
+    if (mine) {
+      double res[3];
+      precond_fma(mown, r, res);
+      for (int d = 0; d < 3; d++) dz[d] = res[d] / Q.theta;
+    }
+    for (int k = 0; k < Q.degree; k++) {
+      for (int d = 0; d < 3; d++) z[d] += dz[d];
+      if (k == Q.degree - 1) break;
+      if (!exchange(z)) return false;
+      double kz[3] = {0, 0, 0};
+      apply(z, kz);
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      if (mine) {
+        double t[3], res[3];
+        for (int d = 0; d < 3; d++) t[d] = r[d] - kz[d];
+        precond_fma(mown, t, res);
+        for (int d = 0; d < 3; d++) dz[d] = rn * rho * dz[d] + 2.0 * rn / Q.delta * res[d];
+      }
+      rho = rn;
+    }
+    for (int d = 0; d < 3; d++) u[d] = z[d];
+    return true;
+  };
+  // u = M r, w = K u, partials r.u, w.u, r.r
+  auto matvec = [&](double (&acc)[3]) -> bool {
+    if (!precond() || !exchange(u)) return false;
+    apply(u, w);
+    acc[0] = acc[1] = acc[2] = 0;
+    if (mine)
+      for (int d = 0; d < 3; d++) {
+        acc[0] += r[d] * u[d];
+        acc[1] += w[d] * u[d];
+        acc[2] += r[d] * r[d];
+      }
+    return true;
+  };
+  auto allreduce = [&](double (&v)[3]) -> bool {
+    publish();
+    block_sum<3, C::NT>(v, red + 16);
+    return grid_allreduce<3, 2>(v, A.part, A.gs, epoch, red, nwg, me);
+  };
+
+  double acc[3];
+  if (!matvec(acc) || !allreduce(acc)) return;
+  double g = acc[0], pap = acc[1];
+  const double bnorm = restart ? A.bnorm_in : sqrt(acc[2]);
+  double rho = acc[2];
+  int it = 0, converged = 0, breakdown = 0;
+  const bool done0 = restart ? (sqrt(rho) < A.tol || sqrt(rho) / bnorm < A.tol) : bnorm < 1e-30;
+  if (done0) {
+    converged = 1;
+  } else if (A.maxit > 0 && fabs(pap) < 1e-30) {
+    breakdown = 1;
+    it = 1;
+  } else {
+    double a = g / pap;
+    for (int d = 0; d < 3; d++) { p[d] = u[d]; s[d] = w[d]; }
+    for (; it < A.maxit;) {
+      if (mine)
+        for (int d = 0; d < 3; d++) x[d] = fma(a, p[d], x[d]);
+      cg_rs(a, s, r);
+      if (!matvec(acc) || !allreduce(acc)) return;
+      it++;
+      const double g1 = acc[0], d1 = acc[1];
+      rho = acc[2];
+      const double rn = sqrt(rho);
+      if (rn < A.tol || rn / bnorm < A.tol) { converged = 1; break; }
+      const double b = g1 / g;
+      pap = d1 - b * g1 / a;
+      if (fabs(pap) < 1e-30) { breakdown = 1; it++; break; }
+      a = g1 / pap;
+      g = g1;
+      for (int d = 0; d < 3; d++) p[d] = fma(b, p[d], u[d]);
+      cg_sw(b, w, s);
+    }
+  }
+  if (mine)
+    for (int d = 0; d < 3; d++) A.x[3 * n + d] = x[d];
+  if (me == 0 && threadIdx.x == 0) {
+    A.st->iterations = it;
+    A.st->converged = converged;
+    A.st->breakdown = breakdown;
+    A.st->final_residual = sqrt(rho);
+    A.st->bnorm = bnorm;
+    for (int k = 0; k < 6; k++) A.st->ticks[k] = 0;
+  }
+}
+
+__global__ void k_scale_dof(int64_t n, double* v, double f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= f;
+}
+
 __global__ void k_zero_sync(GridSync* gs) {
   if (threadIdx.x < 64) {
     gs->count = 0;
@@ -817,7 +1032,7 @@ using CfgR = Tile<18, 18, 384>;
 
 struct PcgShape {
   // 0 = 31x31 resident, 1 = 15x15 resident, 2 = 18x18 strided,
-  // 3 = 18x18 single-reduction resident
+  // 3 = 18x18 single-reduction resident, 4 = 18x18 Chebyshev resident
   int kind;
   int grid;
 };
@@ -832,6 +1047,15 @@ int pcg_shape(pp_plan* pl, int variant, PcgShape* sh) {
   const int nl = tiles(CfgX::TX), ns = tiles(CfgS::TX);
   int per_cu = 0;
   const bool small_tiles = variant == PP_FLAG_SMALL_TILES;
+  if (variant == 3) {  // Chebyshev preconditioner: resident tiles only
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_cheb<CfgR>, CfgR::NT, 0));
+    if (tiles(CfgR::TX) > per_cu * cu)
+      return fail(PP_ERR_UNSUPPORTED,
+                  "the Chebyshev preconditioner needs every 18x18 tile resident at once");
+    sh->kind = 4;
+    sh->grid = tiles(CfgR::TX);
+    return PP_OK;
+  }
   if (variant == 0) {  // default: single-reduction kernel while it is resident
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_cg<CfgR>, CfgR::NT, 0));
     if (tiles(CfgR::TX) <= per_cu * cu) {
@@ -900,8 +1124,22 @@ int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxi
     hipLaunchKernelGGL((k_pcg_res<CfgS>), dim3(sh.grid), dim3(CfgS::NT), 0, s, A);
   else if (sh.kind == 2)
     hipLaunchKernelGGL((k_pcg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
-  else
+  else if (sh.kind == 3)
     hipLaunchKernelGGL((k_pcg_cg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
+  else {
+    // tag 0 never matches: stale words of an earlier launch cannot pass
+    PP_HIP(hipMemsetAsync(pl->p0.p, 0, pl->n_dof * sizeof(double), s));
+    PP_HIP(hipMemsetAsync(pl->p1.p, 0, pl->n_dof * sizeof(double), s));
+    // interval [lmax / kChebRatio, lmax], lmax with a 5% margin over the
+    // power-iteration estimate (q_d stays positive up to lmax)
+    constexpr double kChebRatio = 100.0;
+    const double hi = 1.05 * pl->cheb_lmax, lo = hi / kChebRatio;
+    ChebArgs Q;
+    Q.theta = 0.5 * (hi + lo);
+    Q.delta = 0.5 * (hi - lo);
+    Q.degree = pl->cheb_degree;
+    hipLaunchKernelGGL((k_pcg_cheb<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A, Q);
+  }
   PP_HIP(hipGetLastError());
   GridSync gsh;
   PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
@@ -992,14 +1230,54 @@ int struct_true_residual(pp_plan* pl, const double* dx, double* rel) {
 // true residual F - K x is recomputed in FP64; while it is above the
 // tolerance (the recurrence drifts from it by rounding) PCG restarts from x
 // with that residual, at most kMaxRestarts times.  All of it is timed.
+// lambda_max(B K) by 60 power iterations from a fixed pseudo-random start on
+// the free DOFs (pl->tmp, z and Ap as scratch); kept in the plan.
+int cheb_lambda_max(pp_plan* pl) {
+  const int64_t nd = pl->n_dof;
+  std::vector<double> h(nd);
+  uint64_t sd = 0x9E3779B97F4A7C15ull;
+  for (int64_t i = 0; i < nd; i++) {
+    sd = sd * 6364136223846793005ull + 1442695040888963407ull;
+    h[i] = pl->h_cmask[i / 3] >> (i % 3) & 1 ? 0.0 : (double)(sd >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  }
+  hipStream_t s = pl->ctx->stream;
+  PP_HIP(hipMemcpyAsync(pl->z.p, h.data(), nd * 8, hipMemcpyHostToDevice, s));
+  double lam = 0.0, nv = 0.0, unused = 0.0;
+  int rc;
+  if ((rc = launch_norms(pl, pl->z.p, pl->z.p, &unused, &nv))) return rc;
+  for (int it = 0; it < 60 && nv > 0; it++) {
+    if ((rc = struct_apply_k(pl, pl->z.p, pl->Ap.p)) || (rc = struct_apply_precond(pl, pl->Ap.p, pl->tmp.p)))
+      return rc;
+    double nb = 0.0;
+    if ((rc = launch_norms(pl, pl->tmp.p, pl->tmp.p, &unused, &nb))) return rc;
+    if (!(nb > 0)) break;
+    lam = nb / nv;
+    hipLaunchKernelGGL(k_scale_dof, dim3(grid_blocks(nd, 256)), dim3(256), 0, s, nd, pl->tmp.p, 1.0 / nb);
+    PP_HIP(hipGetLastError());
+    std::swap(pl->z.p, pl->tmp.p);
+    nv = 1.0;
+  }
+  if (!(lam > 0)) return fail(PP_ERR_INVALID, "Chebyshev preconditioner: no lambda_max estimate");
+  pl->cheb_lmax = lam;
+  return PP_OK;
+}
+
 int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   constexpr int kMaxRestarts = 4;
   hipStream_t s = pl->ctx->stream;
+  int rc;
+  // 0: single-reduction kernel; 1: two-reduction kernel; 2: two-reduction on
+  // small tiles; 3: single-reduction with the Chebyshev preconditioner
+  int dw = (o->flags & PP_FLAG_SMALL_TILES) ? 2 : (o->flags & PP_FLAG_CLASSIC_PCG) ? 1 : 0;
+  if (o->precond == PP_PRECOND_CHEBYSHEV) {
+    dw = 3;
+    pl->cheb_degree = o->poly_degree > 0 ? o->poly_degree : 8;
+    if (pl->cheb_degree > 32) return fail(PP_ERR_INVALID, "poly_degree must be at most 32");
+    if (pl->cheb_lmax <= 0 && (rc = cheb_lambda_max(pl))) return rc;
+  }
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   DevStats ds;
-  // 0: single-reduction kernel; 1: two-reduction kernel; 2: two-reduction on small tiles
-  const int dw = (o->flags & PP_FLAG_SMALL_TILES) ? 2 : (o->flags & PP_FLAG_CLASSIC_PCG) ? 1 : 0;
-  int rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, dw, &ds);
+  rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, dw, &ds);
   if (rc) return rc;
   const double bnorm = ds.bnorm;
   int total = ds.iterations;

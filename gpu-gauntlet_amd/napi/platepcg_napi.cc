@@ -297,6 +297,7 @@ napi_value Solve(napi_env env, napi_callback_info info) {
   w->opts.inner_tolerance = prop(env, o, "innerTolerance", 0.0);
   w->opts.max_refinements = (int32_t)prop(env, o, "maxRefinements", 0);
   w->opts.flags = (int32_t)prop(env, o, "flags", 0);
+  w->opts.poly_degree = (int32_t)prop(env, o, "polyDegree", 0);
   w->x32.resize(w->n);
   w->x64.resize(w->n);
   NAPI_CALL(env, napi_create_reference(env, argv[1], 1, &w->fref));  // keep F alive

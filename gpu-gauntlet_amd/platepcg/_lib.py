@@ -20,7 +20,7 @@ PP_ERR_NODEVICE = -6
 PP_FIX_W, PP_FIX_THX, PP_FIX_THY = 1, 2, 4
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1
+PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_CHEBYSHEV = 0, 1, 2
 PATH_AUTO, PATH_STRUCTURED, PATH_COLOURED = 0, 1, 2
 MAX_SWEEPS = 32
 FLAG_NO_TRUE_RESIDUAL = 1
@@ -49,7 +49,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32),
                 ("precision", C.c_int32), ("precond", C.c_int32), ("path", C.c_int32),
                 ("inner_tolerance", C.c_double), ("max_refinements", C.c_int32),
-                ("flags", C.c_int32)]
+                ("flags", C.c_int32), ("poly_degree", C.c_int32)]
 
 
 class SolveStats(C.Structure):

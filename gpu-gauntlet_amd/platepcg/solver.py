@@ -139,9 +139,9 @@ class Plan:
     @staticmethod
     def options(tolerance=1e-8, max_iterations=20000, precision=_lib.PRECISION_FP64,
                 precond=_lib.PRECOND_BLOCK_JACOBI, path=_lib.PATH_AUTO, inner_tolerance=0.0,
-                max_refinements=0, flags=0):
+                max_refinements=0, flags=0, poly_degree=0):
         return _lib.SolveOpts(tolerance, max_iterations, precision, precond, path,
-                              inner_tolerance, max_refinements, flags)
+                              inner_tolerance, max_refinements, flags, poly_degree)
 
     def set_rhs(self, F):
         F = np.ascontiguousarray(F, np.float32)

@@ -68,6 +68,11 @@ typedef struct {
 
 #define PP_PRECOND_BLOCK_JACOBI 0 /* 3x3 nodal block Jacobi (solver.ts:188-339) */
 #define PP_PRECOND_LEARNED 1      /* CNN preconditioner, flexible PCG */
+#define PP_PRECOND_CHEBYSHEV 2    /* block Jacobi inside a Chebyshev polynomial of degree
+                                     poly_degree in B K (no reference counterpart): about
+                                     poly_degree times fewer PCG iterations and all-reduces for
+                                     about the same number of K applies; structured FP64 plates
+                                     whose tiles are co-resident (up to about 1,000^2 nodes) */
 
 #define PP_PATH_AUTO 0       /* structured kernels when the mesh is a grid */
 #define PP_PATH_STRUCTURED 1 /* tensor-grid tile kernels (error if not a grid) */
@@ -85,6 +90,7 @@ typedef struct {
                               fallen by this factor (<= 0: 1e-3) */
   int32_t max_refinements; /* mixed: most FP64 refinement sweeps (<= 0: 200) */
   int32_t flags;           /* PP_FLAG_* */
+  int32_t poly_degree;     /* PP_PRECOND_CHEBYSHEV: degree (<= 0: 8, at most 32) */
 } pp_solve_opts;
 
 /* Skip the FP64 |F - K x| / |F| recomputation after the solve
