@@ -397,6 +397,24 @@ def main():
         basis = "iterations of FP64 block-Jacobi PCG to 1e-8 (extrapolation)"
     elif a.workload == "c2":
         full = int(np.load(os.path.join(ROOT, "tests", "golden", "c2_142.npz"))["it_js"])
+    if not mixed:
+        # the same solve with block Jacobi inside a degree-8 Chebyshev polynomial
+        # (PP_PRECOND_CHEBYSHEV): a reported alternative, not the value above
+        try:
+            oc = plan.options(tolerance=1e-8, max_iterations=400000,
+                              precond=_lib.PRECOND_CHEBYSHEV, poly_degree=8)
+            for _ in range(max(1, a.warmup)):
+                plan.solve_resident(oc)
+            t3, dc = time.perf_counter(), []
+            for _ in range(a.steps):
+                sc = plan.solve_resident(oc)
+                dc.append(sc.device_ms)
+            out["chebyshev8"] = {"ms": (time.perf_counter() - t3) * 1e3 / a.steps,
+                                 "device_ms": float(np.mean(dc)), "iterations": sc.iterations,
+                                 "true_rel_residual": sc.true_rel_residual,
+                                 "converged": bool(sc.converged)}
+        except Exception as e:  # plates too large for the resident kernel
+            out["chebyshev8"] = {"error": str(e)}
     if a.cpu_sample_iters > 0:
         try:
             out["cpu_baseline"] = cpu_baseline(w, F, full, a.cpu_sample_iters, basis)

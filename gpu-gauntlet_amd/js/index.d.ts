@@ -34,7 +34,11 @@ export interface GPUSolverOptions {
   /** Rejected: this library has no CPU solver. */
   forceCPU?: boolean;
   precision?: 'fp64' | 'mixed';
-  preconditioner?: 'blockJacobi' | 'learned';
+  /** chebyshev: block Jacobi inside a Chebyshev polynomial of degree polyDegree in B K
+   *  (structured FP64 path, plates of up to 512 tiles of 18 x 18 nodes) */
+  preconditioner?: 'blockJacobi' | 'learned' | 'chebyshev';
+  /** chebyshev: polynomial degree (default 8, at most 32) */
+  polyDegree?: number;
   path?: 'auto' | 'structured' | 'coloured';
   /** mixed: a refinement sweep ends when the FP32 residual has fallen by this factor (default 1e-3) */
   innerTolerance?: number;

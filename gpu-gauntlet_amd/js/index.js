@@ -17,7 +17,7 @@
 const path = require('path');
 
 const PRECISION = { fp64: 0, mixed: 1 };
-const PRECOND = { blockJacobi: 0, learned: 1 };
+const PRECOND = { blockJacobi: 0, learned: 1, chebyshev: 2 };
 const PATH = { auto: 0, structured: 1, coloured: 2 };
 const FIX = { w: 1, thx: 2, thy: 4 };
 
@@ -111,7 +111,7 @@ class PlateSolver {
       tolerance: o.tolerance, maxIterations: o.maxIterations,
       precision: PRECISION[o.precision || 'fp64'], precond: PRECOND[o.preconditioner || 'blockJacobi'],
       path: PATH[o.path || 'auto'], innerTolerance: o.innerTolerance || 0,
-      maxRefinements: o.maxRefinements || 0,
+      maxRefinements: o.maxRefinements || 0, polyDegree: o.polyDegree || 0,
     });
   }
 

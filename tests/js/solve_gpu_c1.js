@@ -24,6 +24,9 @@ async function main() {
     if (Math.abs(cpu.solution[3 * n]) > Math.abs(cpu.solution[3 * ci])) ci = n;
   }
   const rel = Math.abs(gpu.solution[3 * gi] - cpu.solution[3 * ci]) / Math.abs(cpu.solution[3 * ci]);
+  const cheb = await host.solveGPU(mesh, material, coloring, F, constrained,
+    { tolerance: 1e-8, maxIterations: 20000, preconditioner: 'chebyshev', polyDegree: 6 });
+  const relCheb = Math.abs(cheb.solution[3 * gi] - cpu.solution[3 * ci]) / Math.abs(cpu.solution[3 * ci]);
   let forceCpuRejected = false;
   try {
     await host.solveGPU(mesh, material, coloring, F, constrained, { forceCPU: true });
@@ -32,7 +35,9 @@ async function main() {
   console.log(JSON.stringify({
     usedGPU: gpu.usedGPU, converged: gpu.converged, iterations: gpu.iterations,
     cpuIterations: cpu.iterations, trueRelResidual: gpu.trueRelResidual, maxWRelError: rel,
-    forceCpuRejected: forceCpuRejected,
+    forceCpuRejected: forceCpuRejected, chebConverged: cheb.converged,
+    chebIterations: cheb.iterations, chebTrueRelResidual: cheb.trueRelResidual,
+    chebMaxWRelError: relCheb,
   }));
 }
 

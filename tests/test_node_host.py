@@ -49,6 +49,9 @@ def test_solve_gpu_through_node():
     assert r["usedGPU"] and r["converged"] and r["forceCpuRejected"]
     assert r["trueRelResidual"] <= 1e-8
     assert r["maxWRelError"] < 1e-4
+    # preconditioner: 'chebyshev' through the same host call
+    assert r["chebConverged"] and r["chebTrueRelResidual"] <= 1e-8
+    assert r["chebMaxWRelError"] < 1e-4 and r["chebIterations"] < r["iterations"] / 3
 
 
 @pytest.mark.gpu
