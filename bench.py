@@ -229,7 +229,10 @@ def bench_learned(a):
             dev.append(st.device_ms)
         res[tag] = {"ms": (time.perf_counter() - t1) * 1e3 / a.steps, "device_ms": float(np.mean(dev)),
                     "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
-                    "converged": bool(st.converged)}
+                    "converged": bool(st.converged), "breakdown": bool(st.breakdown),
+                    # FP64 residual-replacement restarts: iterations and true residual of each
+                    "restarts": [[st.sweep_iterations[i], st.sweep_residual[i]]
+                                 for i in range(min(st.sweeps, _lib.MAX_SWEEPS))]}
         print(f"{tag}: {st.iterations} iterations, {res[tag]['ms']:.0f} ms", file=sys.stderr,
               flush=True)
         res[tag]["x"] = plan.solution()[0]

@@ -40,6 +40,7 @@ struct MgLevel {
   DevBuf<float> minv;    // 6 per node (symmetric 3x3 block inverse, FP32 for smoothing)
   DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
   DevBuf<double> b, x, x2, t;  // right-hand side, iterate and its sweep partner, work
+  DevBuf<double> dz;            // Chebyshev smoothing: the last step's increment
 };
 
 struct Mg {
@@ -53,12 +54,13 @@ struct Mg {
   int nd_coarse = 0;
   DevBuf<double> cinv;  // dense inverse of the coarsest operator [nd][nd]
   DevBuf<double> dotp;
-  int setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps);
+  bool jacobi = false;  // smoothing by damped block Jacobi instead of Chebyshev steps
+  int setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps, bool damped_jacobi);
   int vcycle(const double* b, double* x);  // x = V b on level 0 (device FP64 DOF vectors)
   int apply_level(int l, const double* x, double* y);  // y = A_l x
  private:
   int cycle(int l);
-  int sweep(int l, int mode, const double* x, double* xo);
+  int sweep(int l, int mode, const double* x, double* xo, double c1, double c2, double* dz);
   int estimate_omega(int l);
 };
 

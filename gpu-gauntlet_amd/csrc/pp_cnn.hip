@@ -1035,7 +1035,7 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   else if ((rc = estimate_omega())) return rc;
   if (cfg && cfg->multigrid) {
     mg = new Mg();
-    if ((rc = mg->setup(pl, omega, cfg->mg_sweeps, cfg->mg_coarse_sweeps)) ||
+    if ((rc = mg->setup(pl, omega, cfg->mg_sweeps, cfg->mg_coarse_sweeps, cfg->mg_jacobi != 0)) ||
         (rc = mgz.alloc(pl->n_dof)))
       return rc;
   }

@@ -200,11 +200,12 @@ __global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, dou
   for (int d = 0; d < 3; d++) y[3 * n + d] = v[d];
 }
 
-// Fused sweep on a stencil level: MODE 1: xo = x + w B (b - A x) (xo must
-// not alias x); MODE 2: xo = b - A x.
+// Fused sweep on a stencil level: MODE 1: d' = c1 d + c2 B (b - A x),
+// xo = x + d' (damped block Jacobi when c1 = 0 and d is null, a Chebyshev
+// step otherwise, d kept in place; xo must not alias x); MODE 2: xo = b - A x.
 template <int MODE>
-__global__ void k_mg_sweep(int nx, int ny, const float* st, const float* minv, double omega,
-                           const double* x, const double* b, double* xo) {
+__global__ void k_mg_sweep(int nx, int ny, const float* st, const float* minv, double c1, double c2,
+                           double* dz, const double* x, const double* b, double* xo) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
   double v[3], r[3];
@@ -216,12 +217,17 @@ __global__ void k_mg_sweep(int nx, int ny, const float* st, const float* minv, d
   }
   double z[3];
   sym3_apply(minv + 6 * n, r, z);
-  for (int d = 0; d < 3; d++) xo[3 * n + d] = x[3 * n + d] + omega * z[d];
+  for (int d = 0; d < 3; d++) {
+    const double dn = (dz ? c1 * dz[3 * n + d] : 0.0) + c2 * z[d];
+    if (dz) dz[3 * n + d] = dn;
+    xo[3 * n + d] = x[3 * n + d] + dn;
+  }
 }
 
-// mode 0: x = w B b;  mode 1: x += w B (b - t);  mode 2: t = b - t
+// mode 0: x = w B b (and dz = x when dz is given);  mode 1: x += w B (b - t);
+// mode 2: t = b - t
 __global__ void k_mg_point(int64_t nn, int mode, const float* minv, double omega, const double* b,
-                           double* t, double* x) {
+                           double* t, double* x, double* dz) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
   double r[3], z[3];
@@ -231,7 +237,11 @@ __global__ void k_mg_point(int64_t nn, int mode, const float* minv, double omega
     return;
   }
   sym3_apply(minv + 6 * n, r, z);
-  for (int d = 0; d < 3; d++) x[3 * n + d] = (mode == 0 ? 0.0 : x[3 * n + d]) + omega * z[d];
+  for (int d = 0; d < 3; d++) {
+    const double v = (mode == 0 ? 0.0 : x[3 * n + d]) + omega * z[d];
+    x[3 * n + d] = v;
+    if (dz) dz[3 * n + d] = v;
+  }
 }
 
 // Interpolation weight of level-l index f from coarse index c (|f - 2c| <= 1).
@@ -369,10 +379,13 @@ bool spd_inverse(std::vector<double>& A, int n) {
 
 }  // namespace
 
-int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps) {
+int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps, bool damped_jacobi) {
   pl = plan;
   stream = plan->ctx->stream;
-  nu = sweeps > 0 ? sweeps : 3;
+  jacobi = damped_jacobi;
+  // measured at 1024^2: 6 Chebyshev steps (105 flexible-PCG iterations) beat 4 (164) and 8
+  // (99, each dearer); damped Jacobi is best at 3
+  nu = sweeps > 0 ? sweeps : (jacobi ? 3 : 6);
   nuc = coarse_sweeps > 0 ? coarse_sweeps : 1;
   if (nu > 8 || nuc > 8) return fail(PP_ERR_INVALID, "mg_sweeps must be at most 8");
   // hierarchy shape
@@ -399,6 +412,7 @@ int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps) {
     L.nn = (int64_t)(L.nx + 1) * (L.ny + 1);
     if ((rc = L.t.alloc(3 * L.nn))) return rc;
     if (l < nl - 1 && ((rc = L.x2.alloc(3 * L.nn)) || (rc = L.minv.alloc(6 * L.nn)))) return rc;
+    if (l < nl - 1 && !jacobi && (rc = L.dz.alloc(3 * L.nn))) return rc;
     if (l > 0 && ((rc = L.b.alloc(3 * L.nn)) || (rc = L.x.alloc(3 * L.nn)))) return rc;
     if (l > 0 && l < nl - 1 &&
         ((rc = L.st.alloc(45 * L.nn)) || (rc = L.dead.alloc(L.nn))))
@@ -492,7 +506,8 @@ int Mg::estimate_omega(int l) {
     int rc;
     if ((rc = apply_level(l, L.x.p, L.t.p))) return rc;
     hipLaunchKernelGGL(k_mg_point, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nn, 0,
-                       (const float*)L.minv.p, 1.0, (const double*)L.t.p, (double*)nullptr, L.b.p);
+                       (const float*)L.minv.p, 1.0, (const double*)L.t.p, (double*)nullptr, L.b.p,
+                       (double*)nullptr);
     hipLaunchKernelGGL(k_mg_dot2, dim3(kMgDotBlocks), dim3(256), 0, stream, nd,
                        (const double*)L.b.p, (const double*)L.x.p, dotp.p);
     double hp[2 * kMgDotBlocks];
@@ -514,23 +529,39 @@ int Mg::estimate_omega(int l) {
   return PP_OK;
 }
 
-int Mg::sweep(int l, int mode, const double* x, double* xo) {
+// Chebyshev smoothing interval of a level: [hi / kChebSmoothRatio, hi],
+// hi = 1.05 lambda_max(B A), lambda_max recovered from the level's weight
+// w = 4 / (3 * 1.05 lambda_max).
+constexpr double kChebSmoothRatio = 16.0;
+static void cheb_interval(double omega, double& theta, double& delta) {
+  const double hi = 1.05 * 4.0 / (3.0 * 1.05 * omega), lo = hi / kChebSmoothRatio;
+  theta = 0.5 * (hi + lo);
+  delta = 0.5 * (hi - lo);
+}
+
+int Mg::sweep(int l, int mode, const double* x, double* xo, double c1, double c2, double* dz) {
   MgLevel& L = lv[l];
-  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, (const float*)L.minv.p, L.omega, xo);
+  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, (const float*)L.minv.p, c1, c2, dz, xo);
   if (mode == 1)
     hipLaunchKernelGGL(k_mg_sweep<1>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
-                       (const float*)L.st.p, (const float*)L.minv.p, L.omega, x,
+                       (const float*)L.st.p, (const float*)L.minv.p, c1, c2, dz, x,
                        (const double*)L.b.p, xo);
   else
     hipLaunchKernelGGL(k_mg_sweep<2>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
-                       (const float*)L.st.p, (const float*)L.minv.p, L.omega, x,
+                       (const float*)L.st.p, (const float*)L.minv.p, c1, c2, dz, x,
                        (const double*)L.b.p, xo);
   PP_HIP(hipGetLastError());
   return PP_OK;
 }
 
-// The sweeps ping-pong between L.x and L.x2: 2 ns - 1 of them follow the
-// first (x = w B b), an odd count, so starting in x2 the cycle ends in L.x.
+// Smoothing: ns damped block-Jacobi sweeps (x += w B (b - A x)), or, by
+// default, ns steps of the Chebyshev semi-iteration for B A x = B b on
+// [hi / 16, hi] (the first step d = B (b - A x) / theta, then
+// d = rho_k rho_{k-1} d + (2 rho_k / delta) B (b - A x), rho_k = 1 / (2 sigma
+// - rho_{k-1}), rho_0 = 1 / sigma; x += d): the same polynomial before and
+// after the coarse correction keeps the cycle symmetric.  The sweeps
+// ping-pong between L.x and L.x2: 2 ns - 1 of them follow the first (from
+// x = 0), an odd count, so starting in x2 the cycle ends in L.x.
 int Mg::cycle(int l) {
   MgLevel& L = lv[l];
   if (l == nl - 1) {
@@ -544,15 +575,38 @@ int Mg::cycle(int l) {
   const int g = mg_blocks(L.nn);
   int rc;
   const int ns = l == 0 ? nu : nuc;
+  double theta = 1.0, delta = 1.0;
+  if (!jacobi) cheb_interval(L.omega, theta, delta);
+  const double sigma = theta / delta;
+  double* dz = jacobi ? nullptr : L.dz.p;
   double* cur = L.x2.p;
   auto other = [&](double* p) { return p == L.x.p ? L.x2.p : L.x.p; };
-  hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 0, B, L.omega,
-                     (const double*)L.b.p, (double*)nullptr, cur);
-  for (int s = 1; s < ns; s++) {
-    if ((rc = sweep(l, 1, cur, other(cur)))) return rc;
+  // step k of a smoothing pass (k = 0 starts it from the current x)
+  double rho = 0.0;
+  auto step = [&](int k) -> int {
+    double c1 = 0.0, c2 = L.omega;
+    if (!jacobi) {
+      if (k == 0) {
+        rho = 1.0 / sigma;
+        c2 = 1.0 / theta;
+      } else {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c1 = rn * rho;
+        c2 = 2.0 * rn / delta;
+        rho = rn;
+      }
+    }
+    const int r = sweep(l, 1, cur, other(cur), c1, c2, dz);
     cur = other(cur);
-  }
-  if ((rc = sweep(l, 2, cur, L.t.p))) return rc;  // residual
+    return r;
+  };
+  // pre-smoothing from x = 0: the first step is pointwise
+  rho = 1.0 / sigma;
+  hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 0, B,
+                     jacobi ? L.omega : 1.0 / theta, (const double*)L.b.p, (double*)nullptr, cur, dz);
+  for (int s = 1; s < ns; s++)
+    if ((rc = step(s))) return rc;
+  if ((rc = sweep(l, 2, cur, L.t.p, 0.0, 0.0, nullptr))) return rc;  // residual
   MgLevel& C = lv[l + 1];
   hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nn)), dim3(256), 0, stream, L.nx, L.ny, mask,
                      (const double*)L.t.p, C.nx, C.ny, C.b.p);
@@ -560,10 +614,8 @@ int Mg::cycle(int l) {
   if ((rc = cycle(l + 1))) return rc;
   hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, stream, L.nx, L.ny, mask,
                      (const double*)C.x.p, C.nx, cur);
-  for (int s = 0; s < ns; s++) {
-    if ((rc = sweep(l, 1, cur, other(cur)))) return rc;
-    cur = other(cur);
-  }
+  for (int s = 0; s < ns; s++)
+    if ((rc = step(s))) return rc;
   PP_HIP(hipGetLastError());
   return PP_OK;
 }

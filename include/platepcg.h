@@ -243,14 +243,15 @@ typedef struct {
                                   30,000 iterations) where FCG(8) converges (29,897) */
   int32_t multigrid;           /* 0: the smoother S around the network is w B (block Jacobi);
                                   1: S is a Galerkin geometric-multigrid V-cycle (bilinear
-                                  prolongation, damped block-Jacobi sweeps per level, dense
+                                  prolongation, Chebyshev or damped block-Jacobi smoothing, dense
                                   coarsest solve).  Needs the element counts per side to halve
                                   down to a coarsest grid of at most 2,200 DOFs */
-  int32_t mg_sweeps;           /* block-Jacobi sweeps before and after each coarse correction
-                                  on the finest level; 0 = 3 */
-  int32_t mg_coarse_sweeps;    /* the same on the coarser stencil levels; 0 = 1 (1024^2:
-                                  3/1 sweeps converge in 218 iterations, 0.37 s; 2/2 in 217,
-                                  0.40 s; 1/1 in 331, 0.50 s) */
+  int32_t mg_sweeps;           /* smoothing steps before and after each coarse correction on
+                                  the finest level, at most 8; 0 = 6 (Chebyshev) or 3 (Jacobi) */
+  int32_t mg_coarse_sweeps;    /* the same on the coarser stencil levels; 0 = 1 */
+  int32_t mg_jacobi;           /* smoothing steps: 0 = Chebyshev semi-iteration for B A on
+                                  [lambda_max / 16, lambda_max] (the default), 1 = damped block
+                                  Jacobi w B, w = 4 / (3 lambda_max) */
 } pp_cnn_config;
 
 #define PP_TRAIN_CURVE 64

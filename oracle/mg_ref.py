@@ -16,8 +16,10 @@ its level.  The coarsest level is solved exactly.  The cycle, with nu sweeps
 of damped block Jacobi (w_l B_l, B_l the inverse of the 3x3 diagonal blocks)
 before and after the coarse correction, is the library's:
 
-  x = w B b;  nu - 1 times x += w B (b - A x);  b_c = P^T (b - A x);
-  x += P V_{l+1}(b_c);  nu times x += w B (b - A x).
+  x = S_nu(b, 0);  b_c = P^T (b - A x);  x += P V_{l+1}(b_c);  x = S_nu(b, x),
+
+S_nu nu damped block-Jacobi sweeps x += w B (b - A x) or nu Chebyshev steps
+(smooth below).
 
 Coarsening rule (the library's): halve while both element counts are even,
 stopping at the first level (below 0) with at most 1,200 DOFs.
@@ -110,9 +112,10 @@ class MgOracle:
     factor the coarsest level for the cycle."""
 
     def __init__(self, A0, nx, ny, keep0, sweeps=2, omegas=None, blk0=None, factor=True,
-                 coarse_sweeps=None):
+                 coarse_sweeps=None, chebyshev=False):
         self.nu = sweeps  # on level 0; coarse_sweeps (None: the same) on the others
         self.nuc = sweeps if coarse_sweeps is None else coarse_sweeps
+        self.chebyshev = chebyshev
         self.dims = hierarchy_dims(nx, ny)
         self.A, self.P, self.B, self.keep = [A0], [], [], [keep0]
         for l in range(len(self.dims) - 1):
@@ -145,16 +148,35 @@ class MgOracle:
         op = LinearOperator((n, n), matvec=lambda v: self.bj(l, self.A[l] @ v))
         return float(np.max(np.real(eigs(op, k=1, which="LR", tol=1e-8)[0])))
 
+    def smooth(self, l, b, x, nu):
+        """nu damped block-Jacobi sweeps, or nu steps of the Chebyshev
+        semi-iteration for B A x = B b on [hi / 16, hi], hi = 1.05 lambda_max
+        with lambda_max = 4 / (3 * 1.05 w) from the level's weight w."""
+        A, w = self.A[l], self.omega[l]
+        if not self.chebyshev:
+            for _ in range(nu):
+                x = x + w * self.bj(l, b - A @ x)
+            return x
+        hi = 1.05 * 4.0 / (3.0 * 1.05 * w)
+        lo = hi / 16.0
+        theta, delta = (hi + lo) / 2, (hi - lo) / 2
+        sigma = theta / delta
+        rho = 1.0 / sigma
+        d = self.bj(l, b - A @ x) / theta
+        x = x + d
+        for _ in range(nu - 1):
+            rn = 1.0 / (2.0 * sigma - rho)
+            d = rn * rho * d + 2.0 * rn / delta * self.bj(l, b - A @ x)
+            rho = rn
+            x = x + d
+        return x
+
     def cycle(self, b, l=0):
         if l == len(self.A) - 1:
             return sla.cho_solve(self.coarse, b)
-        A, w = self.A[l], self.omega[l]
+        A = self.A[l]
         nu = self.nu if l == 0 else self.nuc
-        x = w * self.bj(l, b)
-        for _ in range(nu - 1):
-            x = x + w * self.bj(l, b - A @ x)
+        x = self.smooth(l, b, np.zeros_like(b), nu)
         bc = self.P[l].T @ (b - A @ x)
         x = x + self.P[l] @ self.cycle(bc, l + 1)
-        for _ in range(nu):
-            x = x + w * self.bj(l, b - A @ x)
-        return x
+        return self.smooth(l, b, x, nu)

@@ -31,7 +31,7 @@ def ctx():
     c.close()
 
 
-def setup(ctx, nx, ny=None, var_t=True, seed=3, sweeps=0, coarse_sweeps=0):
+def setup(ctx, nx, ny=None, var_t=True, seed=3, sweeps=0, coarse_sweeps=0, jacobi=False):
     ny = nx if ny is None else ny
     mesh = P.rect_mesh(nx, ny)
     mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
@@ -41,13 +41,15 @@ def setup(ctx, nx, ny=None, var_t=True, seed=3, sweeps=0, coarse_sweeps=0):
         et = (0.1 * np.exp(0.4 * rng.standard_normal(nx * ny))).astype(np.float32)
     plan = Plan(ctx, mesh, E, NU, 0.1, mask, element_t=et)
     op = O.OracleProblem(mesh, E, NU, 0.1, mask, element_t=et)
-    net = Net(plan, seed=1, multigrid=True, mg_sweeps=sweeps, mg_coarse_sweeps=coarse_sweeps)
+    net = Net(plan, seed=1, multigrid=True, mg_sweeps=sweeps, mg_coarse_sweeps=coarse_sweeps,
+              mg_jacobi=jacobi)
     lv = net.mg_levels()
     K = M.assemble_q4(mesh["nodes"], nx, ny, E, NU, 0.1, et, O.ke_q4)
     free = (((mask[:, None] >> np.arange(3)) & 1) == 0).ravel()
     A0 = M.with_bc(K, free)
-    ref = M.MgOracle(A0, nx, ny, free, sweeps=sweeps or 3, omegas=[w for _, _, w in lv[:-1]],
-                     blk0=op.blk, coarse_sweeps=coarse_sweeps or 1)
+    ref = M.MgOracle(A0, nx, ny, free, sweeps=sweeps or (3 if jacobi else 6),
+                     omegas=[w for _, _, w in lv[:-1]], blk0=op.blk,
+                     coarse_sweeps=coarse_sweeps or 1, chebyshev=not jacobi)
     return dict(mesh=mesh, mask=mask, et=et, plan=plan, op=op, net=net, lv=lv, ref=ref, free=free)
 
 
@@ -80,8 +82,8 @@ def test_levels_operators_and_vcycle_match_oracle(ctx, nx, ny, var_t):
 
 
 def test_sweep_count_and_unsupported_grids(ctx):
-    # 2 sweeps on every level, 64^2 (3 levels)
-    t = setup(ctx, 64, sweeps=2, coarse_sweeps=2)
+    # 2 damped block-Jacobi sweeps on every level, 64^2 (3 levels)
+    t = setup(ctx, 64, sweeps=2, coarse_sweeps=2, jacobi=True)
     b = np.random.default_rng(5).standard_normal(t["ref"].A[0].shape[0]) * t["free"]
     assert rel_err(t["net"].mg_cycle(b), t["ref"].cycle(b)) <= 2e-5
     t["net"].close()
@@ -221,8 +223,8 @@ def test_partial_constraints_and_graded_grid(ctx, case):
     lv = net.mg_levels()
     K = M.assemble_q4(mesh["nodes"], n, n, E, NU, 0.1, None, O.ke_q4)
     free = (((mask[:, None] >> np.arange(3)) & 1) == 0).ravel()
-    ref = M.MgOracle(M.with_bc(K, free), n, n, free, sweeps=3, coarse_sweeps=1,
-                     omegas=[w for _, _, w in lv[:-1]], blk0=op.blk)
+    ref = M.MgOracle(M.with_bc(K, free), n, n, free, sweeps=6, coarse_sweeps=1,
+                     omegas=[w for _, _, w in lv[:-1]], blk0=op.blk, chebyshev=True)
     rng = np.random.default_rng(11)
     for l in range(len(lv) - 1):
         x = rng.standard_normal(ref.A[l].shape[0]) * (free if l == 0 else 1.0)
