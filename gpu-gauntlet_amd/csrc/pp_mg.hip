@@ -203,8 +203,10 @@ __global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, dou
 // Fused sweep on a stencil level: MODE 1: d' = c1 d + c2 B (b - A x),
 // xo = x + d' (damped block Jacobi when c1 = 0 and d is null, a Chebyshev
 // step otherwise, d kept in place; xo must not alias x); MODE 2: xo = b - A x.
+// Launched with 256 threads: the bound lets MODE 1 keep its stencil loads in
+// flight without spilling (under the 1,024-thread default it needs scratch).
 template <int MODE>
-__global__ void k_mg_sweep(int nx, int ny, const float* st, const float* minv, double c1, double c2,
+__global__ __launch_bounds__(256) void k_mg_sweep(int nx, int ny, const float* st, const float* minv, double c1, double c2,
                            double* dz, const double* x, const double* b, double* xo) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
