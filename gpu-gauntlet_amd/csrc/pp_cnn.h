@@ -40,13 +40,12 @@ struct MgLevel {
   DevBuf<float> minv;    // 6 per node (symmetric 3x3 block inverse, FP32 for smoothing)
   DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
   DevBuf<double> b, x, x2, t;  // right-hand side, iterate and its sweep partner, work
-  DevBuf<double> dz;            // Chebyshev smoothing: the last step's increment
 };
 
 struct Mg {
   pp_plan* pl = nullptr;
   hipStream_t stream = nullptr;
-  int nu = 3;   // block-Jacobi sweeps before and after each coarse correction, level 0
+  int nu = 6;   // smoothing steps before and after each coarse correction, level 0
   int nuc = 1;  // the same on levels >= 1
   static constexpr int kMaxLevels = 16;
   MgLevel lv[kMaxLevels];
@@ -60,7 +59,7 @@ struct Mg {
   int apply_level(int l, const double* x, double* y);  // y = A_l x
  private:
   int cycle(int l);
-  int sweep(int l, int mode, const double* x, double* xo, double c1, double c2, double* dz);
+  int sweep(int l, int mode, const double* x, double* xo, double c1, double c2);
   int estimate_omega(int l);
 };
 

@@ -152,11 +152,11 @@ namespace pp {
 // structured kernels (pp_struct.hip)
 int struct_setup(pp_plan* pl);
 int struct_apply_k(pp_plan* pl, const double* dx, double* dy);
-// mode 1: d' = c1 d + c2 B (b - K x), xo = x + d' (B: 6 floats per node, a
-// copy of the plan's block-Jacobi inverse; d may be null when c1 = 0, else
-// updated in place); mode 2: xo = b - K x.  xo must not alias x.
+// mode 1: xo = x + c1 (x - xo) + c2 B (b - K x) (B: 6 floats per node, a copy
+// of the plan's block-Jacobi inverse; xo, the previous iterate, is read only
+// when c1 != 0); mode 2: xo = b - K x.  xo must not alias x.
 int struct_sweep(pp_plan* pl, int mode, const double* x, const double* b, const float* minv,
-                 double c1, double c2, double* dz, double* xo);
+                 double c1, double c2, double* xo);
 int struct_apply_precond(pp_plan* pl, const double* dr, double* dz);
 int struct_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF);
 int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);

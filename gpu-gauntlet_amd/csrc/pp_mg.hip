@@ -200,14 +200,16 @@ __global__ void k_mg_apply(int nx, int ny, const float* st, const double* x, dou
   for (int d = 0; d < 3; d++) y[3 * n + d] = v[d];
 }
 
-// Fused sweep on a stencil level: MODE 1: d' = c1 d + c2 B (b - A x),
-// xo = x + d' (damped block Jacobi when c1 = 0 and d is null, a Chebyshev
-// step otherwise, d kept in place; xo must not alias x); MODE 2: xo = b - A x.
-// Launched with 256 threads: the bound lets MODE 1 keep its stencil loads in
-// flight without spilling (under the 1,024-thread default it needs scratch).
+// Fused sweep on a stencil level: MODE 1: xo = x + c1 (x - xo) + c2 B (b - A x)
+// (damped block Jacobi when c1 = 0, xo then not read; a Chebyshev step
+// otherwise, xo holding the previous iterate; xo must not alias x); MODE 2:
+// xo = b - A x.  Launched with 256 threads: the bound lets MODE 1 keep its
+// stencil loads in flight without spilling (under the 1,024-thread default it
+// needs scratch).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_mg_sweep(int nx, int ny, const float* st, const float* minv, double c1, double c2,
-                           double* dz, const double* x, const double* b, double* xo) {
+__global__ __launch_bounds__(256) void k_mg_sweep(int nx, int ny, const float* st,
+                                                  const float* minv, double c1, double c2,
+                                                  const double* x, const double* b, double* xo) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)(nx + 1) * (ny + 1)) return;
   double v[3], r[3];
@@ -220,16 +222,16 @@ __global__ __launch_bounds__(256) void k_mg_sweep(int nx, int ny, const float* s
   double z[3];
   sym3_apply(minv + 6 * n, r, z);
   for (int d = 0; d < 3; d++) {
-    const double dn = (dz ? c1 * dz[3 * n + d] : 0.0) + c2 * z[d];
-    if (dz) dz[3 * n + d] = dn;
-    xo[3 * n + d] = x[3 * n + d] + dn;
+    const double xc = x[3 * n + d];
+    const double dn = (c1 != 0.0 ? c1 * (xc - xo[3 * n + d]) : 0.0) + c2 * z[d];
+    xo[3 * n + d] = xc + dn;
   }
 }
 
-// mode 0: x = w B b (and dz = x when dz is given);  mode 1: x += w B (b - t);
-// mode 2: t = b - t
+// mode 0: x = w B b (and x0 = 0 when x0 is given: the iterate before it, for
+// the Chebyshev step that follows);  mode 1: x += w B (b - t);  mode 2: t = b - t
 __global__ void k_mg_point(int64_t nn, int mode, const float* minv, double omega, const double* b,
-                           double* t, double* x, double* dz) {
+                           double* t, double* x, double* x0) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
   double r[3], z[3];
@@ -242,7 +244,7 @@ __global__ void k_mg_point(int64_t nn, int mode, const float* minv, double omega
   for (int d = 0; d < 3; d++) {
     const double v = (mode == 0 ? 0.0 : x[3 * n + d]) + omega * z[d];
     x[3 * n + d] = v;
-    if (dz) dz[3 * n + d] = v;
+    if (x0) x0[3 * n + d] = 0.0;
   }
 }
 
@@ -414,7 +416,6 @@ int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps, bool 
     L.nn = (int64_t)(L.nx + 1) * (L.ny + 1);
     if ((rc = L.t.alloc(3 * L.nn))) return rc;
     if (l < nl - 1 && ((rc = L.x2.alloc(3 * L.nn)) || (rc = L.minv.alloc(6 * L.nn)))) return rc;
-    if (l < nl - 1 && !jacobi && (rc = L.dz.alloc(3 * L.nn))) return rc;
     if (l > 0 && ((rc = L.b.alloc(3 * L.nn)) || (rc = L.x.alloc(3 * L.nn)))) return rc;
     if (l > 0 && l < nl - 1 &&
         ((rc = L.st.alloc(45 * L.nn)) || (rc = L.dead.alloc(L.nn))))
@@ -541,16 +542,16 @@ static void cheb_interval(double omega, double& theta, double& delta) {
   delta = 0.5 * (hi - lo);
 }
 
-int Mg::sweep(int l, int mode, const double* x, double* xo, double c1, double c2, double* dz) {
+int Mg::sweep(int l, int mode, const double* x, double* xo, double c1, double c2) {
   MgLevel& L = lv[l];
-  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, (const float*)L.minv.p, c1, c2, dz, xo);
+  if (l == 0) return struct_sweep(pl, mode, x, L.b.p, (const float*)L.minv.p, c1, c2, xo);
   if (mode == 1)
     hipLaunchKernelGGL(k_mg_sweep<1>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
-                       (const float*)L.st.p, (const float*)L.minv.p, c1, c2, dz, x,
+                       (const float*)L.st.p, (const float*)L.minv.p, c1, c2, x,
                        (const double*)L.b.p, xo);
   else
     hipLaunchKernelGGL(k_mg_sweep<2>, dim3(mg_blocks(L.nn)), dim3(256), 0, stream, L.nx, L.ny,
-                       (const float*)L.st.p, (const float*)L.minv.p, c1, c2, dz, x,
+                       (const float*)L.st.p, (const float*)L.minv.p, c1, c2, x,
                        (const double*)L.b.p, xo);
   PP_HIP(hipGetLastError());
   return PP_OK;
@@ -580,7 +581,8 @@ int Mg::cycle(int l) {
   double theta = 1.0, delta = 1.0;
   if (!jacobi) cheb_interval(L.omega, theta, delta);
   const double sigma = theta / delta;
-  double* dz = jacobi ? nullptr : L.dz.p;
+  // x ping-pongs between L.x2 and L.x (2 ns - 1 steps end in L.x); a
+  // Chebyshev step reads the previous iterate from the buffer it overwrites
   double* cur = L.x2.p;
   auto other = [&](double* p) { return p == L.x.p ? L.x2.p : L.x.p; };
   // step k of a smoothing pass (k = 0 starts it from the current x)
@@ -598,17 +600,18 @@ int Mg::cycle(int l) {
         rho = rn;
       }
     }
-    const int r = sweep(l, 1, cur, other(cur), c1, c2, dz);
+    const int r = sweep(l, 1, cur, other(cur), c1, c2);
     cur = other(cur);
     return r;
   };
   // pre-smoothing from x = 0: the first step is pointwise
   rho = 1.0 / sigma;
   hipLaunchKernelGGL(k_mg_point, dim3(g), dim3(256), 0, stream, L.nn, 0, B,
-                     jacobi ? L.omega : 1.0 / theta, (const double*)L.b.p, (double*)nullptr, cur, dz);
+                     jacobi ? L.omega : 1.0 / theta, (const double*)L.b.p, (double*)nullptr, cur,
+                     jacobi ? nullptr : other(cur));
   for (int s = 1; s < ns; s++)
     if ((rc = step(s))) return rc;
-  if ((rc = sweep(l, 2, cur, L.t.p, 0.0, 0.0, nullptr))) return rc;  // residual
+  if ((rc = sweep(l, 2, cur, L.t.p, 0.0, 0.0))) return rc;  // residual
   MgLevel& C = lv[l + 1];
   hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nn)), dim3(256), 0, stream, L.nx, L.ny, mask,
                      (const double*)L.t.p, C.nx, C.ny, C.b.p);

@@ -75,15 +75,15 @@ __global__ __launch_bounds__(CfgS::NT) void k_apply(KGrid G, const double* x, do
 }
 
 // Fused multigrid sweep on the plan's operator: with (K x)_n as in k_apply,
-// MODE 1: d' = c1 d + c2 B (b - K x), xo = x + d' (a damped block-Jacobi
-// sweep when c1 = 0 and d is null; a Chebyshev step otherwise, d kept in
-// place); xo must not alias x, whose halo other workgroups read.
-// MODE 2: xo = b - K x.
+// MODE 1: xo = x + c1 (x - xo) + c2 B (b - K x): a damped block-Jacobi sweep
+// when c1 = 0 (xo is then not read), a Chebyshev step otherwise, xo holding
+// the previous iterate (the increment x - x_prev is the last step's, so no
+// increment vector is stored); xo must not alias x, whose halo other
+// workgroups read.  MODE 2: xo = b - K x.
 template <int MODE>
 __global__ __launch_bounds__(CfgS::NT) void k_apply_sweep(KGrid G, const double* x,
                                                           const double* b, const float* minv,
-                                                          double c1, double c2, double* dz,
-                                                          double* xo) {
+                                                          double c1, double c2, double* xo) {
   using C = CfgS;
   __shared__ double p_s[3 * C::NP];
   __shared__ double ye_s[12 * C::NE];
@@ -110,9 +110,9 @@ __global__ __launch_bounds__(CfgS::NT) void k_apply_sweep(KGrid G, const double*
         double z[3];
         sym3_apply(minv + 6 * n, r, z);
         for (int d = 0; d < 3; d++) {
-          const double dn = (dz ? c1 * dz[3 * n + d] : 0.0) + c2 * z[d];
-          if (dz) dz[3 * n + d] = dn;
-          xo[3 * n + d] = p_s[d * C::NP + c] + dn;
+          const double xc = p_s[d * C::NP + c];
+          const double dn = (c1 != 0.0 ? c1 * (xc - xo[3 * n + d]) : 0.0) + c2 * z[d];
+          xo[3 * n + d] = xc + dn;
         }
       }
     }
@@ -1198,16 +1198,16 @@ int struct_apply_k(pp_plan* pl, const double* dx, double* dy) {
 }
 
 int struct_sweep(pp_plan* pl, int mode, const double* x, const double* b, const float* minv,
-                 double c1, double c2, double* dz, double* xo) {
+                 double c1, double c2, double* xo) {
   KGrid G = make_kgrid(pl);
   const int ntiles =
       ((pl->nx + 1 + CfgS::TX - 1) / CfgS::TX) * ((pl->ny + 1 + CfgS::TY - 1) / CfgS::TY);
   if (mode == 1)
     hipLaunchKernelGGL(k_apply_sweep<1>, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, x, b,
-                       minv, c1, c2, dz, xo);
+                       minv, c1, c2, xo);
   else
     hipLaunchKernelGGL(k_apply_sweep<2>, dim3(ntiles), dim3(CfgS::NT), 0, pl->ctx->stream, G, x, b,
-                       minv, c1, c2, dz, xo);
+                       minv, c1, c2, xo);
   PP_HIP(hipGetLastError());
   return PP_OK;
 }
