@@ -5,14 +5,17 @@
 //   level 0      the plan's operator: matrix-free FP64 K with the reference's
 //                identity rows on constrained DOFs, its FP64 3x3 block-Jacobi
 //                inverse, and the constraint bits as excluded DOFs;
-//   level l + 1  elements (I, J) of size 2 x 2 level-l elements; coarse node I
-//                is level-l node 2I.  P_l interpolates each DOF component
-//                bilinearly in index space (weights 1, 1/2, 1/4) and is zero
+//   level l + 1  elements (I, J) of 2 x 2 level-l elements; coarse node I is
+//                level-l node min(2I, n) for n level-l elements along that
+//                side, so an odd count ends in a coarse element one level-l
+//                element wide (n -> (n + 1) / 2).  P_l interpolates each DOF
+//                component bilinearly in index space over the coarse
+//                element's level-l nodes (weights 1, 1/2, 1/4) and is zero
 //                on the excluded DOFs of level l, so that
 //                A_{l+1} = P_l^T A_l P_l.  Because a level-l node on the
 //                closure of coarse element E interpolates from E's corners
 //                only, A_{l+1} is the assembly of the coarse element matrices
-//                K_E = sum over the 4 children c of P_c^T (D K_c D) P_c
+//                K_E = sum over its (up to 4) children c of P_c^T (D K_c D) P_c
 //                (P_c the 12 x 12 corner weights, D the exclusion mask).
 //                A DOF whose assembled diagonal is exactly zero (its basis
 //                function vanishes) is excluded on that level: identity row,
@@ -58,15 +61,18 @@ __host__ __device__ __forceinline__ int corner_dy(int k) { return k >= 2; }
 // tensor grid, from its corner coordinates and thickness, with the
 // constraint bits of level 0 as the exclusion mask; level >= 1: the stored
 // element matrices kf [fine element][144] with the dead bits of level l.
-__global__ void k_mg_coarse_elems(int ncx, int ncy, int fnx, const double* kf, KGrid G, double E,
-                                  double nu, const uint8_t* mask, double* kc) {
+__global__ void k_mg_coarse_elems(int ncx, int ncy, int fnx, int fny, const double* kf, KGrid G,
+                                  double E, double nu, const uint8_t* mask, double* kc) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)ncx * ncy) return;
   const int I = (int)(e % ncx), J = (int)(e / ncx);
+  // children per side: 2, or 1 for the last coarse element of an odd count
+  const int sx = min(2, fnx - 2 * I), sy = min(2, fny - 2 * J);
   double Kc[144];
   for (int k = 0; k < 144; k++) Kc[k] = 0.0;
   double Kf[144];
   for (int c = 0; c < 4; c++) {
+    if (corner_dx(c) >= sx || corner_dy(c) >= sy) continue;
     const int fi = 2 * I + corner_dx(c), fj = 2 * J + corner_dy(c);  // child element (fi, fj)
     if (kf) {
       const double* src = kf + ((int64_t)fj * fnx + fi) * 144;
@@ -84,10 +90,12 @@ __global__ void k_mg_coarse_elems(int ncx, int ncy, int fnx, const double* kf, K
     unsigned m[4];
     for (int k = 0; k < 4; k++)
       m[k] = mask[(int64_t)(fj + corner_dy(k)) * (fnx + 1) + fi + corner_dx(k)];
-    // corner weights: child corner k sits at (u, v) in {0, 1/2, 1} of E
+    // corner weights: child corner k sits at (u, v) in {0, 1/2, 1} of E (in
+    // {0, 1} along a side with one child)
     double w[4][4];
     for (int k = 0; k < 4; k++) {
-      const double u = 0.5 * (corner_dx(c) + corner_dx(k)), v = 0.5 * (corner_dy(c) + corner_dy(k));
+      const double u = (double)(corner_dx(c) + corner_dx(k)) / sx;
+      const double v = (double)(corner_dy(c) + corner_dy(k)) / sy;
       w[k][0] = (1 - u) * (1 - v);
       w[k][1] = u * (1 - v);
       w[k][2] = u * v;
@@ -248,10 +256,13 @@ __global__ void k_mg_point(int64_t nn, int mode, const float* minv, double omega
   }
 }
 
-// Interpolation weight of level-l index f from coarse index c (|f - 2c| <= 1).
-__device__ __forceinline__ double w1(int f, int c) {
+// Interpolation weight of level-l node index f from coarse node index c along
+// a side of n level-l elements: coarse node c sits on level-l node
+// min(2c, n); an odd f < n between two coarse nodes takes half of each.
+__device__ __forceinline__ double w1(int f, int c, int n) {
+  if (min(2 * c, n) == f) return 1.0;
   const int d = f - 2 * c;
-  return d == 0 ? 1.0 : (d == 1 || d == -1) ? 0.5 : 0.0;
+  return ((f & 1) && f < n && (d == 1 || d == -1)) ? 0.5 : 0.0;
 }
 
 // b_c = P^T r on coarse node (I, J): the level-l nodes (2I + a, 2J + b),
@@ -268,7 +279,7 @@ __global__ void k_mg_restrict(int fnx, int fny, const uint8_t* fmask, const doub
     for (int a = -1; a <= 1; a++) {
       const int fi = 2 * I + a;
       if (fi < 0 || fi > fnx) continue;
-      const double w = w1(fi, I) * w1(fj, J);
+      const double w = w1(fi, I, fnx) * w1(fj, J, fny);
       const int64_t m = (int64_t)fj * (fnx + 1) + fi;
       const unsigned msk = fmask[m];
       for (int d = 0; d < 3; d++)
@@ -290,7 +301,7 @@ __global__ void k_mg_prolong(int fnx, int fny, const uint8_t* fmask, const doubl
   const int ni = i0 == i1 ? 1 : 2, nj = j0 == j1 ? 1 : 2;
   for (int b = 0; b < nj; b++)
     for (int a = 0; a < ni; a++) {
-      const double w = w1(fi, ci[a]) * w1(fj, cj[b]);
+      const double w = w1(fi, ci[a], fnx) * w1(fj, cj[b], fny);
       const int64_t m = (int64_t)cj[b] * (cnx + 1) + ci[a];
       for (int d = 0; d < 3; d++) s[d] += w * xc[3 * m + d];
     }
@@ -398,15 +409,15 @@ int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps, bool 
     const int nx = dims.back().first, ny = dims.back().second;
     const int64_t nd = 3LL * (nx + 1) * (ny + 1);
     if (dims.size() > 1 && nd <= kDenseStop) break;
-    if (nx % 2 || ny % 2 || nx < 2 || ny < 2 || (int)dims.size() == kMaxLevels) break;
-    dims.push_back({nx / 2, ny / 2});
+    if (nx < 2 || ny < 2 || (int)dims.size() == kMaxLevels) break;
+    dims.push_back({(nx + 1) / 2, (ny + 1) / 2});
   }
   nl = (int)dims.size();
   const int64_t ndc = 3LL * (dims.back().first + 1) * (dims.back().second + 1);
   if (nl < 2 || ndc > kDenseMax)
     return fail(PP_ERR_INVALID,
-                "multigrid: the element counts per side must halve down to a coarsest grid of "
-                "at most 2,200 DOFs");
+                "multigrid: halving the element counts per side (odd counts rounded up) must "
+                "reach a coarsest grid of at most 2,200 DOFs");
   nd_coarse = (int)ndc;
   int rc;
   for (int l = 0; l < nl; l++) {
@@ -436,8 +447,8 @@ int Mg::setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps, bool 
     if ((rc = kcoarse.alloc(144 * nec))) return rc;
     const uint8_t* fmask = l == 0 ? (const uint8_t*)pl->cmask.p : (const uint8_t*)F.dead.p;
     hipLaunchKernelGGL(k_mg_coarse_elems, dim3(mg_blocks(nec, 64)), dim3(64), 0, stream, Cl.nx,
-                       Cl.ny, F.nx, l == 0 ? (const double*)nullptr : (const double*)kfine.p, G,
-                       pl->E, pl->nu, fmask, kcoarse.p);
+                       Cl.ny, F.nx, F.ny, l == 0 ? (const double*)nullptr : (const double*)kfine.p,
+                       G, pl->E, pl->nu, fmask, kcoarse.p);
     PP_HIP(hipGetLastError());
     if (l + 1 < nl - 1) {
       MgLevel& C = lv[l + 1];

@@ -21,8 +21,10 @@ before and after the coarse correction, is the library's:
 S_nu nu damped block-Jacobi sweeps x += w B (b - A x) or nu Chebyshev steps
 (smooth below).
 
-Coarsening rule (the library's): halve while both element counts are even,
-stopping at the first level (below 0) with at most 1,200 DOFs.
+Coarsening rule (the library's): n -> (n + 1) / 2 elements per side (coarse
+node c on fine node min(2c, n), so an odd count ends in a coarse interval one
+fine element wide) while both counts are at least 2, stopping at the first
+level (below 0) with at most 1,200 DOFs.
 """
 import numpy as np
 import scipy.linalg as sla
@@ -58,14 +60,19 @@ def with_bc(K, free):
 
 
 def interp_1d(n):
-    """(n + 1) x (n / 2 + 1): even index 2c -> c, odd -> half of each side."""
+    """(n + 1) x ((n + 1) // 2 + 1): fine node i on coarse node i / 2 when i
+    is even, on the last coarse node when i = n is odd, else half of each
+    neighbouring coarse node."""
+    m = (n + 1) // 2
     r, c, v = [], [], []
     for i in range(n + 1):
         if i % 2 == 0:
             r.append(i); c.append(i // 2); v.append(1.0)
+        elif i == n:
+            r.append(i); c.append(m); v.append(1.0)
         else:
             r += [i, i]; c += [(i - 1) // 2, (i + 1) // 2]; v += [0.5, 0.5]
-    return sp.csr_matrix((v, (r, c)), shape=(n + 1, n // 2 + 1))
+    return sp.csr_matrix((v, (r, c)), shape=(n + 1, m + 1))
 
 
 def prolongation(nx, ny, keep):
@@ -98,9 +105,9 @@ def hierarchy_dims(nx, ny):
         a, b = dims[-1]
         if len(dims) > 1 and 3 * (a + 1) * (b + 1) <= DENSE_STOP:
             break
-        if a % 2 or b % 2 or a < 2 or b < 2:
+        if a < 2 or b < 2:
             break
-        dims.append((a // 2, b // 2))
+        dims.append(((a + 1) // 2, (b + 1) // 2))
     return dims
 
 

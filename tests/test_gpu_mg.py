@@ -57,8 +57,12 @@ def rel_err(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-@pytest.mark.parametrize("nx,ny,var_t", [(32, 32, True), (64, 64, True), (48, 40, False)])
+@pytest.mark.parametrize("nx,ny,var_t", [(32, 32, True), (64, 64, True), (48, 40, False),
+                                          (50, 38, True)])
 def test_levels_operators_and_vcycle_match_oracle(ctx, nx, ny, var_t):
+    """Levels, their operators and one V-cycle against the oracle; 50 x 38
+    coarsens through odd counts (25 x 19 -> 13 x 10), whose last coarse
+    element is one level-1 element wide."""
     s = setup(ctx, nx, ny, var_t)
     net, ref, lv = s["net"], s["ref"], s["lv"]
     assert [(a, b) for a, b, _ in lv] == ref.dims
@@ -100,8 +104,9 @@ def test_sweep_count_and_unsupported_grids(ctx):
         no_mg.mg_cycle(b)
     no_mg.close()
     s["plan"].close()
-    # 142 x 142 halves once to 71 x 71 (15,552 DOFs): too large a coarsest level
-    mesh = P.rect_mesh(142)
+    # 1500 x 2 coarsens to 750 x 1 and stops there (4,506 DOFs): too large a
+    # coarsest level
+    mesh = P.rect_mesh(1500, 2, width=150.0, height=0.2)
     mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
     plan = Plan(ctx, mesh, E, NU, 0.1, mask)
     with pytest.raises(PlatePCGError):

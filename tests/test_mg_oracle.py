@@ -23,9 +23,11 @@ import mg_ref as M  # noqa: E402
 E, NU = 210e9, 0.3
 
 
-def plate(n, ny=None, var_t=True, clamped=True, seed=3):
+def plate(n, ny=None, var_t=True, clamped=True, seed=3, h=None):
+    """h: element size (float32 node coordinates are exactly uniform only
+    for a dyadic size; default a 10 m square)."""
     ny = n if ny is None else ny
-    mesh = P.rect_mesh(n, ny)
+    mesh = P.rect_mesh(n, ny) if h is None else P.rect_mesh(n, ny, width=n * h, height=ny * h)
     et = None
     if var_t:
         rng = np.random.default_rng(seed)
@@ -39,9 +41,14 @@ def plate(n, ny=None, var_t=True, clamped=True, seed=3):
 
 
 def level_coords(nodes, nx, ny, l):
+    """Coordinates of the level-l nodes: coarse node c of a side with n
+    elements sits on node min(2c, n) of the level above."""
     xy = np.asarray(nodes, np.float64).reshape(ny + 1, nx + 1, 2)
-    s = 2 ** l
-    return xy[::s, ::s].reshape(-1, 2)
+    ix, iy = np.arange(nx + 1), np.arange(ny + 1)
+    for _ in range(l):
+        ix = ix[np.minimum(2 * np.arange((len(ix) - 1 + 1) // 2 + 1), len(ix) - 1)]
+        iy = iy[np.minimum(2 * np.arange((len(iy) - 1 + 1) // 2 + 1), len(iy) - 1)]
+    return xy[np.ix_(iy, ix)].reshape(-1, 2)
 
 
 def test_hierarchy_shapes():
@@ -49,15 +56,21 @@ def test_hierarchy_shapes():
     assert M.hierarchy_dims(32, 32) == [(32, 32), (16, 16)]
     assert M.hierarchy_dims(48, 40) == [(48, 40), (24, 20), (12, 10)]
     assert M.hierarchy_dims(1024, 1024)[-1] == (16, 16)
-    assert M.hierarchy_dims(142, 142) == [(142, 142), (71, 71)]
+    # odd counts round up: the last coarse interval is one element wide
+    assert M.hierarchy_dims(142, 142) == [(142, 142), (71, 71), (36, 36), (18, 18)]
+    assert M.hierarchy_dims(50, 38) == [(50, 38), (25, 19), (13, 10)]
+    assert M.interp_1d(7).shape == (8, 5)
 
 
-def test_rigid_modes_stay_in_the_kernel():
+@pytest.mark.parametrize("n,ny", [(64, 64), (50, 38)])
+def test_rigid_modes_stay_in_the_kernel(n, ny):
     """Free plate: w = 1 and the two rigid rotations (w linear, the matching
-    rotation constant) give A_l v = 0 on every level."""
-    n = 64
-    mesh, mask, et, K, free = plate(n, clamped=False)
-    mg = M.MgOracle(K, n, n, free, omegas=[1.0] * 3, factor=False)
+    rotation constant) give A_l v = 0 on every level, also where odd element
+    counts leave a last coarse interval one element wide (50 x 38 -> 25 x 19
+    -> 13 x 10).  Elements of 0.25 m: index-space midpoints are then the
+    physical ones in float32 coordinates."""
+    mesh, mask, et, K, free = plate(n, ny, clamped=False, h=0.25)
+    mg = M.MgOracle(K, n, ny, free, omegas=[1.0] * 3, factor=False)
 
     def modes(xy, signs):
         out = [np.tile([1.0, 0.0, 0.0], len(xy))]
@@ -69,7 +82,7 @@ def test_rigid_modes_stay_in_the_kernel():
         return out
 
     # the sign convention of each rotation: the one K annihilates on level 0
-    xy0 = level_coords(mesh["nodes"], n, n, 0)
+    xy0 = level_coords(mesh["nodes"], n, ny, 0)
     scale0 = np.abs(K).max() * 10.0
     signs = []
     for k in range(2):
@@ -80,19 +93,18 @@ def test_rigid_modes_stay_in_the_kernel():
     for l in range(len(mg.dims)):
         A = mg.A[l]
         scale = np.abs(A).max() * 10.0
-        for v in modes(level_coords(mesh["nodes"], n, n, l), signs):
+        for v in modes(level_coords(mesh["nodes"], n, ny, l), signs):
             assert np.abs(A @ v).max() <= 1e-9 * scale, (l, np.abs(A @ v).max() / scale)
         # and a non-rigid field is not annihilated
-        v = modes(level_coords(mesh["nodes"], n, n, l), [-signs[0], -signs[1]])[1]
+        v = modes(level_coords(mesh["nodes"], n, ny, l), [-signs[0], -signs[1]])[1]
         assert np.abs(A @ v).max() > 1e-3 * scale
 
 
-@pytest.mark.parametrize("chebyshev", [False, True])
-def test_vcycle_is_symmetric_and_contracts(chebyshev):
-    n = 64
-    _, mask, _, K, free = plate(n)
+@pytest.mark.parametrize("chebyshev,n,ny", [(False, 64, 64), (True, 64, 64), (True, 50, 38)])
+def test_vcycle_is_symmetric_and_contracts(chebyshev, n, ny):
+    _, mask, _, K, free = plate(n, ny)
     A0 = M.with_bc(K, free)
-    mg = M.MgOracle(A0, n, n, free, sweeps=3, coarse_sweeps=1, chebyshev=chebyshev)
+    mg = M.MgOracle(A0, n, ny, free, sweeps=3, coarse_sweeps=1, chebyshev=chebyshev)
     assert len(mg.A) == 3
     rng = np.random.default_rng(0)
     x, y = rng.standard_normal((2, A0.shape[0]))
