@@ -141,6 +141,7 @@ void plan_release(pp_plan* pl) {
   pp_ctx* c = pl->ctx;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  mg_destroy(pl->mg_pc);
   delete pl;
   ctx_release(c);
 }
@@ -319,7 +320,7 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   if (!pl->rhs_set) return fail(PP_ERR_INVALID, "no right-hand side set");
   if (!(o->tolerance > 0) || o->max_iterations < 0) return fail(PP_ERR_INVALID, "bad tolerance/max_iterations");
   if (o->precond != PP_PRECOND_BLOCK_JACOBI && o->precond != PP_PRECOND_LEARNED &&
-      o->precond != PP_PRECOND_CHEBYSHEV)
+      o->precond != PP_PRECOND_CHEBYSHEV && o->precond != PP_PRECOND_MULTIGRID)
     return fail(PP_ERR_INVALID, "unknown preconditioner");
   memset(st, 0, sizeof(*st));
   PP_HIP(hipSetDevice(pl->ctx->device));
@@ -332,6 +333,10 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     if (o->precision != PP_PRECISION_FP64)
       return fail(PP_ERR_UNSUPPORTED, "the learned preconditioner runs with FP64 flexible PCG");
     rc = learned_solve(pl, o, st);
+  } else if (o->precond == PP_PRECOND_MULTIGRID) {
+    if (!structured || o->precision != PP_PRECISION_FP64)
+      return fail(PP_ERR_UNSUPPORTED, "the multigrid preconditioner runs on the structured FP64 path");
+    rc = mg_solve(pl, o, st);
   } else if (o->precond == PP_PRECOND_CHEBYSHEV) {
     if (!structured || o->precision != PP_PRECISION_FP64)
       return fail(PP_ERR_UNSUPPORTED, "the Chebyshev preconditioner runs on the structured FP64 path");

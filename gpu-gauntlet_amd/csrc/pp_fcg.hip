@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <vector>
 
 #include "pp_cnn.h"
@@ -233,17 +234,21 @@ __global__ void k_fcg_init(FcgState* S, double fnorm, double tol, double rr) {
 
 }  // namespace
 
-// FCG from x (zero on a cold start) with r = F - K x given in pl->r; P and Q
-// are rings of m + 1 vectors each (directions, their K-products).
-static int fcg_run(pp_plan* pl, Cnn& net, double fnorm, double tol, int maxit, FcgState* hs,
-                   DevBuf<FcgState>& S, DevBuf<double>& part, double* P, double* Q) {
+// z = M(r) on device FP64 DOF vectors
+using Precond = std::function<int(const double* r, double* z)>;
+
+// FCG(m) from x (zero on a cold start) with r = F - K x given in pl->r; P and
+// Q are rings of m + 1 vectors each (directions, their K-products).
+static int fcg_run(pp_plan* pl, const Precond& prec, int m, double fnorm, double tol, int maxit,
+                   FcgState* hs, DevBuf<FcgState>& S, DevBuf<double>& part, double* P,
+                   double* Q) {
   hipStream_t s = pl->ctx->stream;
   const int64_t n = pl->n_dof;
-  const int m = net.fcg_window, R = m + 1;
+  const int R = m + 1;
   int rc;
   hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(64), 0, s, S.p, fnorm, tol, 0.0);
   // z = M(r), p_0 = z
-  if ((rc = net.precondition(pl->r.p, pl->z.p, net.line_search != 0))) return rc;
+  if ((rc = prec(pl->r.p, pl->z.p))) return rc;
   Win none{};
   hipLaunchKernelGGL(k_fcg_dir, dim3(nb(n)), dim3(256), 0, s, n, (const double*)pl->z.p, P, none,
                      0, (const FcgState*)S.p, 1);
@@ -265,7 +270,7 @@ static int fcg_run(pp_plan* pl, Cnn& net, double fnorm, double tol, int maxit, F
       PP_HIP(hipStreamSynchronize(s));
       if (hs->done) break;
     }
-    if ((rc = net.precondition(pl->r.p, pl->z.p, net.line_search != 0))) return rc;
+    if ((rc = prec(pl->r.p, pl->z.p))) return rc;
     Win win{};
     win.w = std::min(k + 1, m);
     for (int j = 0; j < win.w; j++) win.slot[j] = (k - j) % R;
@@ -286,13 +291,13 @@ static int fcg_run(pp_plan* pl, Cnn& net, double fnorm, double tol, int maxit, F
 // residual is measured; while it is above tol, FCG restarts from x with that
 // residual (at most kMaxRestarts times).  Replacing r inside the run instead
 // was measured slower: near 1e-8 the true residual is mostly rounding noise.
-int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
-  Cnn& net = *pl->learned;
+static int fcg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st,
+                     const Precond& prec, int m) {
   hipStream_t s = pl->ctx->stream;
   const int64_t n = pl->n_dof;
   DevBuf<FcgState> S;
   DevBuf<double> part, ring;
-  const int R = net.fcg_window + 1;
+  const int R = m + 1;
   int rc;
   if ((rc = S.alloc(1)) || (rc = part.alloc((size_t)std::max(2, kMaxWin) * kBlocks)) ||
       (rc = ring.alloc((size_t)2 * R * n)))
@@ -310,8 +315,8 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     hs.converged = 1;
   } else {
     for (int k = 0; k <= kMaxRestarts; k++) {
-      if ((rc = fcg_run(pl, net, fnorm, o->tolerance, o->max_iterations - total, &hs, S, part,
-                        ring.p, ring.p + (size_t)R * n)))
+      if ((rc = fcg_run(pl, prec, m, fnorm, o->tolerance, o->max_iterations - total, &hs, S,
+                        part, ring.p, ring.p + (size_t)R * n)))
         return rc;
       if (sweeps < PP_MAX_SWEEPS) st->sweep_iterations[sweeps] = hs.it;
       total += hs.it;
@@ -336,6 +341,37 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   st->true_rel_residual = rel;
   st->device_ms = ms;
   return PP_OK;
+}
+
+int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  Cnn& net = *pl->learned;
+  const bool ls = net.line_search != 0;
+  return fcg_solve(
+      pl, o, st, [&](const double* r, double* z) { return net.precondition(r, z, ls); },
+      net.fcg_window);
+}
+
+// PCG with one Galerkin multigrid V-cycle per iteration (symmetric and
+// linear, so FCG(1) is plain PCG); the hierarchy is built on the first such
+// solve of the plan, with level 0 smoothing at w = 4 / (3 1.05 lambda_max(B K))
+// and o->poly_degree smoothing steps on level 0 (0: 6).
+int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  const int nu = o->poly_degree > 0 ? o->poly_degree : 6;
+  if (nu > 8) return fail(PP_ERR_INVALID, "multigrid: poly_degree (smoothing steps) must be at most 8");
+  int rc;
+  if (!pl->mg_pc || pl->mg_pc->nu != nu) {
+    mg_destroy(pl->mg_pc);
+    pl->mg_pc = nullptr;
+    if (pl->cheb_lmax <= 0 && (rc = cheb_lambda_max(pl))) return rc;
+    Mg* mg = new Mg;
+    if ((rc = mg->setup(pl, 4.0 / (3.0 * 1.05 * pl->cheb_lmax), nu, 0, false))) {
+      delete mg;
+      return rc;
+    }
+    pl->mg_pc = mg;
+  }
+  Mg* mg = pl->mg_pc;
+  return fcg_solve(pl, o, st, [mg](const double* r, double* z) { return mg->vcycle(r, z); }, 1);
 }
 
 }  // namespace pp

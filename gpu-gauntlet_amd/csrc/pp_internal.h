@@ -76,6 +76,7 @@ struct DevStats {
 };
 
 struct Cnn;
+struct Mg;
 
 }  // namespace pp
 
@@ -91,6 +92,7 @@ struct pp_plan {
   pp_ctx* ctx = nullptr;
   int refs = 1;             // the owner plus one per live network on this plan
   pp::Cnn* learned = nullptr;  // preconditioner for PP_PRECOND_LEARNED (not owned)
+  pp::Mg* mg_pc = nullptr;     // V-cycle of PP_PRECOND_MULTIGRID (owned, built on first use)
   int64_t n_nodes = 0, n_elems = 0, n_dof = 0;
   int npe = 4;
   bool structured = false;
@@ -160,6 +162,10 @@ int struct_sweep(pp_plan* pl, int mode, const double* x, const double* b, const 
 int struct_apply_precond(pp_plan* pl, const double* dr, double* dz);
 int struct_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF);
 int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+int cheb_lambda_max(pp_plan* pl);  // lambda_max(B K) by power iteration -> pl->cheb_lmax
+// PCG preconditioned by a Galerkin multigrid V-cycle (pp_fcg.hip, pp_mg.hip)
+int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+void mg_destroy(Mg* mg);
 int struct_solve_mixed(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
 int struct_true_residual(pp_plan* pl, const double* dx, double* rel);
 

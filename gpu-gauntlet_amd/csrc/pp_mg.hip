@@ -636,6 +636,8 @@ int Mg::cycle(int l) {
   return PP_OK;
 }
 
+void mg_destroy(Mg* mg) { delete mg; }
+
 int Mg::vcycle(const double* b, double* x) {
   MgLevel& L = lv[0];
   // level 0 reads b and writes x in place of its own buffers

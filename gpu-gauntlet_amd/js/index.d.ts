@@ -35,9 +35,11 @@ export interface GPUSolverOptions {
   forceCPU?: boolean;
   precision?: 'fp64' | 'mixed';
   /** chebyshev: block Jacobi inside a Chebyshev polynomial of degree polyDegree in B K
-   *  (structured FP64 path, plates of up to 512 tiles of 18 x 18 nodes) */
-  preconditioner?: 'blockJacobi' | 'learned' | 'chebyshev';
-  /** chebyshev: polynomial degree (default 8, at most 32) */
+   *  (structured FP64 path, plates of up to 512 tiles of 18 x 18 nodes);
+   *  multigrid: one Galerkin multigrid V-cycle per PCG iteration (structured FP64 path) */
+  preconditioner?: 'blockJacobi' | 'learned' | 'chebyshev' | 'multigrid';
+  /** chebyshev: polynomial degree (default 8, at most 32); multigrid: smoothing steps on the
+   *  finest level (default 6, at most 8) */
   polyDegree?: number;
   path?: 'auto' | 'structured' | 'coloured';
   /** mixed: a refinement sweep ends when the FP32 residual has fallen by this factor (default 1e-3) */

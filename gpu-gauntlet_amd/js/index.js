@@ -17,7 +17,7 @@
 const path = require('path');
 
 const PRECISION = { fp64: 0, mixed: 1 };
-const PRECOND = { blockJacobi: 0, learned: 1, chebyshev: 2 };
+const PRECOND = { blockJacobi: 0, learned: 1, chebyshev: 2, multigrid: 3 };
 const PATH = { auto: 0, structured: 1, coloured: 2 };
 const FIX = { w: 1, thx: 2, thy: 4 };
 

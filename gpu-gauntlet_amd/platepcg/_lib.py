@@ -20,7 +20,7 @@ PP_ERR_NODEVICE = -6
 PP_FIX_W, PP_FIX_THX, PP_FIX_THY = 1, 2, 4
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_CHEBYSHEV = 0, 1, 2
+PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_CHEBYSHEV, PRECOND_MULTIGRID = 0, 1, 2, 3
 PATH_AUTO, PATH_STRUCTURED, PATH_COLOURED = 0, 1, 2
 MAX_SWEEPS = 32
 FLAG_NO_TRUE_RESIDUAL = 1

@@ -73,6 +73,12 @@ typedef struct {
                                      poly_degree times fewer PCG iterations and all-reduces for
                                      about the same number of K applies; structured FP64 plates
                                      whose tiles are co-resident (up to about 1,000^2 nodes) */
+#define PP_PRECOND_MULTIGRID 3    /* one Galerkin geometric-multigrid V-cycle per PCG iteration
+                                     (no reference counterpart): Chebyshev smoothing, bilinear
+                                     prolongation, dense coarsest solve; the hierarchy is built
+                                     on the plan's first such solve.  Structured FP64 plates
+                                     whose coarsest level (element counts halved, odd counts
+                                     rounded up) has at most 2,200 DOFs */
 
 #define PP_PATH_AUTO 0       /* structured kernels when the mesh is a grid */
 #define PP_PATH_STRUCTURED 1 /* tensor-grid tile kernels (error if not a grid) */
@@ -90,7 +96,9 @@ typedef struct {
                               fallen by this factor (<= 0: 1e-3) */
   int32_t max_refinements; /* mixed: most FP64 refinement sweeps (<= 0: 200) */
   int32_t flags;           /* PP_FLAG_* */
-  int32_t poly_degree;     /* PP_PRECOND_CHEBYSHEV: degree (<= 0: 8, at most 32) */
+  int32_t poly_degree;     /* PP_PRECOND_CHEBYSHEV: degree (<= 0: 8, at most 32);
+                              PP_PRECOND_MULTIGRID: smoothing steps on the finest level
+                              (<= 0: 6, at most 8) */
 } pp_solve_opts;
 
 /* Skip the FP64 |F - K x| / |F| recomputation after the solve
