@@ -27,6 +27,9 @@ async function main() {
   const cheb = await host.solveGPU(mesh, material, coloring, F, constrained,
     { tolerance: 1e-8, maxIterations: 20000, preconditioner: 'chebyshev', polyDegree: 6 });
   const relCheb = Math.abs(cheb.solution[3 * gi] - cpu.solution[3 * ci]) / Math.abs(cpu.solution[3 * ci]);
+  const mg = await host.solveGPU(mesh, material, coloring, F, constrained,
+    { tolerance: 1e-8, maxIterations: 2000, preconditioner: 'multigrid' });
+  const relMg = Math.abs(mg.solution[3 * gi] - cpu.solution[3 * ci]) / Math.abs(cpu.solution[3 * ci]);
   let forceCpuRejected = false;
   try {
     await host.solveGPU(mesh, material, coloring, F, constrained, { forceCPU: true });
@@ -37,7 +40,8 @@ async function main() {
     cpuIterations: cpu.iterations, trueRelResidual: gpu.trueRelResidual, maxWRelError: rel,
     forceCpuRejected: forceCpuRejected, chebConverged: cheb.converged,
     chebIterations: cheb.iterations, chebTrueRelResidual: cheb.trueRelResidual,
-    chebMaxWRelError: relCheb,
+    chebMaxWRelError: relCheb, mgConverged: mg.converged, mgIterations: mg.iterations,
+    mgTrueRelResidual: mg.trueRelResidual, mgMaxWRelError: relMg,
   }));
 }
 

@@ -52,6 +52,9 @@ def test_solve_gpu_through_node():
     # preconditioner: 'chebyshev' through the same host call
     assert r["chebConverged"] and r["chebTrueRelResidual"] <= 1e-8
     assert r["chebMaxWRelError"] < 1e-4 and r["chebIterations"] < r["iterations"] / 3
+    # preconditioner: 'multigrid' (one V-cycle per PCG iteration)
+    assert r["mgConverged"] and r["mgTrueRelResidual"] <= 1e-8
+    assert r["mgMaxWRelError"] < 1e-4 and r["mgIterations"] < r["iterations"] / 10
 
 
 @pytest.mark.gpu
