@@ -47,10 +47,11 @@ struct Win {
 
 constexpr int kBlocks = 1024;  // partial sums per reduction: 16 waves per CU stream the vectors
 constexpr int kCheckEvery = 4;
-// FP64 restarts after the first run.  On the 1024^2 log-normal plate 1e-8 is
-// at the FP64 floor of the relative residual (the rounding of x and of K x,
-// about eps |K| |x| / |F|): there restarts of 1-4 iterations each land between
-// 0.95e-8 and 1.05e-8, and it takes several to draw one below the tolerance
+// FP64 restarts after the first run (learned and multigrid solves).  On the
+// 1024^2 log-normal plate 1e-8 is at the FP64 floor of the relative residual
+// (the rounding of x and of K x, about eps |K| |x| / |F|): there restarts of
+// 1-4 iterations each land between 0.95e-8 and 1.05e-8, and it takes several
+// to draw one below the tolerance
 constexpr int kMaxRestarts = 16;
 
 int nb(int64_t n) { return (int)((n + 255) / 256); }
