@@ -236,6 +236,25 @@ def bench_learned(a):
         print(f"{tag}: {st.iterations} iterations, {res[tag]['ms']:.0f} ms", file=sys.stderr,
               flush=True)
         res[tag]["x"] = plan.solution()[0]
+    # plain PCG with one multigrid V-cycle per iteration (PP_PRECOND_MULTIGRID, no
+    # network): a reported alternative, not the value; its hierarchy is built
+    # in the warm-up solve
+    om = plan.options(tolerance=1e-8, max_iterations=a.learned_max_iterations,
+                      precond=_lib.PRECOND_MULTIGRID)
+    for _ in range(max(1, a.warmup)):
+        plan.solve_resident(om)
+    t1, dev = time.perf_counter(), []
+    for _ in range(a.steps):
+        st = plan.solve_resident(om)
+        dev.append(st.device_ms)
+    res["multigrid"] = {"ms": (time.perf_counter() - t1) * 1e3 / a.steps,
+                        "device_ms": float(np.mean(dev)), "iterations": st.iterations,
+                        "true_rel_residual": st.true_rel_residual,
+                        "converged": bool(st.converged)}
+    res["multigrid"]["max_nodal_w_error_vs_bj"] = O.max_nodal_w_error(plan.solution()[0],
+                                                                       res["block_jacobi"]["x"])
+    print(f"multigrid: {st.iterations} iterations, {res['multigrid']['ms']:.0f} ms", file=sys.stderr,
+          flush=True)
     err = O.max_nodal_w_error(res["learned"].pop("x"), res["block_jacobi"]["x"])
     err0 = O.max_nodal_w_error(res["zero_network"].pop("x"), res["block_jacobi"].pop("x"))
     n_dof = plan.n_dof
@@ -255,7 +274,7 @@ def bench_learned(a):
                                                       else ", flexible PCG (FCG(8))"),
                    "precision": "fp64 outer, bf16 network"},
         "learned": res["learned"], "block_jacobi": res["block_jacobi"],
-        "zero_network": res["zero_network"],
+        "zero_network": res["zero_network"], "multigrid": res["multigrid"],
         "network_iteration_gain": (res["zero_network"]["iterations"] / res["learned"]["iterations"]
                                    if res["learned"]["converged"] and res["zero_network"]["converged"]
                                    else None),
