@@ -41,3 +41,15 @@ def test_no_device_reports_cleanly():
         pytest.skip("a HIP device is present")
     out = C.c_void_p()
     assert _lib.load().pp_ctx_create(0, C.byref(out)) == _lib.PP_ERR_NODEVICE
+
+
+def test_python_constants_match_the_header():
+    """Every #define PP_<NAME> <int> of include/platepcg.h has the same value
+    in the Python mirror (as PP_<NAME>, or <NAME> without the prefix)."""
+    text = open(os.path.join(ROOT, "include", "platepcg.h")).read()
+    defs = re.findall(r"^#define\s+PP_(\w+)\s+\(?(-?\d+)u?\)?", text, re.M)
+    assert len(defs) > 20
+    for name, value in defs:
+        mirror = getattr(_lib, "PP_" + name, getattr(_lib, name, None))
+        assert mirror is not None, name
+        assert mirror == int(value), (name, mirror, value)
