@@ -54,6 +54,10 @@ struct Mg {
   DevBuf<double> cinv;  // dense inverse of the coarsest operator [nd][nd]
   DevBuf<double> dotp;
   bool jacobi = false;  // smoothing by damped block Jacobi instead of Chebyshev steps
+  // Chebyshev steps target [lambda_max / cheb_ratio, lambda_max] of B A.  Measured at 1024^2:
+  // 16 suits the learned preconditioner's two V-cycles (105 flexible-PCG iterations against
+  // 122-127 at 24-32); PCG with one V-cycle takes 32 (181 against 195 iterations)
+  double cheb_ratio = 16.0;
   int setup(pp_plan* plan, double omega0, int sweeps, int coarse_sweeps, bool damped_jacobi);
   int vcycle(const double* b, double* x);  // x = V b on level 0 (device FP64 DOF vectors)
   int apply_level(int l, const double* x, double* y);  // y = A_l x

@@ -354,8 +354,8 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
 
 // PCG with one Galerkin multigrid V-cycle per iteration (symmetric and
 // linear, so FCG(1) is plain PCG); the hierarchy is built on the first such
-// solve of the plan, with level 0 smoothing at w = 4 / (3 1.05 lambda_max(B K))
-// and o->poly_degree smoothing steps on level 0 (0: 6).
+// solve of the plan, with level 0 smoothing at w = 4 / (3 1.05 lambda_max(B K)),
+// o->poly_degree Chebyshev steps on level 0 (0: 6) on [lambda_max / 32, lambda_max].
 int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   const int nu = o->poly_degree > 0 ? o->poly_degree : 6;
   if (nu > 8) return fail(PP_ERR_INVALID, "multigrid: poly_degree (smoothing steps) must be at most 8");
@@ -365,6 +365,7 @@ int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     pl->mg_pc = nullptr;
     if (pl->cheb_lmax <= 0 && (rc = cheb_lambda_max(pl))) return rc;
     Mg* mg = new Mg;
+    mg->cheb_ratio = 32.0;
     if ((rc = mg->setup(pl, 4.0 / (3.0 * 1.05 * pl->cheb_lmax), nu, 0, false))) {
       delete mg;
       return rc;

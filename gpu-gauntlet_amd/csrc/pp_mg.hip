@@ -543,12 +543,11 @@ int Mg::estimate_omega(int l) {
   return PP_OK;
 }
 
-// Chebyshev smoothing interval of a level: [hi / kChebSmoothRatio, hi],
-// hi = 1.05 lambda_max(B A), lambda_max recovered from the level's weight
+// Chebyshev smoothing interval of a level: [hi / ratio, hi], hi = 1.05
+// lambda_max(B A), lambda_max recovered from the level's weight
 // w = 4 / (3 * 1.05 lambda_max).
-constexpr double kChebSmoothRatio = 16.0;
-static void cheb_interval(double omega, double& theta, double& delta) {
-  const double hi = 1.05 * 4.0 / (3.0 * 1.05 * omega), lo = hi / kChebSmoothRatio;
+static void cheb_interval(double omega, double ratio, double& theta, double& delta) {
+  const double hi = 1.05 * 4.0 / (3.0 * 1.05 * omega), lo = hi / ratio;
   theta = 0.5 * (hi + lo);
   delta = 0.5 * (hi - lo);
 }
@@ -570,7 +569,7 @@ int Mg::sweep(int l, int mode, const double* x, double* xo, double c1, double c2
 
 // Smoothing: ns damped block-Jacobi sweeps (x += w B (b - A x)), or, by
 // default, ns steps of the Chebyshev semi-iteration for B A x = B b on
-// [hi / 16, hi] (the first step d = B (b - A x) / theta, then
+// [hi / cheb_ratio, hi] (the first step d = B (b - A x) / theta, then
 // d = rho_k rho_{k-1} d + (2 rho_k / delta) B (b - A x), rho_k = 1 / (2 sigma
 // - rho_{k-1}), rho_0 = 1 / sigma; x += d): the same polynomial before and
 // after the coarse correction keeps the cycle symmetric.  The sweeps
@@ -590,7 +589,7 @@ int Mg::cycle(int l) {
   int rc;
   const int ns = l == 0 ? nu : nuc;
   double theta = 1.0, delta = 1.0;
-  if (!jacobi) cheb_interval(L.omega, theta, delta);
+  if (!jacobi) cheb_interval(L.omega, cheb_ratio, theta, delta);
   const double sigma = theta / delta;
   // x ping-pongs between L.x2 and L.x (2 ns - 1 steps end in L.x); a
   // Chebyshev step reads the previous iterate from the buffer it overwrites
