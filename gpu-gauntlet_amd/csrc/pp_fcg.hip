@@ -316,8 +316,15 @@ static int fcg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st,
     hs.converged = 1;
   } else {
     for (int k = 0; k <= kMaxRestarts; k++) {
-      if ((rc = fcg_run(pl, prec, m, fnorm, o->tolerance, o->max_iterations - total, &hs, S,
-                        part, ring.p, ring.p + (size_t)R * n)))
+      // from the second restart on, aim the recursive residual at half the
+      // tolerance: stopped just under it, restarts near the FP64 floor took one
+      // iteration each and left the true residual where it was (1.01e-8 to
+      // 1.04e-8 on the 1024^2 log-normal plate); the first restart keeps the
+      // tolerance (with half of it the linear preconditioners took 25% more
+      // iterations there)
+      const double tol = k <= 1 ? o->tolerance : 0.5 * o->tolerance;
+      if ((rc = fcg_run(pl, prec, m, fnorm, tol, o->max_iterations - total, &hs, S, part, ring.p,
+                        ring.p + (size_t)R * n)))
         return rc;
       if (sweeps < PP_MAX_SWEEPS) st->sweep_iterations[sweeps] = hs.it;
       total += hs.it;
@@ -355,9 +362,12 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
 // PCG with one Galerkin multigrid V-cycle per iteration (symmetric and
 // linear, so FCG(1) is plain PCG); the hierarchy is built on the first such
 // solve of the plan, with level 0 smoothing at w = 4 / (3 1.05 lambda_max(B K)),
-// o->poly_degree Chebyshev steps on level 0 (0: 6) on [lambda_max / 32, lambda_max].
+// o->poly_degree Chebyshev steps on level 0 on [lambda_max / 32, lambda_max].
+// The default steps were measured: 6 at 142^2 (46 iterations, 6.8 ms; 4: 56,
+// 8.1 ms), 5 at 700^2 (64.1 ms; 6: 68.2) and at 1024^2 (120.9 ms; 4: 123.7,
+// 6: 133.8): launch-bound small plates take more steps per V-cycle.
 int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
-  const int nu = o->poly_degree > 0 ? o->poly_degree : 6;
+  const int nu = o->poly_degree > 0 ? o->poly_degree : (pl->n_nodes < 100000 ? 6 : 5);
   if (nu > 8) return fail(PP_ERR_INVALID, "multigrid: poly_degree (smoothing steps) must be at most 8");
   int rc;
   if (!pl->mg_pc || pl->mg_pc->nu != nu) {

@@ -39,7 +39,7 @@ export interface GPUSolverOptions {
    *  multigrid: one Galerkin multigrid V-cycle per PCG iteration (structured FP64 path) */
   preconditioner?: 'blockJacobi' | 'learned' | 'chebyshev' | 'multigrid';
   /** chebyshev: polynomial degree (default 8, at most 32); multigrid: smoothing steps on the
-   *  finest level (default 6, at most 8) */
+   *  finest level (default 6 below 100,000 nodes, else 5; at most 8) */
   polyDegree?: number;
   path?: 'auto' | 'structured' | 'coloured';
   /** mixed: a refinement sweep ends when the FP32 residual has fallen by this factor (default 1e-3) */
