@@ -11,7 +11,6 @@ namespace pp {
 
 namespace {
 thread_local std::string g_last_error;
-constexpr int kNormBlocks = 256;
 
 __global__ __launch_bounds__(256) void k_resid_parts(int64_t n, const double* F, const double* Kx,
                                                      double* part) {
@@ -105,14 +104,19 @@ int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* 
 }
 
 int launch_norms(pp_plan* pl, const double* dF, const double* dKx, double* res, double* fnorm) {
-  hipStream_t s = pl->ctx->stream;
   int rc;
-  if (pl->normp.n < 2 * kNormBlocks && (rc = pl->normp.alloc(2 * kNormBlocks))) return rc;
+  if (pl->normp.n < kNormParts && (rc = pl->normp.alloc(kNormParts))) return rc;
+  return launch_norms_into(pl, pl->normp.p, dF, dKx, res, fnorm);
+}
+
+int launch_norms_into(pp_plan* pl, double* part, const double* dF, const double* dKx, double* res,
+                      double* fnorm) {
+  hipStream_t s = pl->ctx->stream;
   hipLaunchKernelGGL(k_resid_parts, dim3(kNormBlocks), dim3(256), 0, s, (int64_t)pl->n_dof, dF, dKx,
-                     pl->normp.p);
+                     part);
   PP_HIP(hipGetLastError());
-  double h[2 * kNormBlocks];
-  PP_HIP(hipMemcpyAsync(h, pl->normp.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  double h[kNormParts];
+  PP_HIP(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s));
   PP_HIP(hipStreamSynchronize(s));
   double a = 0, b = 0;
   for (int i = 0; i < kNormBlocks; i++) {
@@ -137,12 +141,15 @@ void ctx_release(pp_ctx* c) {
 
 // A plan is freed when its owner and every network made on it have let go.
 void plan_release(pp_plan* pl) {
-  if (--pl->refs > 0) return;
   pp_ctx* c = pl->ctx;
-  (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  mg_destroy(pl->mg_pc);
-  delete pl;
+  {
+    PP_LOCK(c);
+    if (--pl->refs > 0) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    mg_destroy(pl->mg_pc);
+    delete pl;
+  }
   ctx_release(c);
 }
 
@@ -202,6 +209,7 @@ int pp_plan_create(pp_ctx* ctx, const pp_mesh* m, const pp_material* mat,
   const size_t ncon = (size_t)m->element_count * m->nodes_per_element;
   for (size_t k = 0; k < ncon; k++)
     if (m->elements[k] >= m->node_count) return fail(PP_ERR_INVALID, "element node index out of range");
+  PP_LOCK(ctx);
   PP_HIP(hipSetDevice(ctx->device));
   pp_plan* pl = new pp_plan();
   pl->ctx = ctx;
@@ -239,9 +247,15 @@ int pp_plan_create(pp_ctx* ctx, const pp_mesh* m, const pp_material* mat,
   // colour offsets (the coloured path is always available for parity tests)
   if (col && col->colour_count > 0 && col->colour_offsets && col->elements_by_colour) {
     pl->h_colour_off.assign(col->colour_offsets, col->colour_offsets + col->colour_count + 1);
-    if (pl->h_colour_off.back() != m->element_count) {
+    // offsets must run from 0 up to element_count without decreasing before
+    // any loop below indexes elements_by_colour with them
+    bool ordered = pl->h_colour_off.front() == 0 && pl->h_colour_off.back() == m->element_count;
+    for (uint32_t c = 0; ordered && c < col->colour_count; c++)
+      ordered = pl->h_colour_off[c] <= pl->h_colour_off[c + 1];
+    if (!ordered) {
       delete pl;
-      return fail(PP_ERR_INVALID, "colouring does not cover every element exactly once");
+      return fail(PP_ERR_INVALID,
+                  "colour offsets must start at 0, never decrease and end at element_count");
     }
     std::vector<uint8_t> seen(m->element_count, 0);
     for (uint32_t k = 0; k < m->element_count; k++) {
@@ -290,6 +304,7 @@ int pp_plan_is_structured(const pp_plan* pl) { return pl && pl->structured ? 1 :
 
 int pp_set_rhs(pp_plan* pl, const float* F) {
   if (!pl || !F) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(pl->ctx);
   hipStream_t s = pl->ctx->stream;
   PP_HIP(hipMemcpyAsync(pl->F32.p, F, pl->n_dof * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_f32_to_f64, dim3(nblocks(pl->n_dof, 256)), dim3(256), 0, s,
@@ -317,6 +332,7 @@ static int pick_path(pp_plan* pl, int path, bool* structured) {
 
 int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   if (!pl || !o || !st) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(pl->ctx);
   if (!pl->rhs_set) return fail(PP_ERR_INVALID, "no right-hand side set");
   if (!(o->tolerance > 0) || o->max_iterations < 0) return fail(PP_ERR_INVALID, "bad tolerance/max_iterations");
   if (o->precond != PP_PRECOND_BLOCK_JACOBI && o->precond != PP_PRECOND_LEARNED &&
@@ -361,6 +377,7 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
 
 int pp_get_solution(pp_plan* pl, double* x64, float* x32) {
   if (!pl) return fail(PP_ERR_INVALID, "NULL plan");
+  PP_LOCK(pl->ctx);
   hipStream_t s = pl->ctx->stream;
   if (x64) PP_HIP(hipMemcpyAsync(x64, pl->x.p, pl->n_dof * 8, hipMemcpyDeviceToHost, s));
   if (x32) {
@@ -379,6 +396,8 @@ int pp_get_solution(pp_plan* pl, double* x64, float* x32) {
 
 int pp_solve(pp_plan* pl, const float* F, const pp_solve_opts* o, float* x32, double* x64,
              pp_solve_stats* st) {
+  if (!pl) return fail(PP_ERR_INVALID, "NULL plan");
+  PP_LOCK(pl->ctx);
   int rc = pp_set_rhs(pl, F);
   if (rc) return rc;
   rc = pp_solve_resident(pl, o, st);
@@ -401,6 +420,7 @@ static int with_device_vectors(pp_plan* pl, const double* in, double* out,
 
 int pp_apply_k(pp_plan* pl, const double* x, double* y, int path) {
   if (!pl || !x || !y) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(pl->ctx);
   bool structured = false;
   int rc = pick_path(pl, path, &structured);
   if (rc) return rc;
@@ -414,6 +434,7 @@ int pp_apply_k(pp_plan* pl, const double* x, double* y, int path) {
 
 int pp_apply_k_colour(pp_plan* pl, int colour, const double* x, double* y) {
   if (!pl || !x || !y) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(pl->ctx);
   if (pl->h_colour_off.empty()) return fail(PP_ERR_UNSUPPORTED, "plan has no colouring");
   if (colour < 0 || colour + 1 >= (int)pl->h_colour_off.size())
     return fail(PP_ERR_INVALID, "colour out of range");
@@ -427,6 +448,7 @@ int pp_apply_k_colour(pp_plan* pl, int colour, const double* x, double* y) {
 
 int pp_apply_precond(pp_plan* pl, const double* r, double* z) {
   if (!pl || !r || !z) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(pl->ctx);
   return with_device_vectors(
       pl, r, z,
       [](pp_plan* p, const double* a, double* b, void*) {
@@ -437,6 +459,7 @@ int pp_apply_precond(pp_plan* pl, const double* r, double* z) {
 
 int pp_get_block_inverse(pp_plan* pl, double* blocks) {
   if (!pl || !blocks) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(pl->ctx);
   std::vector<double> m(6 * (size_t)pl->n_nodes);
   PP_HIP(hipMemcpyAsync(m.data(), pl->minv.p, m.size() * 8, hipMemcpyDeviceToHost, pl->ctx->stream));
   PP_HIP(hipStreamSynchronize(pl->ctx->stream));
@@ -453,6 +476,7 @@ int pp_get_block_inverse(pp_plan* pl, double* blocks) {
 int pp_pressure_load(pp_plan* pl, double q0, const double* patches, int n_patches, float* F) {
   if (!pl || !F || n_patches < 0 || (n_patches > 0 && !patches))
     return fail(PP_ERR_INVALID, "bad argument");
+  PP_LOCK(pl->ctx);
   DevBuf<double> pt;
   DevBuf<float> f;
   int rc;

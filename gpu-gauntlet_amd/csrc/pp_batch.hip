@@ -523,7 +523,7 @@ struct BatchPlate {
   std::vector<double> ax, by;  // per column (a, 1/a), per row (b, 1/b)
   std::vector<double> kcell;
   int64_t dof_off = 0, node_off = 0, cls_off = 0, k_off = 0, geo_off = 0;
-  int shape = 0, nwg = 1, last_iterations = 0;
+  int shape = 0, nwg = 1, last_iterations = 0, last_converged = 0;
 };
 
 struct pp_batch {
@@ -710,8 +710,9 @@ int apply_schedule(pp_batch* b, const pp_batch_schedule* s) {
   return PP_OK;
 }
 
-// |F - K x| / |F| of every plate into b->norms; returns the worst.
-int true_residuals(pp_batch* b, double* worst) {
+// |F - K x| / |F| of every plate into b->norms (and rel, when given);
+// returns the worst.
+int true_residuals(pp_batch* b, double* worst, std::vector<double>* rel = nullptr) {
   hipStream_t s = b->ctx->stream;
   const int n = (int)b->P.size();
   hipLaunchKernelGGL(k_batch_apply, dim3(b->tile_map.n), dim3(kNT), 0, s,
@@ -724,13 +725,19 @@ int true_residuals(pp_batch* b, double* worst) {
   PP_HIP(hipMemcpyAsync(h.data(), b->norms.p, 2 * n * 8, hipMemcpyDeviceToHost, s));
   PP_HIP(hipStreamSynchronize(s));
   *worst = 0.0;
-  for (int p = 0; p < n; p++)
-    *worst = std::max(*worst, h[2 * p + 1] > 0 ? sqrt(h[2 * p] / h[2 * p + 1]) : 0.0);
+  if (rel) rel->assign(n, 0.0);
+  for (int p = 0; p < n; p++) {
+    const double v = h[2 * p + 1] > 0 ? sqrt(h[2 * p] / h[2 * p + 1]) : 0.0;
+    *worst = std::max(*worst, v);
+    if (rel) (*rel)[p] = v;
+  }
   return PP_OK;
 }
 
-// One persistent launch (cold, or restart from x with r0 = F - K x in tmp).
-int launch(pp_batch* b, const pp_solve_opts* o, bool restart, std::vector<DevStats>& ds) {
+// One persistent launch (cold, or restart from x with r0 = F - K x in tmp)
+// of at most maxit iterations per plate.
+int launch(pp_batch* b, const pp_solve_opts* o, int maxit, bool restart,
+           std::vector<DevStats>& ds) {
   hipStream_t s = b->ctx->stream;
   const int n = (int)b->P.size();
   hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(n, 256)), dim3(256), 0, s, b->gs.p, b->st.p, n);
@@ -739,11 +746,11 @@ int launch(pp_batch* b, const pp_solve_opts* o, bool restart, std::vector<DevSta
   if (b->sched.resident_plates)
     hipLaunchKernelGGL(k_batch_pcg<true>, dim3(b->total_wg), dim3(kNT), 0, s,
                        (const PlateDesc*)b->descs.p, (const int*)b->wg_plate.p, o->tolerance,
-                       o->max_iterations, r0, (const int64_t*)b->dof_off.p);
+                       maxit, r0, (const int64_t*)b->dof_off.p);
   else
     hipLaunchKernelGGL(k_batch_pcg<false>, dim3(b->total_wg), dim3(kNT), 0, s,
                        (const PlateDesc*)b->descs.p, (const int*)b->wg_plate.p, o->tolerance,
-                       o->max_iterations, r0, (const int64_t*)b->dof_off.p);
+                       maxit, r0, (const int64_t*)b->dof_off.p);
   PP_HIP(hipGetLastError());
   std::vector<GridSync> gs(n);
   ds.resize(n);
@@ -759,7 +766,10 @@ int launch(pp_batch* b, const pp_solve_opts* o, bool restart, std::vector<DevSta
 // Solve every plate; with the true-residual check, plates whose FP64
 // |F - K x| / |F| is above tol after the recurrence has converged are
 // restarted from x with that residual (as the single-plate FP64 path), at
-// most four times.  The device time covers all launches and checks.
+// most four times, within what is left of max_iterations (counted for the
+// plate that has used the most).  A plate counts as converged when its
+// recurrence converged and, with the check, its true residual is at most
+// tol.  The device time covers all launches and checks.
 int batch_run(pp_batch* b, const pp_solve_opts* o, pp_batch_stats* out) {
   constexpr int kMaxRestarts = 4;
   hipStream_t s = b->ctx->stream;
@@ -768,7 +778,7 @@ int batch_run(pp_batch* b, const pp_solve_opts* o, pp_batch_stats* out) {
   PP_HIP(hipEventRecord(b->ctx->ev0, s));
   std::vector<DevStats> ds;
   int rc;
-  if ((rc = launch(b, o, false, ds))) return rc;
+  if ((rc = launch(b, o, o->max_iterations, false, ds))) return rc;
   std::vector<int> its(n), conv(n);
   for (int p = 0; p < n; p++) {
     its[p] = ds[p].iterations;
@@ -776,18 +786,22 @@ int batch_run(pp_batch* b, const pp_solve_opts* o, pp_batch_stats* out) {
   }
   double worst = 0.0;
   if (check) {
-    if ((rc = true_residuals(b, &worst))) return rc;
+    std::vector<double> rel;
+    if ((rc = true_residuals(b, &worst, &rel))) return rc;
     for (int k = 0; k < kMaxRestarts && worst > o->tolerance; k++) {
+      const int left = o->max_iterations - *std::max_element(its.begin(), its.end());
+      if (left <= 0) break;
       hipLaunchKernelGGL(k_batch_residual, dim3(ceil_div((int)std::min<int64_t>(b->total_dof, 1 << 30), 256)),
                          dim3(256), 0, s, b->total_dof, (const double*)b->F.p,
                          (const double*)b->tmp.p, b->tmp.p);
-      if ((rc = launch(b, o, true, ds))) return rc;
+      if ((rc = launch(b, o, left, true, ds))) return rc;
       for (int p = 0; p < n; p++) {
         its[p] += ds[p].iterations;
         conv[p] = ds[p].converged;
       }
-      if ((rc = true_residuals(b, &worst))) return rc;
+      if ((rc = true_residuals(b, &worst, &rel))) return rc;
     }
+    for (int p = 0; p < n; p++) conv[p] = conv[p] && rel[p] <= o->tolerance;
   }
   PP_HIP(hipEventRecord(b->ctx->ev1, s));
   PP_HIP(hipEventSynchronize(b->ctx->ev1));
@@ -797,6 +811,7 @@ int batch_run(pp_batch* b, const pp_solve_opts* o, pp_batch_stats* out) {
   int64_t total = 0;
   for (int p = 0; p < n; p++) {
     b->P[p].last_iterations = its[p];
+    b->P[p].last_converged = conv[p];
     nconv += conv[p];
     maxit = std::max(maxit, its[p]);
     total += its[p];
@@ -999,6 +1014,7 @@ int32_t pp_batch_plate_count(const pp_batch* b) { return b ? (int32_t)b->P.size(
 
 int pp_batch_set_rhs(pp_batch* b, int32_t plate, const float* F) {
   if (!b || !F || plate < 0 || plate >= (int)b->P.size()) return fail(PP_ERR_INVALID, "bad argument");
+  PP_LOCK(b->ctx);
   const BatchPlate& q = b->P[plate];
   const int64_t n = 3 * (int64_t)(q.nx + 1) * (q.ny + 1);
   std::vector<double> h(F, F + n);
@@ -1012,6 +1028,7 @@ int pp_batch_pressure_load(pp_batch* b, int32_t plate, double q0, const double* 
   if (!b || !F || plate < 0 || plate >= (int)b->P.size() || n_patches < 0 ||
       (n_patches > 0 && !patches))
     return fail(PP_ERR_INVALID, "bad argument");
+  PP_LOCK(b->ctx);
   const BatchPlate& q = b->P[plate];
   const int64_t nn = (int64_t)(q.nx + 1) * (q.ny + 1);
   return mesh_pressure(q.nodes.data(), q.elems.data(), q.npe, nn, (int64_t)q.elems.size() / q.npe,
@@ -1020,12 +1037,14 @@ int pp_batch_pressure_load(pp_batch* b, int32_t plate, double q0, const double* 
 
 int pp_batch_set_schedule(pp_batch* b, const pp_batch_schedule* s) {
   if (!b || !s) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(b->ctx);
   PP_HIP(hipSetDevice(b->ctx->device));
   return apply_schedule(b, s);
 }
 
 int pp_batch_get_schedule(const pp_batch* b, pp_batch_schedule* s, int32_t* wg_per_plate) {
   if (!b || !s) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(b->ctx);
   *s = b->sched;
   if (wg_per_plate)
     for (size_t p = 0; p < b->P.size(); p++) wg_per_plate[p] = b->P[p].nwg;
@@ -1034,6 +1053,7 @@ int pp_batch_get_schedule(const pp_batch* b, pp_batch_schedule* s, int32_t* wg_p
 
 int pp_batch_solve(pp_batch* b, const pp_solve_opts* o, pp_batch_stats* st) {
   if (!b || !o || !st) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(b->ctx);
   if (!(o->tolerance > 0) || o->max_iterations < 0) return fail(PP_ERR_INVALID, "bad tolerance/max_iterations");
   memset(st, 0, sizeof(*st));
   PP_HIP(hipSetDevice(b->ctx->device));
@@ -1043,6 +1063,7 @@ int pp_batch_solve(pp_batch* b, const pp_solve_opts* o, pp_batch_stats* st) {
 int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* best,
                       pp_batch_stats* best_stats, int32_t* trials) {
   if (!b || !o || !best) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(b->ctx);
   PP_HIP(hipSetDevice(b->ctx->device));
   pp_solve_opts q = *o;
   q.flags |= PP_FLAG_NO_TRUE_RESIDUAL;
@@ -1112,6 +1133,7 @@ int pp_batch_autotune(pp_batch* b, const pp_solve_opts* o, pp_batch_schedule* be
 
 int pp_batch_plate_result(pp_batch* b, int32_t plate, double* x64, float* x32, pp_solve_stats* st) {
   if (!b || plate < 0 || plate >= (int)b->P.size()) return fail(PP_ERR_INVALID, "bad argument");
+  PP_LOCK(b->ctx);
   const BatchPlate& q = b->P[plate];
   const int64_t n = 3 * (int64_t)(q.nx + 1) * (q.ny + 1);
   hipStream_t s = b->ctx->stream;
@@ -1128,7 +1150,7 @@ int pp_batch_plate_result(pp_batch* b, int32_t plate, double* x64, float* x32, p
   if (st) {
     memset(st, 0, sizeof(*st));
     st->iterations = b->P[plate].last_iterations;
-    st->converged = ds.converged;
+    st->converged = b->P[plate].last_converged;
     st->breakdown = ds.breakdown;
     st->sweeps = 1;
     st->final_residual = ds.final_residual;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -85,8 +87,15 @@ struct pp_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int cu_count = 0;
-  int refs = 1;  // the owner plus one per live plan
+  std::atomic<int> refs{1};  // the owner plus one per live plan or batch
+  // Every entry point that works on this context's stream, events or the
+  // state of its plans holds this lock, so calls from several host threads
+  // (the Node addon runs solves on the libuv pool) are serialised per context.
+  std::recursive_mutex mu;
 };
+
+// Holds a context's lock for the rest of the scope.
+#define PP_LOCK(ctx) std::lock_guard<std::recursive_mutex> pp_lock_guard_((ctx)->mu)
 
 struct pp_plan {
   pp_ctx* ctx = nullptr;
@@ -187,7 +196,35 @@ void element_ke_dkt(const double* xy, double E, double nu, double t, double* Ke)
 void ctx_release(pp_ctx* c);
 void plan_release(pp_plan* pl);
 
+// Buffers of the structured FP64 solve: right-hand side, solution, the two
+// exchange vectors of the persistent kernel, a DOF vector for the true
+// residual, grid-reduction partials (part_n doubles), norm partials
+// (kNormParts doubles), the grid barrier words and the device statistics.
+// From the plan (pp_solve_resident) or from the no-allocation scratch array.
+struct StructWs {
+  const double* F;
+  double* x;
+  double *pb0, *pb1, *tmp;
+  double* part;
+  size_t part_n;
+  double* normp;
+  GridSync* gs;
+  DevStats* ds;
+};
+constexpr int kNormBlocks = 256;
+constexpr int kNormParts = 2 * kNormBlocks;
+// The FP64 structured solve on explicit buffers; resident_only: fail with
+// PP_ERR_UNSUPPORTED unless the single-reduction resident kernel is used
+// (the only one that needs no further plan vectors).
+int struct_solve_fp64_ws(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st,
+                         const StructWs& ws, bool resident_only);
+// Grid-reduction partials the persistent kernel of this plan needs, in doubles.
+int struct_partials_needed(pp_plan* pl, int variant, size_t* n);
+
 // shared helpers (pp_api.hip)
+// launch_norms with the partials in part (kNormParts doubles)
+int launch_norms_into(pp_plan* pl, double* part, const double* dF, const double* dKx, double* res,
+                      double* fnorm);
 int launch_norm_ratio(pp_plan* pl, const double* dF, const double* dKx, double* rel);
 // |F - Kx| and |F| (FP64, fixed-order reduction)
 int launch_norms(pp_plan* pl, const double* dF, const double* dKx, double* res, double* fnorm);

@@ -251,6 +251,7 @@ extern "C" {
 
 int pp_moments(pp_plan* pl, const double* x64, float* Mx, float* My, float* Mxy) {
   if (!pl) return fail(PP_ERR_INVALID, "NULL plan");
+  PP_LOCK(pl->ctx);
   PP_HIP(hipSetDevice(pl->ctx->device));
   const double* dx = nullptr;
   int rc;
@@ -270,6 +271,7 @@ int pp_moments(pp_plan* pl, const double* x64, float* Mx, float* My, float* Mxy)
 int pp_max_displacement(pp_plan* pl, const double* x64, double* w, int64_t* node, float* x,
                         float* y) {
   if (!pl) return fail(PP_ERR_INVALID, "NULL plan");
+  PP_LOCK(pl->ctx);
   if (pl->n_nodes <= 0) return fail(PP_ERR_INVALID, "plan has no nodes");
   PP_HIP(hipSetDevice(pl->ctx->device));
   const double* dx = nullptr;

@@ -1029,6 +1029,15 @@ __global__ void k_zero_sync(GridSync* gs) {
   }
 }
 
+// Clears what a persistent launch needs at zero (the statistics block and
+// the tagged all-reduce slots) with stores, not hipMemsetAsync, so that the
+// launch sequence holds no memset node if a caller captures it.
+__global__ void k_zero_launch(DevStats* st, double* part, int64_t n_part) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_part) part[i] = 0.0;
+  if (i == 0) *st = DevStats{};
+}
+
 // Launch shape of the persistent PCG for this plan.
 // Tile of the resident single-reduction kernel and of the strided kernel.
 // Measured on MI355X (us per iteration): 143^2 nodes 5.46 on 18x18 tiles vs
@@ -1101,30 +1110,33 @@ int pcg_shape(pp_plan* pl, int variant, PcgShape* sh) {
 // the barrier spin budget runs out and the call fails with PP_ERR_TIMEOUT
 // instead of hanging.  (A cooperative launch would refuse to run at all
 // while another process has the device open.)
-int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxit,
-               int variant, DevStats* ds) {
+int launch_pcg(pp_plan* pl, const StructWs& ws, const double* r0, double bnorm, double tol,
+               int maxit, int variant, bool resident_only, DevStats* ds) {
   hipStream_t s = pl->ctx->stream;
   PcgShape sh;
   int rc = pcg_shape(pl, variant, &sh);
   if (rc) return rc;
-  if (pl->partials.n < (size_t)8 * sh.grid && (rc = pl->partials.alloc(8 * (size_t)sh.grid)))
-    return rc;
+  if (resident_only && sh.kind != 3)
+    return fail(PP_ERR_UNSUPPORTED,
+                "this entry point runs the single-reduction PCG with every 18x18 tile resident");
+  const size_t npart = (size_t)8 * sh.grid;
+  if (ws.part_n < npart) return fail(PP_ERR_INVALID, "grid-reduction partials buffer too small");
   PcgArgs A;
   A.G = make_kgrid(pl);
-  A.F = pl->F.p;
+  A.F = ws.F;
   A.r0 = r0;
   A.bnorm_in = bnorm;
-  A.x = pl->x.p; A.r = pl->r.p; A.z = pl->z.p; A.pb0 = pl->p0.p; A.pb1 = pl->p1.p;
+  A.x = ws.x; A.r = pl->r.p; A.z = pl->z.p; A.pb0 = ws.pb0; A.pb1 = ws.pb1;
   A.Ap = pl->Ap.p;
   A.minv = pl->minv.p;
-  A.part = pl->partials.p;
-  A.gs = pl->gsync.p;
-  A.st = pl->dstats.p;
+  A.part = ws.part;
+  A.gs = ws.gs;
+  A.st = ws.ds;
   A.tol = tol;
   A.maxit = maxit;
-  hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, s, pl->gsync.p);
-  PP_HIP(hipMemsetAsync(pl->dstats.p, 0, sizeof(DevStats), s));
-  PP_HIP(hipMemsetAsync(pl->partials.p, 0, 8 * (size_t)sh.grid * sizeof(double), s));
+  hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, s, ws.gs);
+  hipLaunchKernelGGL(k_zero_launch, dim3(grid_blocks((int64_t)npart, 256)), dim3(256), 0, s, ws.ds,
+                     ws.part, (int64_t)npart);
   if (sh.kind == 0)
     hipLaunchKernelGGL((k_pcg_res<CfgX>), dim3(sh.grid), dim3(CfgX::NT), 0, s, A);
   else if (sh.kind == 1)
@@ -1135,8 +1147,8 @@ int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxi
     hipLaunchKernelGGL((k_pcg_cg<CfgR>), dim3(sh.grid), dim3(CfgR::NT), 0, s, A);
   else {
     // tag 0 never matches: stale words of an earlier launch cannot pass
-    PP_HIP(hipMemsetAsync(pl->p0.p, 0, pl->n_dof * sizeof(double), s));
-    PP_HIP(hipMemsetAsync(pl->p1.p, 0, pl->n_dof * sizeof(double), s));
+    PP_HIP(hipMemsetAsync(ws.pb0, 0, pl->n_dof * sizeof(double), s));
+    PP_HIP(hipMemsetAsync(ws.pb1, 0, pl->n_dof * sizeof(double), s));
     // interval [lmax / kChebRatio, lmax], lmax with a 5% margin over the
     // power-iteration estimate (q_d stays positive up to lmax)
     constexpr double kChebRatio = 100.0;
@@ -1149,8 +1161,8 @@ int launch_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxi
   }
   PP_HIP(hipGetLastError());
   GridSync gsh;
-  PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
-  PP_HIP(hipMemcpyAsync(&gsh, pl->gsync.p, sizeof(gsh), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipMemcpyAsync(ds, ws.ds, sizeof(*ds), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipMemcpyAsync(&gsh, ws.gs, sizeof(gsh), hipMemcpyDeviceToHost, s));
   PP_HIP(hipStreamSynchronize(s));
   if (gsh.abort) return fail(PP_ERR_TIMEOUT, "grid barrier spin limit reached");
   return PP_OK;
@@ -1269,22 +1281,46 @@ int cheb_lambda_max(pp_plan* pl) {
   return PP_OK;
 }
 
+// 0: single-reduction kernel; 1: two-reduction kernel; 2: two-reduction on
+// small tiles; 3: single-reduction with the Chebyshev preconditioner
+static int solve_variant(const pp_solve_opts* o) {
+  if (o->precond == PP_PRECOND_CHEBYSHEV) return 3;
+  return (o->flags & PP_FLAG_SMALL_TILES) ? 2 : (o->flags & PP_FLAG_CLASSIC_PCG) ? 1 : 0;
+}
+
+int struct_partials_needed(pp_plan* pl, int variant, size_t* n) {
+  PcgShape sh;
+  int rc = pcg_shape(pl, variant, &sh);
+  if (rc) return rc;
+  *n = (size_t)8 * sh.grid;
+  return PP_OK;
+}
+
 int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  int rc;
+  size_t npart = 0;
+  if ((rc = struct_partials_needed(pl, solve_variant(o), &npart))) return rc;
+  if (pl->partials.n < npart && (rc = pl->partials.alloc(npart))) return rc;
+  if (pl->normp.n < kNormParts && (rc = pl->normp.alloc(kNormParts))) return rc;
+  StructWs ws{pl->F.p, pl->x.p, pl->p0.p, pl->p1.p, pl->tmp.p, pl->partials.p, pl->partials.n,
+              pl->normp.p, pl->gsync.p, pl->dstats.p};
+  return struct_solve_fp64_ws(pl, o, st, ws, false);
+}
+
+int struct_solve_fp64_ws(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st,
+                         const StructWs& ws, bool resident_only) {
   constexpr int kMaxRestarts = 4;
   hipStream_t s = pl->ctx->stream;
   int rc;
-  // 0: single-reduction kernel; 1: two-reduction kernel; 2: two-reduction on
-  // small tiles; 3: single-reduction with the Chebyshev preconditioner
-  int dw = (o->flags & PP_FLAG_SMALL_TILES) ? 2 : (o->flags & PP_FLAG_CLASSIC_PCG) ? 1 : 0;
+  const int dw = solve_variant(o);
   if (o->precond == PP_PRECOND_CHEBYSHEV) {
-    dw = 3;
     pl->cheb_degree = o->poly_degree > 0 ? o->poly_degree : 8;
     if (pl->cheb_degree > 32) return fail(PP_ERR_INVALID, "poly_degree must be at most 32");
     if (pl->cheb_lmax <= 0 && (rc = cheb_lambda_max(pl))) return rc;
   }
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   DevStats ds;
-  rc = launch_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, dw, &ds);
+  rc = launch_pcg(pl, ws, nullptr, 0.0, o->tolerance, o->max_iterations, dw, resident_only, &ds);
   if (rc) return rc;
   const double bnorm = ds.bnorm;
   int total = ds.iterations;
@@ -1292,14 +1328,17 @@ int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   st->sweeps = 1;
   st->sweep_iterations[0] = ds.iterations;
   double rel = 0;
-  if (ds.converged && bnorm > 0) {
+  if (ds.converged && bnorm > 0 && !(o->flags & PP_FLAG_NO_TRUE_RESIDUAL)) {
     for (int k = 0;; k++) {
-      if ((rc = struct_apply_k(pl, pl->x.p, pl->tmp.p))) return rc;
-      if ((rc = launch_norm_ratio(pl, pl->F.p, pl->tmp.p, &rel))) return rc;
+      if ((rc = struct_apply_k(pl, ws.x, ws.tmp))) return rc;
+      double res = 0, fn = 0;
+      if ((rc = launch_norms_into(pl, ws.normp, ws.F, ws.tmp, &res, &fn))) return rc;
+      rel = fn > 0 ? res / fn : 0.0;
       st->sweep_residual[std::min(st->sweeps, PP_MAX_SWEEPS) - 1] = rel;
       if (rel <= o->tolerance || k == kMaxRestarts || total >= o->max_iterations) break;
-      if ((rc = launch_residual(pl, pl->F.p, pl->tmp.p, pl->tmp.p))) return rc;
-      rc = launch_pcg(pl, pl->tmp.p, bnorm, o->tolerance, o->max_iterations - total, dw, &ds);
+      if ((rc = launch_residual(pl, ws.F, ws.tmp, ws.tmp))) return rc;
+      rc = launch_pcg(pl, ws, ws.tmp, bnorm, o->tolerance, o->max_iterations - total, dw,
+                      resident_only, &ds);
       if (rc) return rc;
       total += ds.iterations;
       for (int k = 0; k < 6; k++) st->phase_us[k] += 0.01 * (double)ds.ticks[k];

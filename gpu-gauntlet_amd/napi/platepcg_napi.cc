@@ -85,6 +85,34 @@ napi_value make_num(napi_env env, double v) {
   return out;
 }
 
+// A plan as JavaScript holds it.  Solves run on the libuv pool; destroyPlan
+// during one (or more) of them only marks the handle, and the last solve to
+// complete frees the plan, so no worker ever uses a freed plan.  The C ABI
+// itself serialises calls per context, so concurrent solves on one plan or
+// one context are safe and each returns its own answer.
+struct PlanHandle {
+  pp_plan* plan = nullptr;
+  int inflight = 0;        // solves queued or running (touched on the JS thread only)
+  bool destroyed = false;  // destroyPlan was called
+};
+
+PlanHandle* plan_handle(napi_env env, napi_value v) {
+  PlanHandle* h = (PlanHandle*)external_ptr(env, v);
+  return h && !h->destroyed ? h : nullptr;
+}
+
+pp_plan* plan_of(napi_env env, napi_value v) {
+  PlanHandle* h = plan_handle(env, v);
+  return h ? h->plan : nullptr;
+}
+
+void handle_release_if_done(PlanHandle* h) {
+  if (h->destroyed && h->inflight == 0) {
+    pp_plan_destroy(h->plan);
+    delete h;
+  }
+}
+
 // ----------------------------------------------------------- functions
 
 napi_value DeviceCount(napi_env env, napi_callback_info) { return make_int(env, pp_device_count()); }
@@ -191,8 +219,10 @@ napi_value CreatePlan(napi_env env, napi_callback_info info) {
   pp_plan* plan = nullptr;
   int rc = pp_plan_create(ctx, &m, &mt, cp, mask, &plan);
   if (rc) return throw_pp(env, rc);
+  PlanHandle* h = new PlanHandle();
+  h->plan = plan;
   napi_value out;
-  NAPI_CALL(env, napi_create_external(env, plan, nullptr, nullptr, &out));
+  NAPI_CALL(env, napi_create_external(env, h, nullptr, nullptr, &out));
   return out;
 }
 
@@ -200,7 +230,10 @@ napi_value DestroyPlan(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  pp_plan_destroy((pp_plan*)external_ptr(env, argv[0]));
+  PlanHandle* h = plan_handle(env, argv[0]);
+  if (!h) return nullptr;  // already destroyed
+  h->destroyed = true;
+  handle_release_if_done(h);
   return nullptr;
 }
 
@@ -209,7 +242,7 @@ napi_value IsStructured(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   napi_value out;
-  napi_get_boolean(env, pp_plan_is_structured((pp_plan*)external_ptr(env, argv[0])) != 0, &out);
+  napi_get_boolean(env, pp_plan_is_structured(plan_of(env, argv[0])) != 0, &out);
   return out;
 }
 
@@ -219,6 +252,7 @@ struct SolveWork {
   napi_async_work work = nullptr;
   napi_deferred deferred = nullptr;
   napi_ref fref = nullptr;
+  PlanHandle* handle = nullptr;  // counted in handle->inflight until complete
   pp_plan* plan = nullptr;
   const float* F = nullptr;
   size_t n = 0;
@@ -271,6 +305,8 @@ void solve_complete(napi_env env, napi_status, void* data) {
   }
   napi_delete_reference(env, w->fref);
   napi_delete_async_work(env, w->work);
+  w->handle->inflight--;
+  handle_release_if_done(w->handle);
   delete w;
 }
 
@@ -280,11 +316,12 @@ napi_value Solve(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   SolveWork* w = new SolveWork();
-  w->plan = (pp_plan*)external_ptr(env, argv[0]);
+  w->handle = argc >= 3 ? plan_handle(env, argv[0]) : nullptr;
+  w->plan = w->handle ? w->handle->plan : nullptr;
   float* F = nullptr;
   if (!w->plan || !typed(env, argv[1], napi_float32_array, &F, &w->n)) {
     delete w;
-    napi_throw_type_error(env, nullptr, "solve(plan, F: Float32Array, options)");
+    napi_throw_type_error(env, nullptr, "solve(plan, F: Float32Array, options) on a live plan");
     return nullptr;
   }
   w->F = F;
@@ -307,6 +344,7 @@ napi_value Solve(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_create_async_work(env, nullptr, name, solve_execute, solve_complete, w,
                                         &w->work));
   NAPI_CALL(env, napi_queue_async_work(env, w->work));
+  w->handle->inflight++;
   return promise;
 }
 
@@ -322,7 +360,7 @@ napi_value ApplyK(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "applyK(plan, x: Float64Array, y: Float64Array, path)");
     return nullptr;
   }
-  int rc = pp_apply_k((pp_plan*)external_ptr(env, argv[0]), x, y,
+  int rc = pp_apply_k(plan_of(env, argv[0]), x, y,
                       argc > 3 ? (int)num(env, argv[3]) : PP_PATH_AUTO);
   if (rc) return throw_pp(env, rc);
   return nullptr;
@@ -341,7 +379,7 @@ napi_value PressureLoad(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "pressureLoad(plan, q0, patches, F)");
     return nullptr;
   }
-  int rc = pp_pressure_load((pp_plan*)external_ptr(env, argv[0]), num(env, argv[1]),
+  int rc = pp_pressure_load(plan_of(env, argv[0]), num(env, argv[1]),
                             lp ? pt : nullptr, (int)(lp / 4), F);
   if (rc) return throw_pp(env, rc);
   return nullptr;
@@ -362,7 +400,7 @@ napi_value Moments(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "moments(plan, x64|null, Mx, My, Mxy)");
     return nullptr;
   }
-  int rc = pp_moments((pp_plan*)external_ptr(env, argv[0]), x, M[0], M[1], M[2]);
+  int rc = pp_moments(plan_of(env, argv[0]), x, M[0], M[1], M[2]);
   if (rc) return throw_pp(env, rc);
   return nullptr;
 }
@@ -382,7 +420,7 @@ napi_value MaxDisplacement(napi_env env, napi_callback_info info) {
   double w = 0;
   int64_t node = 0;
   float cx = 0, cy = 0;
-  int rc = pp_max_displacement((pp_plan*)external_ptr(env, argv[0]), x, &w, &node, &cx, &cy);
+  int rc = pp_max_displacement(plan_of(env, argv[0]), x, &w, &node, &cx, &cy);
   if (rc) return throw_pp(env, rc);
   napi_value out, v;
   NAPI_CALL(env, napi_create_object(env, &out));

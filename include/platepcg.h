@@ -101,8 +101,10 @@ typedef struct {
                               (<= 0: 6 below 100,000 nodes, else 5; at most 8) */
 } pp_solve_opts;
 
-/* Skip the FP64 |F - K x| / |F| recomputation after the solve
- * (stats.true_rel_residual is then 0). */
+/* Skip the FP64 |F - K x| / |F| recomputation after the solve, and with it
+ * the residual-replacement restarts (stats.true_rel_residual is then 0 and
+ * converged reflects the recurrence alone).  Mixed precision ignores it: its
+ * refinement sweeps are built on that FP64 residual. */
 #define PP_FLAG_NO_TRUE_RESIDUAL 1
 /* Structured path: use the 15x15-node tiles even where the plate fits the
  * large-tile persistent kernel (for comparison and testing). */

@@ -158,3 +158,21 @@ def test_full_256_plate_batch(ctx):
     print("worst max-nodal w error vs single-plate FP64 solves: %.2e" % worst)
     assert worst < 1e-4
     b.close()
+
+
+def test_iteration_budget_spans_restarts(ctx):
+    """max_iterations bounds every plate's total over the first launch and the
+    residual-replacement restarts, and a plate whose true residual is still
+    above the tolerance is not reported converged."""
+    plates = P.batch_plates(count=6, seed=4, n_max=40)
+    b, _ = load_batch(ctx, plates)
+    full = b.solve(tolerance=1e-8)
+    cap = max(2, full["max_iterations"] // 2)
+    st = b.solve(tolerance=1e-8, max_iterations=cap)
+    assert st["max_iterations"] <= cap
+    for i in range(len(plates)):
+        _, s = b.result(i)
+        assert s["iterations"] <= cap
+        assert not s["converged"] or s["trueRelResidual"] <= 1e-8
+    assert st["converged"] < len(plates)
+    b.close()

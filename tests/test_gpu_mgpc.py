@@ -91,3 +91,33 @@ def test_unsupported_cases(ctx):
     with pytest.raises(PlatePCGError):
         p4.solve(p4.pressure_load(1e4), precond=MG)
     p4.close()
+
+
+def test_restarts_near_the_fp64_floor(ctx):
+    """Near the FP64 floor of the true residual the recurrence converges before
+    F - K x does, so flexible PCG restarts from x with the true residual (the
+    first tolerance of 1e-10 ... 1e-13 at which the 142^2 plate needs a
+    restart; 1e-9 needs none).  The sweep statistics add up, and the solve
+    reports convergence only when the FP64 true residual is at most the
+    tolerance."""
+    w = P.workload("c2")
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    F = plan.pressure_load(w["q0"], w["patches"])
+    for tol in (1e-10, 1e-11, 1e-12, 1e-13):
+        x, _, st = plan.solve(F, tolerance=tol, max_iterations=2000, precond=MG)
+        print("tol", tol, "restart sweeps", st["sweeps"], st["sweepIterations"],
+              st["sweepResidual"], "converged", st["converged"], st["trueRelResidual"])
+        if st["sweeps"] > 1:
+            break
+    assert st["sweeps"] > 1
+    assert sum(st["sweepIterations"]) == st["iterations"]
+    assert st["sweepResidual"][st["sweeps"] - 1] == st["trueRelResidual"]
+    assert st["converged"] == (st["trueRelResidual"] <= tol)
+    w_ref = O.OracleProblem(w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    assert w_ref.true_rel_residual(x, F.astype(np.float64)) < max(3 * tol, 1e-9)
+    # without the FP64 check: one run, no true residual, convergence of the recurrence
+    _, _, s1 = plan.solve(F, tolerance=tol, max_iterations=2000, precond=MG,
+                          flags=_lib.FLAG_NO_TRUE_RESIDUAL)
+    assert s1["sweeps"] == 1 and s1["trueRelResidual"] == 0.0 and s1["converged"]
+    assert s1["iterations"] == st["sweepIterations"][0]
+    plan.close()

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_api as O
-from platepcg import Context, Plan, _lib
+from platepcg import Context, Plan, PlatePCGError, _lib
 from platepcg import problem as P
 
 pytestmark = pytest.mark.gpu
@@ -234,3 +234,22 @@ def test_rectangular_plate_strided_iterates(ctx):
     xo, so = op.pcg(F.astype(np.float64), max_iter=30)
     assert rel(x64, xo) < 1e-9
     plan.close()
+
+
+def test_malformed_colour_offsets_are_rejected():
+    """Colour offsets must start at 0, never decrease and end at the element
+    count; anything else is refused before the offsets index anything."""
+    ctx = Context(0)
+    tri = P.split_quads(P.rect_mesh(8))
+    mask = P.constrained_mask(tri, [{"type": "fixed", "location": "all_edges"}])
+    good = P.colour_greedy(tri)
+    ne = int(tri["elementCount"])
+    for offs in ([0, 5000, 100] + [ne] * (good["colour_count"] - 2),
+                 [1] + list(good["colour_offsets"][1:]),
+                 list(good["colour_offsets"][:-1]) + [ne - 1]):
+        bad = dict(good, colour_offsets=np.array(offs, np.uint32))
+        with pytest.raises(PlatePCGError) as e:
+            Plan(ctx, tri, 210e9, 0.3, 0.1, mask, bad)
+        assert e.value.code == _lib.PP_ERR_INVALID
+    Plan(ctx, tri, 210e9, 0.3, 0.1, mask, good).close()
+    ctx.close()

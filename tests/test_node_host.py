@@ -80,3 +80,17 @@ def test_solve_plate_through_node():
         np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-7 * np.abs(want).max())
     i = int(np.argmax(np.abs(x[0::3])))
     assert r["maxDisp"]["nodeIndex"] == i and r["maxDisp"]["maxW"] == x[3 * i]
+
+
+@pytest.mark.gpu
+def test_concurrent_solves_through_node():
+    """Two solveGPU calls in flight together, two solve() calls on one
+    PlateSolver together, and destroy() during a solve: every answer equals
+    the same solve run alone, bit for bit, and the float32 restatement of
+    solver.ts to the harness metric."""
+    out = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "concurrent_solves.js")],
+                         check=True, capture_output=True, text=True, timeout=180)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert r["converged"] and r["loadsDiffer"] and r["deviceMsPositive"]
+    assert r["solveGpuSame"] and r["planSame"] and r["destroyedSame"]
+    assert r["maxWRelError"] < 1e-4
