@@ -186,6 +186,13 @@ int pp_ctx_create(int device, pp_ctx** out) {
     return fail(PP_ERR_HIP, std::string("context: ") + hipGetErrorString(e));
   }
   c->cu_count = prop.multiProcessorCount;
+  if (int rc = noalloc_scratch_init(device)) {
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return PP_OK;
 }

@@ -192,6 +192,10 @@ int mesh_pressure(const float* nodes, const uint32_t* elems, int npe, int64_t n_
 void element_ke_q4(const double* xy, double E, double nu, double t, double* Ke);
 void element_ke_dkt(const double* xy, double E, double nu, double t, double* Ke);
 
+// device address of the no-allocation scratch array on `device` (current),
+// looked up once per device (pp_noalloc.hip)
+int noalloc_scratch_init(int device);
+
 // lifetime (pp_api.hip)
 void ctx_release(pp_ctx* c);
 void plan_release(pp_plan* pl);

@@ -111,6 +111,8 @@ SIGNATURES = {
     "pp_get_solution": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveOpts), C.c_void_p, C.c_void_p,
                            C.POINTER(SolveStats)]),
+    "pp_solve_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "pp_apply_k": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "pp_apply_k_colour": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "pp_apply_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

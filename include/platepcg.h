@@ -182,6 +182,25 @@ int pp_get_solution(pp_plan* plan, double* x64, float* x32);
 int pp_solve(pp_plan* plan, const float* F, const pp_solve_opts* opts, float* x32, double* x64,
              pp_solve_stats* stats);
 
+/* The FP64 block-Jacobi solve of pp_solve_resident (structured plan, the
+ * single-reduction PCG with every 18x18 tile resident, residual-replacement
+ * restarts and the FP64 true-residual check) on caller device buffers:
+ * F_dev and x_dev are device pointers to n_dof = 3 * node_count doubles; F is
+ * read, the solution written to x.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc / hipFree: every scratch buffer (exchange vectors, residual
+ * vector, reduction partials, barrier words, statistics) lives in one
+ * 16 MiB device array per GPU, held until the process ends, whose device
+ * address is looked up when a context is created.  Plates above about
+ * 690,000 DOFs (the three scratch vectors must fit) are rejected with
+ * PP_ERR_INVALID; other preconditioners, precisions and the strided
+ * kernel with PP_ERR_UNSUPPORTED.  Calls on one GPU share the array and
+ * are serialised by a lock; the array keeps the last call's contents.
+ * The plan's own resident F and x are not touched. */
+int pp_solve_device_noalloc(pp_plan* plan, const double* F_dev, double* x_dev,
+                            const pp_solve_opts* opts, pp_solve_stats* stats);
+
 /* y = K x with the reference's BC rows (y[c] = x[c] on constrained DOFs):
  * replaces gpuApplyK (gpu/solver.ts:413-461) + apply_k_q4/apply_k_dkt +
  * apply_bc.  path picks the kernels (PP_PATH_*). */
