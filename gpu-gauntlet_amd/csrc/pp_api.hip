@@ -148,6 +148,7 @@ void plan_release(pp_plan* pl) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     mg_destroy(pl->mg_pc);
+    if (pl->mgf_pinned) (void)hipHostFree(pl->mgf_pinned);
     delete pl;
   }
   ctx_release(c);

@@ -388,6 +388,7 @@ int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
     pl->mg_pc = mg;
   }
   Mg* mg = pl->mg_pc;
+  if (!(o->flags & PP_FLAG_MG_UNFUSED)) return mgf_solve(pl, mg, o, st);
   return fcg_solve(pl, o, st, [mg](const double* r, double* z) { return mg->vcycle(r, z); }, 1);
 }
 

@@ -156,6 +156,7 @@ struct pp_plan {
 
   bool rhs_set = false;
   double rhs_norm = 0;
+  void* mgf_pinned = nullptr;  // pinned host ring of the fused multigrid PCG's state
 };
 
 namespace pp {
@@ -172,8 +173,10 @@ int struct_apply_precond(pp_plan* pl, const double* dr, double* dz);
 int struct_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF);
 int struct_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
 int cheb_lambda_max(pp_plan* pl);  // lambda_max(B K) by power iteration -> pl->cheb_lmax
-// PCG preconditioned by a Galerkin multigrid V-cycle (pp_fcg.hip, pp_mg.hip)
+// PCG preconditioned by a Galerkin multigrid V-cycle (pp_fcg.hip, pp_mg.hip;
+// the fused launches in pp_mgf.hip)
 int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
+int mgf_solve(pp_plan* pl, Mg* mg, const pp_solve_opts* o, pp_solve_stats* st);
 void mg_destroy(Mg* mg);
 int struct_solve_mixed(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
 int struct_true_residual(pp_plan* pl, const double* dx, double* rel);

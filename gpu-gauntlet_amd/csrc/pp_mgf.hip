@@ -1,0 +1,839 @@
+// PCG preconditioned by one Galerkin multigrid V-cycle per iteration
+// (PP_PRECOND_MULTIGRID), with the V-cycle and the PCG arithmetic fused into
+// few launches.
+//
+// The hierarchy (levels, stencils, block inverses, coarsest dense inverse)
+// is the one pp_mg.hip builds; this file only changes how a PCG iteration is
+// launched.  Per iteration, with nu smoothing steps on level 0 and one on
+// the coarser stencil levels:
+//   A  q = K p with p = z + beta p_old staged per tile (FCG(1) direction),
+//      partials of p.q and p.r; the last workgroup to finish sums them in a
+//      fixed order and sets alpha (or the breakdown flag, pcg.ts:154)
+//   B  x += alpha p, r -= alpha q, partials of r.r (the last workgroup
+//      applies the stopping rule of pcg.ts:174), and the V-cycle's first
+//      level-0 smoothing step from x = 0
+//   V-cycle: level-0 steps 2 .. nu; then per level one kernel computing the
+//      residual on the tile plus a one-node ring, restricting it onto the
+//      coarse nodes the tile owns and taking the coarse level's first step;
+//      the dense coarsest solve; per level one kernel that prolongs the
+//      coarse correction while staging its tile and takes the first
+//      post-smoothing step; level-0 steps 2 .. nu, the last with the
+//      partials of z.q, whose last workgroup sets beta = -z.q / p.q.
+// 19 launches per iteration on the 142^2 plate (4 levels, nu = 6) where the
+// launch-per-operation form takes 34.  Every kernel returns at once when the
+// solve has stopped, so the host can queue iterations ahead of the
+// convergence flag it reads back.
+//
+// Cross-workgroup sums: each workgroup stores its partial with a
+// device-scope store and counts itself in with a device-scope acq_rel
+// fetch-add; the workgroup that arrives last (count nwg - 1) reads all
+// partials with device-scope loads and sums them in a fixed order, so the
+// result does not depend on which workgroup is last.  Nothing waits on
+// another workgroup.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "pp_cnn.h"
+
+namespace pp {
+
+namespace {
+
+struct MgfState {
+  double alpha, beta, pq, rr, fnorm, tol;
+  int it, done, converged, breakdown;
+  unsigned cnt[4];  // arrival counters of the fused reductions (reset by the last arrival)
+};
+
+// One level as the kernels see it.
+struct LvView {
+  int nx, ny;           // elements per side
+  const float* st;      // stencil planes (levels >= 1)
+  const float* minv;    // FP32 block inverse, 6 per node
+  const uint8_t* mask;  // constraint bits (level 0) or dead bits (levels >= 1)
+};
+
+// Coarse level written by a residual + restriction kernel.
+struct Coarse {
+  int nx, ny;
+  double* b;          // restricted residual
+  const float* minv;  // non-null: also take the coarse level's first step x = c2 B b
+  double c2;
+  double* cur;        // that step's result
+  double* prev;       // zeroed: the iterate before it
+};
+
+// Coarse correction prolonged while a post-smoothing step stages its tile.
+struct Pro {
+  const double* xc;  // coarse solution
+  int cnx;
+  double* xpro;      // out: the prolonged iterate on own nodes
+};
+
+// A fused reduction: per-workgroup partials and the arrival counter index.
+struct Red {
+  double* part;
+  int slot;
+};
+
+struct TileArgs {
+  KGrid G;            // level 0 geometry (matrix-free operator)
+  LvView L;
+  const double* x;    // iterate to stage (KP: z)
+  const double* y;    // KP: previous direction
+  double* xo;         // STEP: out, read as the previous iterate when c1 != 0; KP: new direction
+  const double* b;    // STEP / RES: right-hand side; KP: r
+  double c1, c2;
+  Pro pro;
+  Coarse co;
+  double* q;          // KP: out K p; ZQ: in
+  Red red;
+  MgfState* S;
+};
+
+enum { M_STEP = 0, M_RES = 1, M_KP = 2 };
+
+template <int TS, int H>
+struct TG {
+  static constexpr int PW = TS + 2 * H, NP = PW * PW;  // staged nodes
+  static constexpr int EW = PW - 1, NE = EW * EW;      // elements touching the computed nodes
+  static constexpr int CW = PW - 2, NC = CW * CW;      // computed nodes: tile + (H - 1) ring
+};
+
+__device__ __forceinline__ double w1(int f, int c, int n) {
+  if (min(2 * c, n) == f) return 1.0;
+  const int d = f - 2 * c;
+  return ((f & 1) && f < n && (d == 1 || d == -1)) ? 0.5 : 0.0;
+}
+
+// P x_c at fine node (fi, fj): bilinear in index space over the coarse
+// element's corners (fine node f sits between coarse nodes f/2 and (f+1)/2).
+__device__ __forceinline__ void prolong_node(int fi, int fj, int fnx, int fny, const double* xc,
+                                             int cnx, double (&s)[3]) {
+  const int i0 = fi / 2, i1 = (fi + 1) / 2, j0 = fj / 2, j1 = (fj + 1) / 2;
+  s[0] = s[1] = s[2] = 0.0;
+  const int ci[2] = {i0, i1}, cj[2] = {j0, j1};
+  const int ni = i0 == i1 ? 1 : 2, nj = j0 == j1 ? 1 : 2;
+  for (int b = 0; b < nj; b++)
+    for (int a = 0; a < ni; a++) {
+      const double w = w1(fi, ci[a], fnx) * w1(fj, cj[b], fny);
+      const int64_t m = (int64_t)cj[b] * (cnx + 1) + ci[a];
+      for (int d = 0; d < 3; d++) s[d] += w * xc[3 * m + d];
+    }
+}
+
+// Fixed-order grid sum of NV partials, finished by the last workgroup to
+// arrive.  v: this workgroup's partials (valid in thread 0).  Returns true
+// in every thread of the last workgroup, with the totals in v of thread 0.
+template <int NV, int NT>
+__device__ bool grid_finish(double (&v)[NV], double* part, unsigned* cnt, unsigned nwg,
+                            unsigned me, double* lds, int* flag) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      __hip_atomic_store(&part[(size_t)k * nwg + me], v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned old =
+        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *flag = old == nwg - 1;
+  }
+  __syncthreads();
+  if (!*flag) return false;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) s[k] = 0.0;
+  for (unsigned w = threadIdx.x; w < nwg; w += NT)
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      s[k] += __hip_atomic_load(&part[(size_t)k * nwg + w], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<NV, NT>(s, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = s[k];
+    __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
+}
+
+// Stage src + beta y (+ P xc, masked) on the tile plus an H-node halo; zero
+// outside the grid.  SoA: u_s[d * NP + local].
+template <int TS, int H, int NT>
+__device__ __forceinline__ void stage(int i0, int j0, int nx, int ny, const double* src,
+                                      const double* y, double beta, const double* xc, int cnx,
+                                      const uint8_t* mask, double* u_s) {
+  using T = TG<TS, H>;
+  for (int idx = threadIdx.x; idx < T::NP; idx += NT) {
+    const int lx = idx % T::PW, ly = idx / T::PW;
+    const int gi = i0 - H + lx, gj = j0 - H + ly;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (gi >= 0 && gi <= nx && gj >= 0 && gj <= ny) {
+      const int64_t n = (int64_t)gj * (nx + 1) + gi;
+      for (int d = 0; d < 3; d++) v[d] = src[3 * n + d];
+      if (y)
+        for (int d = 0; d < 3; d++) v[d] += beta * y[3 * n + d];
+      if (xc) {
+        double s[3];
+        prolong_node(gi, gj, nx, ny, xc, cnx, s);
+        const unsigned msk = mask[n];
+        for (int d = 0; d < 3; d++)
+          if (!(msk >> d & 1)) v[d] += s[d];
+      }
+    }
+    for (int d = 0; d < 3; d++) u_s[d * T::NP + idx] = v[d];
+  }
+}
+
+// Level 0: every element touching the computed nodes, matrix-free, into
+// ye_s[k * NE + element].
+template <int TS, int H, int NT>
+__device__ __forceinline__ void l0_elements(const KGrid& G, int i0, int j0, const double* u_s,
+                                            double* ye_s) {
+  using T = TG<TS, H>;
+  for (int le = threadIdx.x; le < T::NE; le += NT) {
+    const int ex = le % T::EW, ey = le / T::EW;
+    const int gi = i0 - H + ex, gj = j0 - H + ey;
+    double y[12];
+    if (gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny) {
+      const int c0 = ey * T::PW + ex, c1 = c0 + 1, c2 = c0 + T::PW + 1, c3 = c0 + T::PW;
+      constexpr int NP = T::NP;
+      const double u[12] = {u_s[c0], u_s[NP + c0], u_s[2 * NP + c0], u_s[c1], u_s[NP + c1],
+                            u_s[2 * NP + c1], u_s[c2], u_s[NP + c2], u_s[2 * NP + c2],
+                            u_s[c3], u_s[NP + c3], u_s[2 * NP + c3]};
+      double D, Ds;
+      elem_material(G, gi, gj, D, Ds);
+      q4_rect_apply_fast<double>(G.ax[2 * gi], G.by[2 * gj], G.ax[2 * gi + 1], G.by[2 * gj + 1],
+                                 D, D * G.nu, D * (1.0 - G.nu) / 2.0, Ds, u, y);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) y[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) ye_s[k * T::NE + le] = y[k];
+  }
+}
+
+// Stencil row (A x)_n of a level >= 1 from the staged x: the centre and
+// forward blocks of node n, then the transposed forward blocks of its
+// backward neighbours (the order of pp_mg.hip's stencil rows).  Staged
+// values outside the grid are zero, where the forward coefficients are zero.
+template <int PW, int NP>
+__device__ __forceinline__ void stencil_row_lds(int nx, int ny, const float* st, int gi, int gj,
+                                                int sx, int sy, const double* u_s,
+                                                double (&y)[3]) {
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
+  const int64_t n = (int64_t)gj * (nx + 1) + gi;
+  const int fdj[5] = {0, 0, 1, 1, 1}, fdi[5] = {0, 1, -1, 0, 1};
+  y[0] = y[1] = y[2] = 0.0;
+#pragma unroll
+  for (int sb = 0; sb < 5; sb++) {
+    const int li = (sy + fdj[sb]) * PW + sx + fdi[sb];
+    const double x0 = u_s[li], x1 = u_s[NP + li], x2 = u_s[2 * NP + li];
+    const float* s = st + (int64_t)sb * 9 * nn + n;
+    y[0] += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
+    y[1] += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
+    y[2] += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
+  }
+#pragma unroll
+  for (int sb = 1; sb < 5; sb++) {
+    const int jj = gj - fdj[sb], ii = gi - fdi[sb];
+    if (jj < 0 || ii < 0 || ii > nx) continue;
+    const int64_t m = (int64_t)jj * (nx + 1) + ii;
+    const int li = (sy - fdj[sb]) * PW + sx - fdi[sb];
+    const double x0 = u_s[li], x1 = u_s[NP + li], x2 = u_s[2 * NP + li];
+    const float* s = st + (int64_t)sb * 9 * nn + m;  // A(m, n) = A(n, m)^T
+    y[0] += (double)s[0] * x0 + (double)s[3 * nn] * x1 + (double)s[6 * nn] * x2;
+    y[1] += (double)s[nn] * x0 + (double)s[4 * nn] * x1 + (double)s[7 * nn] * x2;
+    y[2] += (double)s[2 * nn] * x0 + (double)s[5 * nn] * x1 + (double)s[8 * nn] * x2;
+  }
+}
+
+// One tile kernel for every fused phase.  L0: level 0 (matrix-free Q4 with
+// the constraint rows) or a stencil level; TS x TS nodes per workgroup,
+// staged with an H-node halo.
+//   M_STEP  xo = x + c1 (x - xo) + c2 B (b - A x) on the tile (PRO: x is
+//           staged as x + P xc and also written to xpro; ZQ: partials of
+//           xo.q, the last workgroup sets beta = -z.q / p.q)
+//   M_RES   t = b - A x on the tile and a one-node ring (H = 2), restricted
+//           onto the coarse nodes whose fine node min(2I, n) the tile holds,
+//           optionally with the coarse first step
+//   M_KP    p = z + beta p_old staged, q = K p, p and q stored, partials of
+//           p.q and p.r; the last workgroup sets alpha or the breakdown flag
+template <bool L0, int TS, int H, int NT, int MODE, bool PRO, bool ZQ>
+__global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
+  using T = TG<TS, H>;
+  __shared__ double u_s[3 * T::NP];
+  __shared__ double ye_s[L0 ? 12 * T::NE : 1];
+  __shared__ double t_s[MODE == M_RES ? 3 * T::NC : 1];
+  __shared__ double red_s[2 * (NT / 64) + 2];
+  __shared__ int flag_s;
+  if (A.S->done) return;
+  const int nx = A.L.nx, ny = A.L.ny;
+  const int ntx = (nx + 1 + TS - 1) / TS;
+  const int i0 = (blockIdx.x % ntx) * TS, j0 = (blockIdx.x / ntx) * TS;
+  const double beta = MODE == M_KP ? A.S->beta : 0.0;
+  stage<TS, H, NT>(i0, j0, nx, ny, A.x, MODE == M_KP ? A.y : nullptr, beta,
+                   PRO ? A.pro.xc : nullptr, A.pro.cnx, A.L.mask, u_s);
+  __syncthreads();
+  if constexpr (L0) {
+    l0_elements<TS, H, NT>(A.G, i0, j0, u_s, ye_s);
+    __syncthreads();
+  }
+  double acc[2] = {0.0, 0.0};
+  for (int c = threadIdx.x; c < T::NC; c += NT) {
+    const int cx = c % T::CW, cy = c / T::CW;
+    const int sx = cx + 1, sy = cy + 1;
+    const int gi = i0 - H + sx, gj = j0 - H + sy;
+    const bool in = gi >= 0 && gi <= nx && gj >= 0 && gj <= ny;
+    if (!in) {
+      if (MODE == M_RES)
+        for (int d = 0; d < 3; d++) t_s[d * T::NC + c] = 0.0;
+      continue;
+    }
+    const int64_t n = (int64_t)gj * (nx + 1) + gi;
+    const int si = sy * T::PW + sx;
+    double Kx[3];
+    if constexpr (L0) {
+      constexpr int EW = T::EW, NE = T::NE;
+      const int sw = cy * EW + cx, se = sw + 1, ne = sw + EW + 1, nw = sw + EW;
+      const unsigned fix = A.L.mask[n];
+      for (int d = 0; d < 3; d++) {
+        const double v = ye_s[(6 + d) * NE + sw] + ye_s[(9 + d) * NE + se] + ye_s[d * NE + ne] +
+                         ye_s[(3 + d) * NE + nw];
+        Kx[d] = (fix >> d) & 1u ? u_s[d * T::NP + si] : v;
+      }
+    } else {
+      stencil_row_lds<T::PW, T::NP>(nx, ny, A.L.st, gi, gj, sx, sy, u_s, Kx);
+    }
+    if (MODE == M_RES) {
+      for (int d = 0; d < 3; d++) t_s[d * T::NC + c] = A.b[3 * n + d] - Kx[d];
+    } else if (MODE == M_KP) {
+      for (int d = 0; d < 3; d++) {
+        const double p = u_s[d * T::NP + si];
+        A.xo[3 * n + d] = p;
+        A.q[3 * n + d] = Kx[d];
+        acc[0] += p * Kx[d];
+        acc[1] += p * A.b[3 * n + d];
+      }
+    } else {
+      double r[3], z[3];
+      for (int d = 0; d < 3; d++) r[d] = A.b[3 * n + d] - Kx[d];
+      sym3_apply(A.L.minv + 6 * n, r, z);
+      for (int d = 0; d < 3; d++) {
+        const double xc = u_s[d * T::NP + si];
+        const double dn = (A.c1 != 0.0 ? A.c1 * (xc - A.xo[3 * n + d]) : 0.0) + A.c2 * z[d];
+        const double xn = xc + dn;
+        A.xo[3 * n + d] = xn;
+        if (PRO) A.pro.xpro[3 * n + d] = xc;
+        if (ZQ) acc[0] += xn * A.q[3 * n + d];
+      }
+    }
+  }
+  if constexpr (MODE == M_RES) {
+    __syncthreads();
+    const Coarse& co = A.co;
+    const int fnx = nx, fny = ny, cnx = co.nx, cny = co.ny;
+    constexpr int CWc = TS / 2 + 2;
+    const int Ib = i0 / 2, Jb = j0 / 2;
+    for (int k = threadIdx.x; k < CWc * CWc; k += NT) {
+      const int I = Ib + k % CWc, J = Jb + k / CWc;
+      if (I > cnx || J > cny) continue;
+      const int fi0 = min(2 * I, fnx), fj0 = min(2 * J, fny);
+      if (fi0 < i0 || fi0 >= i0 + TS || fj0 < j0 || fj0 >= j0 + TS) continue;
+      double s[3] = {0.0, 0.0, 0.0};
+      for (int b = -1; b <= 1; b++) {
+        const int fj = 2 * J + b;
+        if (fj < 0 || fj > fny) continue;
+        for (int a = -1; a <= 1; a++) {
+          const int fi = 2 * I + a;
+          if (fi < 0 || fi > fnx) continue;
+          const double w = w1(fi, I, fnx) * w1(fj, J, fny);
+          const unsigned msk = A.L.mask[(int64_t)fj * (fnx + 1) + fi];
+          const int ci = (fj - (j0 - H + 1)) * T::CW + (fi - (i0 - H + 1));
+          for (int d = 0; d < 3; d++)
+            if (!(msk >> d & 1)) s[d] += w * t_s[d * T::NC + ci];
+        }
+      }
+      const int64_t nc = (int64_t)J * (cnx + 1) + I;
+      for (int d = 0; d < 3; d++) co.b[3 * nc + d] = s[d];
+      if (co.minv) {
+        double z[3];
+        sym3_apply(co.minv + 6 * nc, s, z);
+        for (int d = 0; d < 3; d++) {
+          co.cur[3 * nc + d] = 0.0 + co.c2 * z[d];
+          co.prev[3 * nc + d] = 0.0;
+        }
+      }
+    }
+    return;
+  }
+  if constexpr (MODE == M_KP || ZQ) {
+    constexpr int NV = MODE == M_KP ? 2 : 1;
+    double v[NV];
+    for (int k = 0; k < NV; k++) v[k] = acc[k];
+    block_sum<NV, NT>(v, red_s);
+    const unsigned nwg = gridDim.x;
+    if (!grid_finish<NV, NT>(v, A.red.part, &A.S->cnt[A.red.slot], nwg, blockIdx.x, red_s, &flag_s))
+      return;
+    if (threadIdx.x == 0) {
+      MgfState* S = A.S;
+      if (MODE == M_KP) {
+        S->pq = v[0];
+        if (fabs(v[0]) < 1e-30) {
+          S->breakdown = 1;
+          S->done = 1;
+        } else {
+          S->alpha = v[NV - 1] / v[0];
+        }
+      } else {
+        S->beta = -v[0] / S->pq;
+      }
+    }
+  }
+}
+
+// x += alpha p, r -= alpha q; partials of r.r (the last workgroup applies
+// the stopping rule |r| < tol or |r| / |F| < tol and counts the
+// iteration); and the V-cycle's first level-0 step from x = 0 on the new r:
+// cur = c2 B r, prev = 0.
+__global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r, const double* p,
+                                                const double* q, const float* minv, double c2,
+                                                double* cur, double* prev, double* part,
+                                                MgfState* S) {
+  __shared__ double red_s[8];
+  __shared__ int flag_s;
+  if (S->done) return;
+  const double a = S->alpha;
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v[1] = {0.0};
+  if (n < nn) {
+    double rv[3];
+    for (int d = 0; d < 3; d++) {
+      x[3 * n + d] += a * p[3 * n + d];
+      const double ri = r[3 * n + d] - a * q[3 * n + d];
+      r[3 * n + d] = ri;
+      rv[d] = ri;
+      v[0] += ri * ri;
+    }
+    double z[3];
+    sym3_apply(minv + 6 * n, rv, z);
+    for (int d = 0; d < 3; d++) {
+      cur[3 * n + d] = 0.0 + c2 * z[d];
+      prev[3 * n + d] = 0.0;
+    }
+  }
+  block_sum<1, 256>(v, red_s);
+  if (!grid_finish<1, 256>(v, part, &S->cnt[1], gridDim.x, blockIdx.x, red_s, &flag_s)) return;
+  if (threadIdx.x == 0) {
+    S->rr = v[0];
+    S->it++;
+    const double rn = sqrt(v[0]);
+    if (rn < S->tol || rn / S->fnorm < S->tol) {
+      S->converged = 1;
+      S->done = 1;
+    }
+  }
+}
+
+// The V-cycle's first level-0 step from x = 0 without an update (start of a run).
+__global__ void k_point(int64_t nn, const float* minv, double c2, const double* b, double* cur,
+                        double* prev) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double rv[3] = {b[3 * n], b[3 * n + 1], b[3 * n + 2]}, z[3];
+  sym3_apply(minv + 6 * n, rv, z);
+  for (int d = 0; d < 3; d++) {
+    cur[3 * n + d] = 0.0 + c2 * z[d];
+    prev[3 * n + d] = 0.0;
+  }
+}
+
+// x = C b on the coarsest level (C dense symmetric nd x nd), one wave per row.
+__global__ void k_dense(int nd, const double* C, const double* b, double* x, const MgfState* S) {
+  if (S->done) return;
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x & 63;
+  if (row >= nd) return;
+  double s = 0.0;
+  for (int k = lane; k < nd; k += 64) s += C[(int64_t)row * nd + k] * b[k];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) x[row] = s;
+}
+
+__global__ void k_state_init(MgfState* S, double fnorm, double tol) {
+  if (threadIdx.x) return;
+  *S = MgfState{};
+  S->fnorm = fnorm > 0.0 ? fnorm : 1.0;
+  S->tol = tol;
+}
+
+__global__ void k_fill0(int64_t n, double* a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+int ntiles(int nx, int ny, int ts) { return ((nx + 1 + ts - 1) / ts) * ((ny + 1 + ts - 1) / ts); }
+
+// tile shapes: level-0 steps and K p on 15 x 15 nodes (16 x 16 elements: one
+// element per thread), level-0 residuals on 13 x 13 (a 2-node halo: again
+// 16 x 16 elements); stencil levels 16 x 16 and 14 x 14
+constexpr int kTs0 = 15, kTr0 = 13, kTsS = 16, kTrS = 14;
+
+struct Coef {
+  double c1, c2;
+};
+
+// Host side of the fused iteration.
+struct Fused {
+  pp_plan* pl = nullptr;
+  Mg* mg = nullptr;
+  hipStream_t s = nullptr;
+  KGrid G;
+  int nl = 0;
+  struct Lv {
+    LvView v;
+    int64_t nn = 0;
+    double* buf[3] = {nullptr, nullptr, nullptr};  // smoothing iterates
+    double* b = nullptr;
+    double* out = nullptr;                         // where the cycle left x
+    std::vector<Coef> cs;                          // step coefficients of a smoothing pass
+  } L[Mg::kMaxLevels];
+  MgfState* S = nullptr;
+  double* part = nullptr;  // partials: K p (2 x maxwg), update (maxwg), z.q (maxwg)
+  int maxwg = 0;
+  // buffers the level-0 first step writes (cur, prev), chosen so that the
+  // cycle ends in buf[0] = z: (1, 0) for an odd step count, (2, 1) for even
+  int pc0 = 1, pp0 = 0;
+
+  void setup(pp_plan* p, Mg* m, MgfState* st, double* pt, int mw) {
+    pl = p;
+    mg = m;
+    s = p->ctx->stream;
+    G = make_kgrid(p);
+    nl = m->nl;
+    S = st;
+    part = pt;
+    maxwg = mw;
+    pc0 = m->nu % 2 ? 1 : 2;
+    pp0 = m->nu % 2 ? 0 : 1;
+    for (int l = 0; l < nl; l++) {
+      MgLevel& M = m->lv[l];
+      Lv& V = L[l];
+      V.v.nx = M.nx;
+      V.v.ny = M.ny;
+      V.v.st = l > 0 ? M.st.p : nullptr;
+      V.v.minv = M.minv.p;
+      V.v.mask = l == 0 ? (const uint8_t*)p->cmask.p : (const uint8_t*)M.dead.p;
+      V.nn = M.nn;
+      if (l == 0) {
+        V.buf[0] = p->z.p;
+        V.buf[1] = M.x2.p;
+        V.buf[2] = M.t.p;
+        V.b = p->r.p;
+      } else {
+        V.buf[0] = M.x.p;
+        V.buf[1] = M.x2.p;
+        V.buf[2] = M.t.p;
+        V.b = M.b.p;
+      }
+      if (l == nl - 1) continue;
+      const int ns = l == 0 ? m->nu : m->nuc;
+      V.cs.resize(ns);
+      double theta = 1.0, delta = 1.0, rho = 0.0;
+      if (!m->jacobi) {
+        const double hi = 4.0 / (3.0 * M.omega), lo = hi / m->cheb_ratio;
+        theta = 0.5 * (hi + lo);
+        delta = 0.5 * (hi - lo);
+      }
+      const double sigma = theta / delta;
+      for (int k = 0; k < ns; k++) {
+        if (m->jacobi) {
+          V.cs[k] = {0.0, M.omega};
+        } else if (k == 0) {
+          rho = 1.0 / sigma;
+          V.cs[k] = {0.0, 1.0 / theta};
+        } else {
+          const double rn = 1.0 / (2.0 * sigma - rho);
+          V.cs[k] = {rn * rho, 2.0 * rn / delta};
+          rho = rn;
+        }
+      }
+    }
+  }
+
+  TileArgs args(int l) const {
+    TileArgs A;
+    memset(&A, 0, sizeof(A));
+    A.G = G;
+    A.L = L[l].v;
+    A.S = S;
+    return A;
+  }
+
+  // one smoothing step on level l: x -> xo (xo holds the previous iterate)
+  int step(int l, const double* x, double* xo, Coef c, const Pro* pro, bool zq) {
+    TileArgs A = args(l);
+    A.x = x;
+    A.xo = xo;
+    A.b = L[l].b;
+    A.c1 = c.c1;
+    A.c2 = c.c2;
+    if (pro) A.pro = *pro;
+    if (zq) {
+      A.q = pl->Ap.p;
+      A.red = {part + 3 * (size_t)maxwg, 2};
+    }
+    const int nx = L[l].v.nx, ny = L[l].v.ny;
+    if (l == 0) {
+      const int g = ntiles(nx, ny, kTs0);
+      if (pro && zq)
+        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, true, true>), dim3(g), dim3(256), 0, s, A);
+      else if (pro)
+        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, true, false>), dim3(g), dim3(256), 0, s, A);
+      else if (zq)
+        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, false, true>), dim3(g), dim3(256), 0, s, A);
+      else
+        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, false, false>), dim3(g), dim3(256), 0, s, A);
+    } else {
+      const int g = ntiles(nx, ny, kTsS);
+      if (pro)
+        hipLaunchKernelGGL((k_tile<false, kTsS, 1, 256, M_STEP, true, false>), dim3(g), dim3(256), 0, s, A);
+      else
+        hipLaunchKernelGGL((k_tile<false, kTsS, 1, 256, M_STEP, false, false>), dim3(g), dim3(256), 0, s, A);
+    }
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+  }
+
+  // residual of level l at x, restricted to level l + 1, with its first step
+  // when l + 1 is not the coarsest (into buf[1], buf[0] zeroed)
+  int res(int l, const double* x) {
+    TileArgs A = args(l);
+    A.x = x;
+    A.b = L[l].b;
+    Lv& C = L[l + 1];
+    A.co.nx = C.v.nx;
+    A.co.ny = C.v.ny;
+    A.co.b = C.b;
+    if (l + 1 < nl - 1) {
+      A.co.minv = C.v.minv;
+      A.co.c2 = C.cs[0].c2;
+      A.co.cur = C.buf[1];
+      A.co.prev = C.buf[0];
+    }
+    const int nx = L[l].v.nx, ny = L[l].v.ny;
+    if (l == 0)
+      hipLaunchKernelGGL((k_tile<true, kTr0, 2, 256, M_RES, false, false>),
+                         dim3(ntiles(nx, ny, kTr0)), dim3(256), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_tile<false, kTrS, 2, 256, M_RES, false, false>),
+                         dim3(ntiles(nx, ny, kTrS)), dim3(256), 0, s, A);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+  }
+
+  // V-cycle on level l whose first step is already done (level 0: cur =
+  // buf[pc0], prev = buf[pp0]; coarser: buf[1], buf[0]); the result is left
+  // in L[l].out (level 0: buf[0] = z).  zq: the last level-0
+  // step also sums z.q for beta.
+  int cycle(int l, bool zq) {
+    Lv& V = L[l];
+    if (l == nl - 1) {
+      hipLaunchKernelGGL(k_dense, dim3(nblk(mg->nd_coarse, 4)), dim3(256), 0, s, mg->nd_coarse,
+                         (const double*)mg->cinv.p, (const double*)V.b, V.buf[0],
+                         (const MgfState*)S);
+      PP_HIP(hipGetLastError());
+      V.out = V.buf[0];
+      return PP_OK;
+    }
+    const int ns = (int)V.cs.size();
+    int cur = l == 0 ? pc0 : 1, prev = l == 0 ? pp0 : 0, rc;
+    for (int k = 1; k < ns; k++) {
+      if ((rc = step(l, V.buf[cur], V.buf[prev], V.cs[k], nullptr, false))) return rc;
+      std::swap(cur, prev);
+    }
+    if ((rc = res(l, V.buf[cur]))) return rc;
+    if ((rc = cycle(l + 1, false))) return rc;
+    // prolongation with the first post-smoothing step: out and the prolonged
+    // iterate go to the two buffers other than cur (which other tiles still
+    // read as their halo); pick them so that the pass ends in buf[0] (level
+    // 0: z)
+    const int o1 = (cur + 1) % 3, o2 = (cur + 2) % 3;
+    int out, pro;
+    if ((ns - 1) % 2 == 0) {  // the pass ends where the first post step writes
+      out = o1 == 0 ? o1 : (o2 == 0 ? o2 : o1);
+      pro = out == o1 ? o2 : o1;
+    } else {                  // ... or where the prolonged iterate goes
+      pro = o1 == 0 ? o1 : (o2 == 0 ? o2 : o1);
+      out = pro == o1 ? o2 : o1;
+    }
+    Pro P{L[l + 1].out, L[l + 1].v.nx, V.buf[pro]};
+    if ((rc = step(l, V.buf[cur], V.buf[out], V.cs[0], &P, zq && ns == 1))) return rc;
+    cur = out;
+    prev = pro;
+    for (int k = 1; k < ns; k++) {
+      if ((rc = step(l, V.buf[cur], V.buf[prev], V.cs[k], nullptr, zq && k == ns - 1))) return rc;
+      std::swap(cur, prev);
+    }
+    V.out = V.buf[cur];
+    return PP_OK;
+  }
+
+  // K p with p = z + beta p_old (beta from the state), partials for alpha
+  int kp(const double* pold, double* pnew) {
+    TileArgs A = args(0);
+    A.x = pl->z.p;
+    A.y = pold;
+    A.xo = pnew;
+    A.b = pl->r.p;
+    A.q = pl->Ap.p;
+    A.red = {part, 0};
+    hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_KP, false, false>),
+                       dim3(ntiles(L[0].v.nx, L[0].v.ny, kTs0)), dim3(256), 0, s, A);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+  }
+
+  int update(const double* p) {
+    const Lv& V = L[0];
+    hipLaunchKernelGGL(k_update, dim3(nblk(V.nn, 256)), dim3(256), 0, s, V.nn, pl->x.p, pl->r.p,
+                       p, (const double*)pl->Ap.p, V.v.minv, V.cs[0].c2, V.buf[pc0], V.buf[pp0],
+                       part + 2 * (size_t)maxwg, S);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+  }
+
+  // z = V r from the current r (start of a run)
+  int first_cycle() {
+    const Lv& V = L[0];
+    hipLaunchKernelGGL(k_point, dim3(nblk(V.nn, 256)), dim3(256), 0, s, V.nn, V.v.minv,
+                       V.cs[0].c2, (const double*)V.b, V.buf[pc0], V.buf[pp0]);
+    PP_HIP(hipGetLastError());
+    return cycle(0, false);
+  }
+};
+
+// workgroups of the largest fused reduction
+int max_workgroups(const pp_plan* pl) {
+  return std::max(ntiles(pl->nx, pl->ny, kTs0), nblk(pl->n_nodes, 256));
+}
+
+}  // namespace
+
+// FCG(1) with the fused V-cycle from x (zero on a cold start) with r = F - K x
+// in pl->r; stops by the rule of pcg.ts:174 or at maxit iterations.  The host
+// keeps kAhead iterations queued beyond the last one whose state it has read.
+static int mgf_run(pp_plan* pl, Fused& F, double fnorm, double tol, int maxit, MgfState* hs) {
+  constexpr int kAhead = 2, kRing = 4;
+  hipStream_t s = pl->ctx->stream;
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, F.S, fnorm, tol);
+  // p_old = 0 (beta = 0 on the first iteration makes p = z)
+  hipLaunchKernelGGL(k_fill0, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, pl->n_dof, pl->p0.p);
+  hipLaunchKernelGGL(k_fill0, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, pl->n_dof, pl->p1.p);
+  PP_HIP(hipGetLastError());
+  int rc;
+  if ((rc = F.first_cycle())) return rc;
+  if (F.L[0].out != pl->z.p) return fail(PP_ERR_INVALID, "multigrid: V-cycle did not end in z");
+  // pinned copies of the state, one per queued iteration (kept by the plan)
+  if (!pl->mgf_pinned)
+    PP_HIP(hipHostMalloc(&pl->mgf_pinned, kRing * sizeof(MgfState), hipHostMallocDefault));
+  MgfState* pinned = (MgfState*)pl->mgf_pinned;
+  hipEvent_t ev[kRing];
+  for (int k = 0; k < kRing; k++) PP_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+  int queued = 0, checked = 0;
+  memset(hs, 0, sizeof(*hs));
+  rc = PP_OK;
+  while (rc == PP_OK) {
+    while (queued < maxit && queued < checked + kAhead) {
+      double* pold = (queued & 1) ? pl->p0.p : pl->p1.p;
+      double* pnew = (queued & 1) ? pl->p1.p : pl->p0.p;
+      if ((rc = F.kp(pold, pnew)) || (rc = F.update(pnew)) || (rc = F.cycle(0, true))) break;
+      hipError_t e = hipMemcpyAsync(&pinned[queued % kRing], F.S, sizeof(MgfState),
+                                    hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipEventRecord(ev[queued % kRing], s);
+      if (e != hipSuccess) {
+        rc = fail(PP_ERR_HIP, std::string("multigrid queue: ") + hipGetErrorString(e));
+        break;
+      }
+      queued++;
+    }
+    if (rc || checked == queued) break;
+    hipError_t e = hipEventSynchronize(ev[checked % kRing]);
+    if (e != hipSuccess) {
+      rc = fail(PP_ERR_HIP, std::string("multigrid wait: ") + hipGetErrorString(e));
+      break;
+    }
+    *hs = pinned[checked % kRing];
+    checked++;
+    if (hs->done) break;
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  for (int k = 0; k < kRing; k++) (void)hipEventDestroy(ev[k]);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(PP_ERR_HIP, std::string("multigrid: ") + hipGetErrorString(e));
+  PP_HIP(hipMemcpy(hs, F.S, sizeof(MgfState), hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+// The fused multigrid PCG with residual replacement (the restart rule of the
+// launch-per-operation form in pp_fcg.hip).
+int mgf_solve(pp_plan* pl, Mg* mg, const pp_solve_opts* o, pp_solve_stats* st) {
+  constexpr int kMaxRestarts = 16;
+  hipStream_t s = pl->ctx->stream;
+  const int64_t n = pl->n_dof;
+  const int mw = max_workgroups(pl);
+  DevBuf<MgfState> S;
+  DevBuf<double> part;
+  int rc;
+  if ((rc = S.alloc(1)) || (rc = part.alloc(4 * (size_t)mw))) return rc;
+  Fused F;
+  F.setup(pl, mg, S.p, part.p, mw);
+  PP_HIP(hipEventRecord(pl->ctx->ev0, s));
+  hipLaunchKernelGGL(k_fill0, dim3(nblk(n, 256)), dim3(256), 0, s, n, pl->x.p);
+  PP_HIP(hipMemcpyAsync(pl->r.p, pl->F.p, n * 8, hipMemcpyDeviceToDevice, s));
+  double res = 0, fnorm = 0;
+  if ((rc = launch_norms(pl, pl->F.p, pl->F.p, &res, &fnorm))) return rc;
+  MgfState hs;
+  memset(&hs, 0, sizeof(hs));
+  int total = 0, sweeps = 0;
+  double rel = 0.0;
+  const bool check = !(o->flags & PP_FLAG_NO_TRUE_RESIDUAL);
+  if (fnorm < 1e-30) {
+    hs.converged = 1;
+  } else {
+    for (int k = 0; k <= kMaxRestarts; k++) {
+      const double tol = k <= 1 ? o->tolerance : 0.5 * o->tolerance;
+      if ((rc = mgf_run(pl, F, fnorm, tol, o->max_iterations - total, &hs))) return rc;
+      if (sweeps < PP_MAX_SWEEPS) st->sweep_iterations[sweeps] = hs.it;
+      total += hs.it;
+      if (!check) {
+        sweeps++;
+        break;
+      }
+      if ((rc = struct_apply_k(pl, pl->x.p, pl->tmp.p))) return rc;
+      if ((rc = launch_norms(pl, pl->F.p, pl->tmp.p, &res, &fnorm))) return rc;
+      rel = res / fnorm;
+      if (sweeps < PP_MAX_SWEEPS) st->sweep_residual[sweeps] = rel;
+      sweeps++;
+      if (!hs.converged || rel <= o->tolerance || total >= o->max_iterations) break;
+      if ((rc = launch_residual(pl, pl->F.p, pl->tmp.p, pl->r.p))) return rc;
+    }
+  }
+  PP_HIP(hipEventRecord(pl->ctx->ev1, s));
+  PP_HIP(hipEventSynchronize(pl->ctx->ev1));
+  float ms = 0;
+  PP_HIP(hipEventElapsedTime(&ms, pl->ctx->ev0, pl->ctx->ev1));
+  st->iterations = total;
+  st->converged = hs.converged && (!check || rel <= o->tolerance);
+  st->breakdown = hs.breakdown;
+  st->sweeps = sweeps;
+  st->final_residual = sqrt(hs.rr);
+  st->true_rel_residual = check ? rel : 0.0;
+  st->device_ms = ms;
+  return PP_OK;
+}
+
+}  // namespace pp

@@ -27,6 +27,7 @@ FLAG_NO_TRUE_RESIDUAL = 1
 FLAG_SMALL_TILES = 2
 FLAG_CLASSIC_PCG = 4
 FLAG_RELIABLE_UPDATE = 8
+FLAG_MG_UNFUSED = 16
 
 
 class Mesh(C.Structure):

@@ -116,6 +116,11 @@ typedef struct {
  * residual replacement ("reliable update") instead of restarting the inner
  * PCG from p = z at every refinement sweep. */
 #define PP_FLAG_RELIABLE_UPDATE 8
+/* PP_PRECOND_MULTIGRID: one launch per smoothing step, transfer and PCG
+ * operation (34 per iteration on the 142^2 plate) instead of the fused
+ * launches (19), for comparison; same arithmetic up to the order of the
+ * dot-product sums. */
+#define PP_FLAG_MG_UNFUSED 16
 
 #define PP_MAX_SWEEPS 32
 

@@ -121,3 +121,38 @@ def test_restarts_near_the_fp64_floor(ctx):
     assert s1["sweeps"] == 1 and s1["trueRelResidual"] == 0.0 and s1["converged"]
     assert s1["iterations"] == st["sweepIterations"][0]
     plan.close()
+
+
+@pytest.mark.parametrize("case", [("c1", 6), ("c2", 1), ("c2", 3), ("c2", 5), ("c2", 6),
+                                  ("rect", 4)])
+def test_fused_launches_match_launch_per_operation(ctx, case):
+    """The fused iteration (19 launches on the 142^2 plate: K p with the
+    direction update, x/r update with the first smoothing step, residual with
+    restriction and the coarse first step, prolongation with the first
+    post-smoothing step, reductions finished by the last workgroup) computes
+    the same V-cycle and PCG as one launch per operation: the same iteration
+    count within one, and the same displacements to 1e-8 of max |w|.  Odd and
+    even step counts (the buffer rotation ends in z either way), a plate with
+    no stencil level (32^2: level 0 then the dense coarsest) and a
+    rectangular odd-sized plate."""
+    name, steps = case
+    if name == "rect":
+        mesh = P.rect_mesh(90, 37, width=10.0, height=4.0)
+        mask = P.edge_mask(90, 37, ["fixed", "pinned", "fixed", "pinned"])
+        plan = Plan(ctx, mesh, E, NU, 0.05, mask)
+        F = plan.pressure_load(1e4, np.array([[3e4, 6.0, 1.5, 0.7]]))
+    else:
+        w = P.workload(name)
+        plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+        F = plan.pressure_load(w["q0"], w["patches"])
+    xf, _, sf = plan.solve(F, tolerance=1e-8, max_iterations=2000, precond=MG, poly_degree=steps)
+    xu, _, su = plan.solve(F, tolerance=1e-8, max_iterations=2000, precond=MG, poly_degree=steps,
+                           flags=_lib.FLAG_MG_UNFUSED)
+    print(case, "fused", sf["iterations"], f"{sf['deviceMs']:.2f} ms", "unfused",
+          su["iterations"], f"{su['deviceMs']:.2f} ms")
+    assert sf["converged"] and su["converged"]
+    assert sf["trueRelResidual"] <= 1e-8
+    assert abs(sf["iterations"] - su["iterations"]) <= 1
+    wmax = np.abs(xu[0::3]).max()
+    assert np.abs(xf[0::3] - xu[0::3]).max() < 1e-8 * wmax
+    plan.close()
