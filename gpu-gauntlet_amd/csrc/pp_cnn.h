@@ -40,6 +40,7 @@ struct MgLevel {
   DevBuf<float> minv;    // 6 per node (symmetric 3x3 block inverse, FP32 for smoothing)
   DevBuf<uint8_t> dead;  // level >= 1: per-node bits of DOFs with a zero diagonal
   DevBuf<double> b, x, x2, t;  // right-hand side, iterate and its sweep partner, work
+  DevBuf<float> rows;    // levels >= 1, fused form: each node's full 9-block row, 84 floats
 };
 
 struct Mg {

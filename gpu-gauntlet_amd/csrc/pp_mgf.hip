@@ -34,6 +34,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "pp_cnn.h"
@@ -48,10 +49,29 @@ struct MgfState {
   unsigned cnt[4];  // arrival counters of the fused reductions (reset by the last arrival)
 };
 
+// Progress record of one iteration, written by the device into coherent
+// pinned host memory (slot = iteration % ring) by the last workgroup of the
+// update (or, on a breakdown, of K p); seq = iteration + 1 is stored last.
+struct MgfRec {
+  double rr;
+  int it, done, converged, breakdown;
+  int seq, pad;
+};
+
+__device__ __forceinline__ void write_record(MgfRec* ring, int nring, int iter, const MgfState* S) {
+  MgfRec* r = ring + iter % nring;
+  r->rr = S->rr;
+  r->it = S->it;
+  r->done = S->done;
+  r->converged = S->converged;
+  r->breakdown = S->breakdown;
+  __hip_atomic_store(&r->seq, iter + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // One level as the kernels see it.
 struct LvView {
   int nx, ny;           // elements per side
-  const float* st;      // stencil planes (levels >= 1)
+  const float* st;      // stencil rows (levels >= 1): 84 floats per node (see k_rows)
   const float* minv;    // FP32 block inverse, 6 per node
   const uint8_t* mask;  // constraint bits (level 0) or dead bits (levels >= 1)
 };
@@ -73,13 +93,18 @@ struct Pro {
   double* xpro;      // out: the prolonged iterate on own nodes
 };
 
-// A fused reduction: per-workgroup partials and the arrival counter index.
+// A fused reduction: per-workgroup partials [value][workgroup], plain
+// stores; with `last`, the last workgroup to arrive also finishes the sum
+// into the state (arrival counter `slot`).
 struct Red {
   double* part;
   int slot;
+  int last;
 };
 
 struct TileArgs {
+  MgfRec* rec;        // KP: progress records (breakdown)
+  int nrec;
   KGrid G;            // level 0 geometry (matrix-free operator)
   LvView L;
   const double* x;    // iterate to stage (KP: z)
@@ -91,8 +116,48 @@ struct TileArgs {
   Coarse co;
   double* q;          // KP: out K p; ZQ: in
   Red red;
+  // Partials of the reduction before this kernel, summed by every workgroup
+  // itself (null: the producer's last workgroup left the scalar in the
+  // state): KP: z.q of the last smoothing step; check: r.r of the update.
+  const double* cin;
+  int cn;
+  int check;          // apply the stopping rule to the update before (with cin)
   MgfState* S;
 };
+
+// Sum of NV partials [k * n + i] in a fixed order by wave 0 (lane-strided
+// partial sums, then the butterfly); the result is valid in every lane of
+// wave 0 and identical in every workgroup.
+template <int NV>
+__device__ __forceinline__ void wave_sum_parts(const double* part, int n, double (&v)[NV]) {
+  const int lane = threadIdx.x & 63;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) s[k] = 0.0;
+  for (int i = lane; i < n; i += 64)
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k] += part[(size_t)k * n + i];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(s[k]);
+}
+
+// The stopping rule of pcg.ts:174 after an update with |r|^2 = rr, applied
+// by every workgroup alike; workgroup 0 counts the iteration and records it.
+// Returns true when the solve stops here.
+__device__ __forceinline__ bool stop_rule(double rr, MgfState* S, MgfRec* rec, int nrec) {
+  const double rn = sqrt(rr);
+  const bool conv = rn < S->tol || rn / S->fnorm < S->tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->rr = rr;
+    S->it++;
+    if (conv) {
+      S->converged = 1;
+      S->done = 1;
+    }
+    write_record(rec, nrec, S->it - 1, S);
+  }
+  return conv;
+}
 
 enum { M_STEP = 0, M_RES = 1, M_KP = 2 };
 
@@ -187,73 +252,67 @@ __device__ __forceinline__ void stage(int i0, int j0, int nx, int ny, const doub
   }
 }
 
-// Level 0: every element touching the computed nodes, matrix-free, into
-// ye_s[k * NE + element].
-template <int TS, int H, int NT>
-__device__ __forceinline__ void l0_elements(const KGrid& G, int i0, int j0, const double* u_s,
-                                            double* ye_s) {
-  using T = TG<TS, H>;
-  for (int le = threadIdx.x; le < T::NE; le += NT) {
-    const int ex = le % T::EW, ey = le / T::EW;
-    const int gi = i0 - H + ex, gj = j0 - H + ey;
-    double y[12];
-    if (gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny) {
-      const int c0 = ey * T::PW + ex, c1 = c0 + 1, c2 = c0 + T::PW + 1, c3 = c0 + T::PW;
-      constexpr int NP = T::NP;
-      const double u[12] = {u_s[c0], u_s[NP + c0], u_s[2 * NP + c0], u_s[c1], u_s[NP + c1],
-                            u_s[2 * NP + c1], u_s[c2], u_s[NP + c2], u_s[2 * NP + c2],
-                            u_s[c3], u_s[NP + c3], u_s[2 * NP + c3]};
-      double D, Ds;
-      elem_material(G, gi, gj, D, Ds);
-      q4_rect_apply_fast<double>(G.ax[2 * gi], G.by[2 * gj], G.ax[2 * gi + 1], G.by[2 * gj + 1],
-                                 D, D * G.nu, D * (1.0 - G.nu) / 2.0, Ds, u, y);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 12; k++) y[k] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 12; k++) ye_s[k * T::NE + le] = y[k];
+// Stencil row of a level >= 1 in the fused form: per node 84 floats (21
+// float4), the 9 blocks of its row of A in the order the apply adds them:
+// the centre and forward blocks E, NW, N, NE as stored in the planes, then
+// for the backward neighbours W, SE, S, SW the block A(n, m) = A(m, n)^T,
+// transposed from m's forward block (zero outside the grid); 3 floats pad.
+constexpr int kRow = 84;
+
+__global__ void k_rows(int nx, int ny, const float* st, float* rows) {
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const int i = (int)(n % (nx + 1)), j = (int)(n / (nx + 1));
+  float* R = rows + n * kRow;
+  for (int k = 0; k < 45; k++) R[k] = st[k * nn + n];
+  const int fdj[5] = {0, 0, 1, 1, 1}, fdi[5] = {0, 1, -1, 0, 1};
+  for (int sb = 1; sb < 5; sb++) {
+    const int jj = j - fdj[sb], ii = i - fdi[sb];
+    const bool ok = jj >= 0 && ii >= 0 && ii <= nx;
+    const int64_t m = (int64_t)jj * (nx + 1) + ii;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++)
+        R[45 + (sb - 1) * 9 + 3 * r + c] = ok ? st[(sb * 9 + 3 * c + r) * nn + m] : 0.0f;
   }
+  for (int k = 81; k < kRow; k++) R[k] = 0.0f;
 }
 
-// Stencil row (A x)_n of a level >= 1 from the staged x: the centre and
-// forward blocks of node n, then the transposed forward blocks of its
-// backward neighbours (the order of pp_mg.hip's stencil rows).  Staged
-// values outside the grid are zero, where the forward coefficients are zero.
-template <int PW, int NP>
-__device__ __forceinline__ void stencil_row_lds(int nx, int ny, const float* st, int gi, int gj,
-                                                int sx, int sy, const double* u_s,
-                                                double (&y)[3]) {
-  const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
+struct StRow {
+  float4 v[kRow / 4];
+};
+
+__device__ __forceinline__ void load_st_row(int nx, const float* rows, int gi, int gj, StRow& R) {
   const int64_t n = (int64_t)gj * (nx + 1) + gi;
-  const int fdj[5] = {0, 0, 1, 1, 1}, fdi[5] = {0, 1, -1, 0, 1};
+  const float4* p = (const float4*)(rows + n * kRow);
+#pragma unroll
+  for (int k = 0; k < kRow / 4; k++) R.v[k] = p[k];
+}
+
+// (A x)_n from the prefetched row and the staged x: the 9 blocks in row
+// order, block by block as pp_mg.hip's stencil rows add them.
+template <int PW, int NP>
+__device__ __forceinline__ void stencil_apply(const StRow& R, int sx, int sy, const double* u_s,
+                                              double (&y)[3]) {
+  const float* f = (const float*)R.v;
+  // neighbour offsets (dj, di): centre, E, NW, N, NE, W, SE, S, SW
+  const int odj[9] = {0, 0, 1, 1, 1, 0, -1, -1, -1}, odi[9] = {0, 1, -1, 0, 1, -1, 1, 0, -1};
   y[0] = y[1] = y[2] = 0.0;
 #pragma unroll
-  for (int sb = 0; sb < 5; sb++) {
-    const int li = (sy + fdj[sb]) * PW + sx + fdi[sb];
+  for (int b = 0; b < 9; b++) {
+    const int li = (sy + odj[b]) * PW + sx + odi[b];
     const double x0 = u_s[li], x1 = u_s[NP + li], x2 = u_s[2 * NP + li];
-    const float* s = st + (int64_t)sb * 9 * nn + n;
-    y[0] += (double)s[0] * x0 + (double)s[nn] * x1 + (double)s[2 * nn] * x2;
-    y[1] += (double)s[3 * nn] * x0 + (double)s[4 * nn] * x1 + (double)s[5 * nn] * x2;
-    y[2] += (double)s[6 * nn] * x0 + (double)s[7 * nn] * x1 + (double)s[8 * nn] * x2;
-  }
-#pragma unroll
-  for (int sb = 1; sb < 5; sb++) {
-    const int jj = gj - fdj[sb], ii = gi - fdi[sb];
-    if (jj < 0 || ii < 0 || ii > nx) continue;
-    const int64_t m = (int64_t)jj * (nx + 1) + ii;
-    const int li = (sy - fdj[sb]) * PW + sx - fdi[sb];
-    const double x0 = u_s[li], x1 = u_s[NP + li], x2 = u_s[2 * NP + li];
-    const float* s = st + (int64_t)sb * 9 * nn + m;  // A(m, n) = A(n, m)^T
-    y[0] += (double)s[0] * x0 + (double)s[3 * nn] * x1 + (double)s[6 * nn] * x2;
-    y[1] += (double)s[nn] * x0 + (double)s[4 * nn] * x1 + (double)s[7 * nn] * x2;
-    y[2] += (double)s[2 * nn] * x0 + (double)s[5 * nn] * x1 + (double)s[8 * nn] * x2;
+    const float* s = f + 9 * b;
+    y[0] += (double)s[0] * x0 + (double)s[1] * x1 + (double)s[2] * x2;
+    y[1] += (double)s[3] * x0 + (double)s[4] * x1 + (double)s[5] * x2;
+    y[2] += (double)s[6] * x0 + (double)s[7] * x1 + (double)s[8] * x2;
   }
 }
 
 // One tile kernel for every fused phase.  L0: level 0 (matrix-free Q4 with
 // the constraint rows) or a stencil level; TS x TS nodes per workgroup,
-// staged with an H-node halo.
+// staged with an H-node halo; one computed node (and at level 0 one element)
+// per thread.
 //   M_STEP  xo = x + c1 (x - xo) + c2 B (b - A x) on the tile (PRO: x is
 //           staged as x + P xc and also written to xpro; ZQ: partials of
 //           xo.q, the last workgroup sets beta = -z.q / p.q)
@@ -262,87 +321,175 @@ __device__ __forceinline__ void stencil_row_lds(int nx, int ny, const float* st,
 //           optionally with the coarse first step
 //   M_KP    p = z + beta p_old staged, q = K p, p and q stored, partials of
 //           p.q and p.r; the last workgroup sets alpha or the breakdown flag
+// Every operand a thread needs after the staging (its node's right-hand
+// side, block inverse, previous iterate, constraint bits and stencil row,
+// its element's widths and material) is loaded before the staging, so the
+// kernel waits on global memory once.
 template <bool L0, int TS, int H, int NT, int MODE, bool PRO, bool ZQ>
 __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   using T = TG<TS, H>;
+  static_assert(T::NC <= NT, "one computed node per thread");
+  static_assert(!L0 || T::NE <= NT, "one element per thread");
   __shared__ double u_s[3 * T::NP];
+  __shared__ double y_s[MODE == M_KP ? 3 * T::NP : 1];  // KP: p_old (p = z + beta p_old)
   __shared__ double ye_s[L0 ? 12 * T::NE : 1];
   __shared__ double t_s[MODE == M_RES ? 3 * T::NC : 1];
+  __shared__ uint8_t m_s[MODE == M_RES ? T::NC : 1];
   __shared__ double red_s[2 * (NT / 64) + 2];
+  __shared__ double in_s[2];
   __shared__ int flag_s;
   if (A.S->done) return;
   const int nx = A.L.nx, ny = A.L.ny;
   const int ntx = (nx + 1 + TS - 1) / TS;
   const int i0 = (blockIdx.x % ntx) * TS, j0 = (blockIdx.x / ntx) * TS;
-  const double beta = MODE == M_KP ? A.S->beta : 0.0;
-  stage<TS, H, NT>(i0, j0, nx, ny, A.x, MODE == M_KP ? A.y : nullptr, beta,
-                   PRO ? A.pro.xc : nullptr, A.pro.cnx, A.L.mask, u_s);
+  // ---- this thread's computed node and its operands
+  const int c = threadIdx.x;
+  const int cx = c % T::CW, cy = c / T::CW;
+  const int sx = cx + 1, sy = cy + 1;
+  const int gi = i0 - H + sx, gj = j0 - H + sy;
+  const bool in = c < T::NC && gi >= 0 && gi <= nx && gj >= 0 && gj <= ny;
+  const int64_t n = in ? (int64_t)gj * (nx + 1) + gi : 0;
+  unsigned fix = 0;
+  double bv[3] = {0.0, 0.0, 0.0}, xov[3] = {0.0, 0.0, 0.0}, qv[3] = {0.0, 0.0, 0.0};
+  float mv[6] = {0, 0, 0, 0, 0, 0};
+  StRow R;
+  if (in) {
+    if (L0 || MODE == M_RES) fix = A.L.mask[n];
+    for (int d = 0; d < 3; d++) bv[d] = A.b[3 * n + d];
+    if (MODE == M_STEP) {
+      for (int k = 0; k < 6; k++) mv[k] = A.L.minv[6 * n + k];
+      if (A.c1 != 0.0)
+        for (int d = 0; d < 3; d++) xov[d] = A.xo[3 * n + d];
+      if (ZQ)
+        for (int d = 0; d < 3; d++) qv[d] = A.q[3 * n + d];
+    }
+    if constexpr (!L0) load_st_row(nx, A.L.st, gi, gj, R);
+  }
+  // ---- this thread's element (level 0)
+  const int le = threadIdx.x;
+  const int ex = le % T::EW, ey = le / T::EW;
+  const int egi = i0 - H + ex, egj = j0 - H + ey;
+  const bool ein = L0 && le < T::NE && egi >= 0 && egi < A.G.nx && egj >= 0 && egj < A.G.ny;
+  double ea = 0, eb = 0, eia = 0, eib = 0, eD = 0, eDs = 0;
+  if (ein) {
+    ea = A.G.ax[2 * egi];
+    eia = A.G.ax[2 * egi + 1];
+    eb = A.G.by[2 * egj];
+    eib = A.G.by[2 * egj + 1];
+    elem_material(A.G, egi, egj, eD, eDs);
+  }
+  // ---- (RES) this thread's coarse node and its block inverse
+  constexpr int CWc = TS / 2 + 2;
+  int cI = -1, cJ = -1;
+  float cm[6] = {0, 0, 0, 0, 0, 0};
+  if (MODE == M_RES && (int)threadIdx.x < CWc * CWc) {
+    const int I = i0 / 2 + threadIdx.x % CWc, J = j0 / 2 + threadIdx.x / CWc;
+    const int fi0 = min(2 * I, nx), fj0 = min(2 * J, ny);
+    if (I <= A.co.nx && J <= A.co.ny && fi0 >= i0 && fi0 < i0 + TS && fj0 >= j0 &&
+        fj0 < j0 + TS) {
+      cI = I;
+      cJ = J;
+      if (A.co.minv)
+        for (int q = 0; q < 6; q++) cm[q] = A.co.minv[6 * ((int64_t)J * (A.co.nx + 1) + I) + q];
+    }
+  }
+  // ---- consumed partials (wave 0; the sum reaches the other waves through LDS)
+  double cv[1] = {0.0};
+  if (A.cin && threadIdx.x < 64) wave_sum_parts<1>(A.cin, A.cn, cv);
+  stage<TS, H, NT>(i0, j0, nx, ny, A.x, nullptr, 0.0, PRO ? A.pro.xc : nullptr, A.pro.cnx,
+                   A.L.mask, u_s);
+  if (MODE == M_KP)
+    stage<TS, H, NT>(i0, j0, nx, ny, A.y, nullptr, 0.0, nullptr, 0, A.L.mask, y_s);
+  if (threadIdx.x == 0) in_s[0] = cv[0];
   __syncthreads();
+  if (A.check && stop_rule(in_s[0], A.S, A.rec, A.nrec)) return;
+  // KP: beta = -z.q / p.q of the last direction (0 on the first iteration)
+  const double beta =
+      MODE == M_KP ? (A.cin ? -in_s[0] / A.S->pq : A.S->beta) : 0.0;
   if constexpr (L0) {
-    l0_elements<TS, H, NT>(A.G, i0, j0, u_s, ye_s);
+    if (le < T::NE) {
+      double y[12];
+      if (ein) {
+        const int c0 = ey * T::PW + ex, c1 = c0 + 1, c2 = c0 + T::PW + 1, c3 = c0 + T::PW;
+        constexpr int NP = T::NP;
+        const int cc[4] = {c0, c1, c2, c3};
+        double u[12];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+            u[3 * k + d] = MODE == M_KP ? u_s[d * NP + cc[k]] + beta * y_s[d * NP + cc[k]]
+                                        : u_s[d * NP + cc[k]];
+        q4_rect_apply_fast<double>(ea, eb, eia, eib, eD, eD * A.G.nu, eD * (1.0 - A.G.nu) / 2.0,
+                                   eDs, u, y);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) y[k] = 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 12; k++) ye_s[k * T::NE + le] = y[k];
+    }
     __syncthreads();
   }
   double acc[2] = {0.0, 0.0};
-  for (int c = threadIdx.x; c < T::NC; c += NT) {
-    const int cx = c % T::CW, cy = c / T::CW;
-    const int sx = cx + 1, sy = cy + 1;
-    const int gi = i0 - H + sx, gj = j0 - H + sy;
-    const bool in = gi >= 0 && gi <= nx && gj >= 0 && gj <= ny;
+  if (c < T::NC) {
     if (!in) {
-      if (MODE == M_RES)
+      if (MODE == M_RES) {
         for (int d = 0; d < 3; d++) t_s[d * T::NC + c] = 0.0;
-      continue;
-    }
-    const int64_t n = (int64_t)gj * (nx + 1) + gi;
-    const int si = sy * T::PW + sx;
-    double Kx[3];
-    if constexpr (L0) {
-      constexpr int EW = T::EW, NE = T::NE;
-      const int sw = cy * EW + cx, se = sw + 1, ne = sw + EW + 1, nw = sw + EW;
-      const unsigned fix = A.L.mask[n];
-      for (int d = 0; d < 3; d++) {
-        const double v = ye_s[(6 + d) * NE + sw] + ye_s[(9 + d) * NE + se] + ye_s[d * NE + ne] +
-                         ye_s[(3 + d) * NE + nw];
-        Kx[d] = (fix >> d) & 1u ? u_s[d * T::NP + si] : v;
+        m_s[c] = 0;
       }
     } else {
-      stencil_row_lds<T::PW, T::NP>(nx, ny, A.L.st, gi, gj, sx, sy, u_s, Kx);
-    }
-    if (MODE == M_RES) {
-      for (int d = 0; d < 3; d++) t_s[d * T::NC + c] = A.b[3 * n + d] - Kx[d];
-    } else if (MODE == M_KP) {
-      for (int d = 0; d < 3; d++) {
-        const double p = u_s[d * T::NP + si];
-        A.xo[3 * n + d] = p;
-        A.q[3 * n + d] = Kx[d];
-        acc[0] += p * Kx[d];
-        acc[1] += p * A.b[3 * n + d];
+      const int si = sy * T::PW + sx;
+      double own[3];  // the staged value of this node (KP: p = z + beta p_old)
+      for (int d = 0; d < 3; d++)
+        own[d] = MODE == M_KP ? u_s[d * T::NP + si] + beta * y_s[d * T::NP + si]
+                              : u_s[d * T::NP + si];
+      double Kx[3];
+      if constexpr (L0) {
+        constexpr int EW = T::EW, NE = T::NE;
+        const int sw = cy * EW + cx, se = sw + 1, ne = sw + EW + 1, nw = sw + EW;
+        for (int d = 0; d < 3; d++) {
+          const double v = ye_s[(6 + d) * NE + sw] + ye_s[(9 + d) * NE + se] + ye_s[d * NE + ne] +
+                           ye_s[(3 + d) * NE + nw];
+          Kx[d] = (fix >> d) & 1u ? own[d] : v;
+        }
+      } else {
+        stencil_apply<T::PW, T::NP>(R, sx, sy, u_s, Kx);
       }
-    } else {
-      double r[3], z[3];
-      for (int d = 0; d < 3; d++) r[d] = A.b[3 * n + d] - Kx[d];
-      sym3_apply(A.L.minv + 6 * n, r, z);
-      for (int d = 0; d < 3; d++) {
-        const double xc = u_s[d * T::NP + si];
-        const double dn = (A.c1 != 0.0 ? A.c1 * (xc - A.xo[3 * n + d]) : 0.0) + A.c2 * z[d];
-        const double xn = xc + dn;
-        A.xo[3 * n + d] = xn;
-        if (PRO) A.pro.xpro[3 * n + d] = xc;
-        if (ZQ) acc[0] += xn * A.q[3 * n + d];
+      if (MODE == M_RES) {
+        for (int d = 0; d < 3; d++) t_s[d * T::NC + c] = bv[d] - Kx[d];
+        m_s[c] = (uint8_t)fix;
+      } else if (MODE == M_KP) {
+        for (int d = 0; d < 3; d++) {
+          const double p = own[d];
+          A.xo[3 * n + d] = p;
+          A.q[3 * n + d] = Kx[d];
+          acc[0] += p * Kx[d];
+          acc[1] += p * bv[d];
+        }
+      } else {
+        double r[3], z[3];
+        for (int d = 0; d < 3; d++) r[d] = bv[d] - Kx[d];
+        sym3_apply(mv, r, z);
+        for (int d = 0; d < 3; d++) {
+          const double xc = own[d];
+          const double dn = (A.c1 != 0.0 ? A.c1 * (xc - xov[d]) : 0.0) + A.c2 * z[d];
+          const double xn = xc + dn;
+          A.xo[3 * n + d] = xn;
+          if (PRO) A.pro.xpro[3 * n + d] = xc;
+          if (ZQ) acc[0] += xn * qv[d];
+        }
       }
     }
   }
   if constexpr (MODE == M_RES) {
     __syncthreads();
     const Coarse& co = A.co;
-    const int fnx = nx, fny = ny, cnx = co.nx, cny = co.ny;
-    constexpr int CWc = TS / 2 + 2;
-    const int Ib = i0 / 2, Jb = j0 / 2;
-    for (int k = threadIdx.x; k < CWc * CWc; k += NT) {
-      const int I = Ib + k % CWc, J = Jb + k / CWc;
-      if (I > cnx || J > cny) continue;
-      const int fi0 = min(2 * I, fnx), fj0 = min(2 * J, fny);
-      if (fi0 < i0 || fi0 >= i0 + TS || fj0 < j0 || fj0 >= j0 + TS) continue;
+    const int fnx = nx, fny = ny, cnx = co.nx;
+    static_assert(CWc * CWc <= NT, "one coarse node per thread");
+    if (cI >= 0) {
+      const int I = cI, J = cJ;
+      const int64_t nc = (int64_t)J * (cnx + 1) + I;
       double s[3] = {0.0, 0.0, 0.0};
       for (int b = -1; b <= 1; b++) {
         const int fj = 2 * J + b;
@@ -351,17 +498,16 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
           const int fi = 2 * I + a;
           if (fi < 0 || fi > fnx) continue;
           const double w = w1(fi, I, fnx) * w1(fj, J, fny);
-          const unsigned msk = A.L.mask[(int64_t)fj * (fnx + 1) + fi];
           const int ci = (fj - (j0 - H + 1)) * T::CW + (fi - (i0 - H + 1));
+          const unsigned msk = m_s[ci];
           for (int d = 0; d < 3; d++)
             if (!(msk >> d & 1)) s[d] += w * t_s[d * T::NC + ci];
         }
       }
-      const int64_t nc = (int64_t)J * (cnx + 1) + I;
       for (int d = 0; d < 3; d++) co.b[3 * nc + d] = s[d];
       if (co.minv) {
         double z[3];
-        sym3_apply(co.minv + 6 * nc, s, z);
+        sym3_apply(cm, s, z);
         for (int d = 0; d < 3; d++) {
           co.cur[3 * nc + d] = 0.0 + co.c2 * z[d];
           co.prev[3 * nc + d] = 0.0;
@@ -376,6 +522,11 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
     for (int k = 0; k < NV; k++) v[k] = acc[k];
     block_sum<NV, NT>(v, red_s);
     const unsigned nwg = gridDim.x;
+    if (!A.red.last) {
+      if (threadIdx.x == 0)
+        for (int k = 0; k < NV; k++) A.red.part[(size_t)k * nwg + blockIdx.x] = v[k];
+      return;
+    }
     if (!grid_finish<NV, NT>(v, A.red.part, &A.S->cnt[A.red.slot], nwg, blockIdx.x, red_s, &flag_s))
       return;
     if (threadIdx.x == 0) {
@@ -385,6 +536,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
         if (fabs(v[0]) < 1e-30) {
           S->breakdown = 1;
           S->done = 1;
+          write_record(A.rec, A.nrec, S->it, S);
         } else {
           S->alpha = v[NV - 1] / v[0];
         }
@@ -395,38 +547,84 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   }
 }
 
-// x += alpha p, r -= alpha q; partials of r.r (the last workgroup applies
-// the stopping rule |r| < tol or |r| / |F| < tol and counts the
-// iteration); and the V-cycle's first level-0 step from x = 0 on the new r:
-// cur = c2 B r, prev = 0.
+// x += alpha p, r -= alpha q with alpha = p.r / p.q, the partials of r.r,
+// and the V-cycle's first level-0 step from x = 0 on the new r: cur = c2 B r,
+// prev = 0.  Consumer form (cin): every workgroup sums the K p partials
+// [p.q | p.r] itself and applies the breakdown rule (pcg.ts:154); the
+// kernel after this one applies the stopping rule.  Otherwise alpha comes
+// from the state and the last workgroup applies the stopping rule.
 __global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r, const double* p,
                                                 const double* q, const float* minv, double c2,
-                                                double* cur, double* prev, double* part,
-                                                MgfState* S) {
+                                                double* cur, double* prev, const double* cin,
+                                                int cn, Red red, MgfState* S, MgfRec* rec,
+                                                int nrec) {
   __shared__ double red_s[8];
+  __shared__ double in_s[2];
   __shared__ int flag_s;
   if (S->done) return;
-  const double a = S->alpha;
   const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, qv[3] = {0, 0, 0};
+  float mv[6] = {0, 0, 0, 0, 0, 0};
+  if (n < nn) {
+    for (int d = 0; d < 3; d++) {
+      xv[d] = x[3 * n + d];
+      rv[d] = r[3 * n + d];
+      pv[d] = p[3 * n + d];
+      qv[d] = q[3 * n + d];
+    }
+    for (int k = 0; k < 6; k++) mv[k] = minv[6 * n + k];
+  }
+  double a;
+  if (cin) {
+    double pp[2] = {0.0, 0.0};
+    if (threadIdx.x < 64) wave_sum_parts<2>(cin, cn, pp);
+    if (threadIdx.x == 0) {
+      in_s[0] = pp[0];
+      in_s[1] = pp[1];
+    }
+    __syncthreads();
+    const double pq = in_s[0];
+    if (fabs(pq) < 1e-30) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->pq = pq;
+        S->breakdown = 1;
+        S->done = 1;
+        write_record(rec, nrec, S->it, S);
+      }
+      return;
+    }
+    a = in_s[1] / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S->pq = pq;
+      S->alpha = a;
+    }
+  } else {
+    a = S->alpha;
+  }
   double v[1] = {0.0};
   if (n < nn) {
-    double rv[3];
+    double rn[3];
     for (int d = 0; d < 3; d++) {
-      x[3 * n + d] += a * p[3 * n + d];
-      const double ri = r[3 * n + d] - a * q[3 * n + d];
+      x[3 * n + d] = xv[d] + a * pv[d];
+      const double ri = rv[d] - a * qv[d];
       r[3 * n + d] = ri;
-      rv[d] = ri;
+      rn[d] = ri;
       v[0] += ri * ri;
     }
     double z[3];
-    sym3_apply(minv + 6 * n, rv, z);
+    sym3_apply(mv, rn, z);
     for (int d = 0; d < 3; d++) {
       cur[3 * n + d] = 0.0 + c2 * z[d];
       prev[3 * n + d] = 0.0;
     }
   }
   block_sum<1, 256>(v, red_s);
-  if (!grid_finish<1, 256>(v, part, &S->cnt[1], gridDim.x, blockIdx.x, red_s, &flag_s)) return;
+  if (!red.last) {
+    if (threadIdx.x == 0) red.part[blockIdx.x] = v[0];
+    return;
+  }
+  if (!grid_finish<1, 256>(v, red.part, &S->cnt[red.slot], gridDim.x, blockIdx.x, red_s, &flag_s))
+    return;
   if (threadIdx.x == 0) {
     S->rr = v[0];
     S->it++;
@@ -435,6 +633,7 @@ __global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r
       S->converged = 1;
       S->done = 1;
     }
+    write_record(rec, nrec, S->it - 1, S);
   }
 }
 
@@ -456,8 +655,18 @@ __global__ void k_dense(int nd, const double* C, const double* b, double* x, con
   if (S->done) return;
   const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x & 63;
   if (row >= nd) return;
-  double s = 0.0;
-  for (int k = lane; k < nd; k += 64) s += C[(int64_t)row * nd + k] * b[k];
+  // four independent partial sums per lane keep four loads in flight
+  const double* Cr = C + (int64_t)row * nd;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int k = lane;
+  for (; k + 192 < nd; k += 256) {
+    s0 += Cr[k] * b[k];
+    s1 += Cr[k + 64] * b[k + 64];
+    s2 += Cr[k + 128] * b[k + 128];
+    s3 += Cr[k + 192] * b[k + 192];
+  }
+  for (; k < nd; k += 64) s0 += Cr[k] * b[k];
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) x[row] = s;
 }
@@ -502,13 +711,30 @@ struct Fused {
     std::vector<Coef> cs;                          // step coefficients of a smoothing pass
   } L[Mg::kMaxLevels];
   MgfState* S = nullptr;
+  MgfRec* rec = nullptr;  // coherent pinned progress records
+  int nrec = 0;
   double* part = nullptr;  // partials: K p (2 x maxwg), update (maxwg), z.q (maxwg)
   int maxwg = 0;
+  // consumer form of the reductions (few workgroups): every workgroup of the
+  // next kernel sums the partials itself, so no kernel ends with a last-arrival
+  // tail; with many workgroups the last arrival finishes each sum instead
+  bool consumer = false;
+  bool pending_check = false;  // the next V-cycle kernel applies the stopping rule
   // buffers the level-0 first step writes (cur, prev), chosen so that the
   // cycle ends in buf[0] = z: (1, 0) for an odd step count, (2, 1) for even
   int pc0 = 1, pp0 = 0;
 
-  void setup(pp_plan* p, Mg* m, MgfState* st, double* pt, int mw) {
+  int setup(pp_plan* p, Mg* m, MgfState* st, double* pt, int mw) {
+    // the fused stencil rows of the stencil levels, built once per hierarchy
+    for (int l = 1; l + 1 < m->nl; l++) {
+      MgLevel& M = m->lv[l];
+      if (M.rows.n == (size_t)kRow * M.nn) continue;
+      int rc = M.rows.alloc((size_t)kRow * M.nn);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_rows, dim3(nblk(M.nn, 256)), dim3(256), 0, p->ctx->stream, M.nx, M.ny,
+                         (const float*)M.st.p, M.rows.p);
+      PP_HIP(hipGetLastError());
+    }
     pl = p;
     mg = m;
     s = p->ctx->stream;
@@ -524,7 +750,7 @@ struct Fused {
       Lv& V = L[l];
       V.v.nx = M.nx;
       V.v.ny = M.ny;
-      V.v.st = l > 0 ? M.st.p : nullptr;
+      V.v.st = l > 0 ? M.rows.p : nullptr;
       V.v.minv = M.minv.p;
       V.v.mask = l == 0 ? (const uint8_t*)p->cmask.p : (const uint8_t*)M.dead.p;
       V.nn = M.nn;
@@ -562,14 +788,23 @@ struct Fused {
         }
       }
     }
+    return PP_OK;
   }
 
-  TileArgs args(int l) const {
+  TileArgs args(int l) {
     TileArgs A;
     memset(&A, 0, sizeof(A));
     A.G = G;
     A.L = L[l].v;
     A.S = S;
+    A.rec = rec;
+    A.nrec = nrec;
+    if (pending_check) {
+      A.check = 1;
+      A.cin = part + 2 * (size_t)maxwg;
+      A.cn = nblk(L[0].nn, 256);
+      pending_check = false;
+    }
     return A;
   }
 
@@ -584,7 +819,7 @@ struct Fused {
     if (pro) A.pro = *pro;
     if (zq) {
       A.q = pl->Ap.p;
-      A.red = {part + 3 * (size_t)maxwg, 2};
+      A.red = {part + 3 * (size_t)maxwg, 2, !consumer};
     }
     const int nx = L[l].v.nx, ny = L[l].v.ny;
     if (l == 0) {
@@ -682,15 +917,19 @@ struct Fused {
     return PP_OK;
   }
 
-  // K p with p = z + beta p_old (beta from the state), partials for alpha
-  int kp(const double* pold, double* pnew) {
+  // K p with p = z + beta p_old, partials for alpha; first: beta = 0
+  int kp(const double* pold, double* pnew, bool first) {
     TileArgs A = args(0);
     A.x = pl->z.p;
     A.y = pold;
     A.xo = pnew;
     A.b = pl->r.p;
     A.q = pl->Ap.p;
-    A.red = {part, 0};
+    A.red = {part, 0, !consumer};
+    if (consumer && !first) {
+      A.cin = part + 3 * (size_t)maxwg;
+      A.cn = ntiles(L[0].v.nx, L[0].v.ny, kTs0);
+    }
     hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_KP, false, false>),
                        dim3(ntiles(L[0].v.nx, L[0].v.ny, kTs0)), dim3(256), 0, s, A);
     PP_HIP(hipGetLastError());
@@ -699,10 +938,14 @@ struct Fused {
 
   int update(const double* p) {
     const Lv& V = L[0];
+    const double* cin = consumer ? part : nullptr;
+    const int cn = ntiles(V.v.nx, V.v.ny, kTs0);
+    Red red{part + 2 * (size_t)maxwg, 1, !consumer};
     hipLaunchKernelGGL(k_update, dim3(nblk(V.nn, 256)), dim3(256), 0, s, V.nn, pl->x.p, pl->r.p,
                        p, (const double*)pl->Ap.p, V.v.minv, V.cs[0].c2, V.buf[pc0], V.buf[pp0],
-                       part + 2 * (size_t)maxwg, S);
+                       cin, cn, red, S, rec, nrec);
     PP_HIP(hipGetLastError());
+    pending_check = consumer;
     return PP_OK;
   }
 
@@ -716,6 +959,9 @@ struct Fused {
   }
 };
 
+// largest producer grid for which every consumer workgroup sums the partials itself
+constexpr int kConsumerMax = 512;
+
 // workgroups of the largest fused reduction
 int max_workgroups(const pp_plan* pl) {
   return std::max(ntiles(pl->nx, pl->ny, kTs0), nblk(pl->n_nodes, 256));
@@ -723,12 +969,25 @@ int max_workgroups(const pp_plan* pl) {
 
 }  // namespace
 
+constexpr int kRecRing = 8;
+
 // FCG(1) with the fused V-cycle from x (zero on a cold start) with r = F - K x
-// in pl->r; stops by the rule of pcg.ts:174 or at maxit iterations.  The host
-// keeps kAhead iterations queued beyond the last one whose state it has read.
+// in pl->r; stops by the rule of pcg.ts:174 or at maxit iterations.  Every
+// iteration that starts before the stop leaves one progress record in
+// coherent pinned host memory; the host spins on the record of the oldest
+// queued iteration and keeps up to kAhead iterations queued.  From the last
+// two recorded residual norms it predicts the iteration at which the rule
+// will hold and queues nothing beyond it until that is confirmed (queued
+// iterations after the stop return at once but still cost a dispatch each).
 static int mgf_run(pp_plan* pl, Fused& F, double fnorm, double tol, int maxit, MgfState* hs) {
-  constexpr int kAhead = 2, kRing = 4;
+  constexpr int kAhead = 2;
   hipStream_t s = pl->ctx->stream;
+  if (!pl->mgf_pinned)
+    PP_HIP(hipHostMalloc(&pl->mgf_pinned, kRecRing * sizeof(MgfRec), hipHostMallocCoherent));
+  MgfRec* rec = (MgfRec*)pl->mgf_pinned;
+  memset(rec, 0, kRecRing * sizeof(MgfRec));  // the stream is idle: earlier runs were waited for
+  F.rec = rec;
+  F.nrec = kRecRing;
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, F.S, fnorm, tol);
   // p_old = 0 (beta = 0 on the first iteration makes p = z)
   hipLaunchKernelGGL(k_fill0, dim3(nblk(pl->n_dof, 256)), dim3(256), 0, s, pl->n_dof, pl->p0.p);
@@ -737,41 +996,55 @@ static int mgf_run(pp_plan* pl, Fused& F, double fnorm, double tol, int maxit, M
   int rc;
   if ((rc = F.first_cycle())) return rc;
   if (F.L[0].out != pl->z.p) return fail(PP_ERR_INVALID, "multigrid: V-cycle did not end in z");
-  // pinned copies of the state, one per queued iteration (kept by the plan)
-  if (!pl->mgf_pinned)
-    PP_HIP(hipHostMalloc(&pl->mgf_pinned, kRing * sizeof(MgfState), hipHostMallocDefault));
-  MgfState* pinned = (MgfState*)pl->mgf_pinned;
-  hipEvent_t ev[kRing];
-  for (int k = 0; k < kRing; k++) PP_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+  const double fn = fnorm > 0.0 ? fnorm : 1.0;
+  auto converges = [&](double rn) { return rn < tol || rn / fn < tol; };
   int queued = 0, checked = 0;
-  memset(hs, 0, sizeof(*hs));
+  double rn_last = -1.0, rho = -1.0;  // |r| of the last checked iteration, and its ratio to the one before
   rc = PP_OK;
-  while (rc == PP_OK) {
-    while (queued < maxit && queued < checked + kAhead) {
+  for (;;) {
+    while (queued < maxit && queued - checked < kAhead) {
+      // predicted |r| after iteration queued - 1 (slack of 1.5 for the estimate)
+      if (queued > checked && rn_last > 0.0 && rho > 0.0 && rho < 1.0 &&
+          converges(rn_last * pow(rho, queued - checked) / 1.5))
+        break;
       double* pold = (queued & 1) ? pl->p0.p : pl->p1.p;
       double* pnew = (queued & 1) ? pl->p1.p : pl->p0.p;
-      if ((rc = F.kp(pold, pnew)) || (rc = F.update(pnew)) || (rc = F.cycle(0, true))) break;
-      hipError_t e = hipMemcpyAsync(&pinned[queued % kRing], F.S, sizeof(MgfState),
-                                    hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess) e = hipEventRecord(ev[queued % kRing], s);
-      if (e != hipSuccess) {
-        rc = fail(PP_ERR_HIP, std::string("multigrid queue: ") + hipGetErrorString(e));
+      if ((rc = F.kp(pold, pnew, queued == 0)) || (rc = F.update(pnew)) ||
+          (rc = F.cycle(0, true)))
         break;
-      }
       queued++;
     }
     if (rc || checked == queued) break;
-    hipError_t e = hipEventSynchronize(ev[checked % kRing]);
-    if (e != hipSuccess) {
-      rc = fail(PP_ERR_HIP, std::string("multigrid wait: ") + hipGetErrorString(e));
-      break;
+    // wait for the record of iteration `checked` (or for the stream to drain:
+    // then no record is coming and the device state is final)
+    const MgfRec* r = rec + checked % kRecRing;
+    bool got = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; spin++) {
+      if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) == checked + 1) {
+        got = true;
+        break;
+      }
+      if ((spin & 255) == 255) {
+        if (hipStreamQuery(s) == hipSuccess) {
+          got = __atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) == checked + 1;
+          break;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60)) {
+          rc = fail(PP_ERR_TIMEOUT, "multigrid: no progress record within 60 s");
+          break;
+        }
+      }
     }
-    *hs = pinned[checked % kRing];
+    if (rc || !got) break;
+    const double rn = sqrt(r->rr);
+    const int done = r->done;
+    if (rn_last > 0.0) rho = rn / rn_last;
+    rn_last = rn;
     checked++;
-    if (hs->done) break;
+    if (done) break;
   }
   hipError_t e = hipStreamSynchronize(s);
-  for (int k = 0; k < kRing; k++) (void)hipEventDestroy(ev[k]);
   if (rc) return rc;
   if (e != hipSuccess) return fail(PP_ERR_HIP, std::string("multigrid: ") + hipGetErrorString(e));
   PP_HIP(hipMemcpy(hs, F.S, sizeof(MgfState), hipMemcpyDeviceToHost));
@@ -790,7 +1063,8 @@ int mgf_solve(pp_plan* pl, Mg* mg, const pp_solve_opts* o, pp_solve_stats* st) {
   int rc;
   if ((rc = S.alloc(1)) || (rc = part.alloc(4 * (size_t)mw))) return rc;
   Fused F;
-  F.setup(pl, mg, S.p, part.p, mw);
+  if ((rc = F.setup(pl, mg, S.p, part.p, mw))) return rc;
+  F.consumer = mw <= kConsumerMax;
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   hipLaunchKernelGGL(k_fill0, dim3(nblk(n, 256)), dim3(256), 0, s, n, pl->x.p);
   PP_HIP(hipMemcpyAsync(pl->r.p, pl->F.p, n * 8, hipMemcpyDeviceToDevice, s));
