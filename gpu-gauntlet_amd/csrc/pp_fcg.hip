@@ -368,11 +368,11 @@ int learned_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
 // linear, so FCG(1) is plain PCG); the hierarchy is built on the first such
 // solve of the plan, with level 0 smoothing at w = 4 / (3 1.05 lambda_max(B K)),
 // o->poly_degree Chebyshev steps on level 0 on [lambda_max / 32, lambda_max].
-// The default steps were measured: 6 at 142^2 (46 iterations, 6.8 ms; 4: 56,
-// 8.1 ms), 5 at 700^2 (64.1 ms; 6: 68.2) and at 1024^2 (120.9 ms; 4: 123.7,
-// 6: 133.8): launch-bound small plates take more steps per V-cycle.
+// The default of 5 steps was measured with the fused launches at 142^2 (49
+// iterations, 4.96 ms; 6: 46, 5.08 ms; 3: 63, 5.28 ms) and with one launch per
+// operation at 700^2 (64.1 ms; 6: 68.2) and 1024^2 (120.9 ms; 4: 123.7; 6: 133.8).
 int mg_solve(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
-  const int nu = o->poly_degree > 0 ? o->poly_degree : (pl->n_nodes < 100000 ? 6 : 5);
+  const int nu = o->poly_degree > 0 ? o->poly_degree : 5;
   if (nu > 8) return fail(PP_ERR_INVALID, "multigrid: poly_degree (smoothing steps) must be at most 8");
   int rc;
   if (!pl->mg_pc || pl->mg_pc->nu != nu) {

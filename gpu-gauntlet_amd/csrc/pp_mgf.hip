@@ -161,6 +161,10 @@ __device__ __forceinline__ bool stop_rule(double rr, MgfState* S, MgfRec* rec, i
 
 enum { M_STEP = 0, M_RES = 1, M_KP = 2 };
 
+// stencil-level tile sides: steps 16 x 16 nodes, residuals 14 x 14 (a
+// 2-node halo keeps the computed nodes at 16 x 16: one per thread)
+constexpr int kTsS_ = 16, kTrS_ = 14;
+
 template <int TS, int H>
 struct TG {
   static constexpr int PW = TS + 2 * H, NP = PW * PW;  // staged nodes
@@ -309,10 +313,9 @@ __device__ __forceinline__ void stencil_apply(const StRow& R, int sx, int sy, co
   }
 }
 
-// One tile kernel for every fused phase.  L0: level 0 (matrix-free Q4 with
-// the constraint rows) or a stencil level; TS x TS nodes per workgroup,
-// staged with an H-node halo; one computed node (and at level 0 one element)
-// per thread.
+// One tile kernel for every fused level-0 phase (the matrix-free Q4 operator
+// with the constraint rows); TS x TS nodes per workgroup, staged with an
+// H-node halo; one computed node and one element per thread.
 //   M_STEP  xo = x + c1 (x - xo) + c2 B (b - A x) on the tile (PRO: x is
 //           staged as x + P xc and also written to xpro; ZQ: partials of
 //           xo.q, the last workgroup sets beta = -z.q / p.q)
@@ -322,17 +325,17 @@ __device__ __forceinline__ void stencil_apply(const StRow& R, int sx, int sy, co
 //   M_KP    p = z + beta p_old staged, q = K p, p and q stored, partials of
 //           p.q and p.r; the last workgroup sets alpha or the breakdown flag
 // Every operand a thread needs after the staging (its node's right-hand
-// side, block inverse, previous iterate, constraint bits and stencil row,
-// its element's widths and material) is loaded before the staging, so the
-// kernel waits on global memory once.
-template <bool L0, int TS, int H, int NT, int MODE, bool PRO, bool ZQ>
+// side, block inverse, previous iterate and constraint bits, its element's
+// widths and material) is loaded before the staging, so the kernel waits on
+// global memory once.
+template <int TS, int H, int NT, int MODE, bool PRO, bool ZQ>
 __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   using T = TG<TS, H>;
   static_assert(T::NC <= NT, "one computed node per thread");
-  static_assert(!L0 || T::NE <= NT, "one element per thread");
+  static_assert(T::NE <= NT, "one element per thread");
   __shared__ double u_s[3 * T::NP];
   __shared__ double y_s[MODE == M_KP ? 3 * T::NP : 1];  // KP: p_old (p = z + beta p_old)
-  __shared__ double ye_s[L0 ? 12 * T::NE : 1];
+  __shared__ double ye_s[12 * T::NE];
   __shared__ double t_s[MODE == M_RES ? 3 * T::NC : 1];
   __shared__ uint8_t m_s[MODE == M_RES ? T::NC : 1];
   __shared__ double red_s[2 * (NT / 64) + 2];
@@ -352,9 +355,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   unsigned fix = 0;
   double bv[3] = {0.0, 0.0, 0.0}, xov[3] = {0.0, 0.0, 0.0}, qv[3] = {0.0, 0.0, 0.0};
   float mv[6] = {0, 0, 0, 0, 0, 0};
-  StRow R;
   if (in) {
-    if (L0 || MODE == M_RES) fix = A.L.mask[n];
+    fix = A.L.mask[n];
     for (int d = 0; d < 3; d++) bv[d] = A.b[3 * n + d];
     if (MODE == M_STEP) {
       for (int k = 0; k < 6; k++) mv[k] = A.L.minv[6 * n + k];
@@ -363,13 +365,12 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
       if (ZQ)
         for (int d = 0; d < 3; d++) qv[d] = A.q[3 * n + d];
     }
-    if constexpr (!L0) load_st_row(nx, A.L.st, gi, gj, R);
   }
   // ---- this thread's element (level 0)
   const int le = threadIdx.x;
   const int ex = le % T::EW, ey = le / T::EW;
   const int egi = i0 - H + ex, egj = j0 - H + ey;
-  const bool ein = L0 && le < T::NE && egi >= 0 && egi < A.G.nx && egj >= 0 && egj < A.G.ny;
+  const bool ein = le < T::NE && egi >= 0 && egi < A.G.nx && egj >= 0 && egj < A.G.ny;
   double ea = 0, eb = 0, eia = 0, eib = 0, eD = 0, eDs = 0;
   if (ein) {
     ea = A.G.ax[2 * egi];
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   // KP: beta = -z.q / p.q of the last direction (0 on the first iteration)
   const double beta =
       MODE == M_KP ? (A.cin ? -in_s[0] / A.S->pq : A.S->beta) : 0.0;
-  if constexpr (L0) {
+  {
     if (le < T::NE) {
       double y[12];
       if (ein) {
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
         own[d] = MODE == M_KP ? u_s[d * T::NP + si] + beta * y_s[d * T::NP + si]
                               : u_s[d * T::NP + si];
       double Kx[3];
-      if constexpr (L0) {
+      {
         constexpr int EW = T::EW, NE = T::NE;
         const int sw = cy * EW + cx, se = sw + 1, ne = sw + EW + 1, nw = sw + EW;
         for (int d = 0; d < 3; d++) {
@@ -453,8 +454,6 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
                            ye_s[(3 + d) * NE + nw];
           Kx[d] = (fix >> d) & 1u ? own[d] : v;
         }
-      } else {
-        stencil_apply<T::PW, T::NP>(R, sx, sy, u_s, Kx);
       }
       if (MODE == M_RES) {
         for (int d = 0; d < 3; d++) t_s[d * T::NC + c] = bv[d] - Kx[d];
@@ -545,6 +544,173 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
       }
     }
   }
+}
+
+// ----------------------------------------------------------- stencil levels
+//
+// One tile of a stencil level (levels >= 1), as k_tile does it for level 0.
+// COH: vectors are read with device-coherent loads and stored with
+// device-coherent stores, for a caller whose workgroups exchange them within
+// one launch; plain loads and stores otherwise.  (A persistent launch of the
+// whole coarse part of the V-cycle, levels 1 and 2 and the dense solve of
+// the 142^2 plate behind grid barriers, was measured slower than separate
+// launches: 45.7 against 32.2 us per V-cycle, the coherent loads and the
+// barrier of each phase costing more than a launch.)
+
+template <bool COH>
+__device__ __forceinline__ double ldv(const double* p) {
+  if constexpr (COH) return ld_coh(p);
+  else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void stv(double* p, double v) {
+  if constexpr (COH) st_coh(p, v);
+  else *p = v;
+}
+
+// LDS of a stencil tile: the staged x (tile + 2-node halo at most), the
+// residual and the constraint bits of the computed nodes (tile + 1 ring).
+struct StSmem {
+  double u[3 * 18 * 18];
+  double t[3 * 16 * 16];
+  uint8_t m[16 * 16];
+};
+
+template <int TS, int H, int MODE, bool PRO, bool COH>
+__device__ __forceinline__ void st_tile(const TileArgs& A, int tile, StSmem& sm) {
+  using T = TG<TS, H>;
+  static_assert(T::NP <= 18 * 18 && T::NC <= 16 * 16, "tile larger than the LDS plan");
+  static_assert(T::NC <= 256, "one computed node per thread");
+  const int nx = A.L.nx, ny = A.L.ny;
+  const int ntx = (nx + 1 + TS - 1) / TS;
+  const int i0 = (tile % ntx) * TS, j0 = (tile / ntx) * TS;
+  const int c = threadIdx.x;
+  const int cx = c % T::CW, cy = c / T::CW;
+  const int sx = cx + 1, sy = cy + 1;
+  const int gi = i0 - H + sx, gj = j0 - H + sy;
+  const bool in = c < T::NC && gi >= 0 && gi <= nx && gj >= 0 && gj <= ny;
+  const int64_t n = in ? (int64_t)gj * (nx + 1) + gi : 0;
+  unsigned fix = 0;
+  double bv[3] = {0.0, 0.0, 0.0}, xov[3] = {0.0, 0.0, 0.0};
+  float mv[6] = {0, 0, 0, 0, 0, 0};
+  StRow R;
+  if (in) {
+    if (MODE == M_RES) fix = A.L.mask[n];
+    for (int d = 0; d < 3; d++) bv[d] = ldv<COH>(A.b + 3 * n + d);
+    if (MODE == M_STEP) {
+      for (int k = 0; k < 6; k++) mv[k] = A.L.minv[6 * n + k];
+      if (A.c1 != 0.0)
+        for (int d = 0; d < 3; d++) xov[d] = ldv<COH>(A.xo + 3 * n + d);
+    }
+    load_st_row(nx, A.L.st, gi, gj, R);
+  }
+  constexpr int CWc = TS / 2 + 2;
+  static_assert(CWc * CWc <= 256, "one coarse node per thread");
+  int cI = -1, cJ = -1;
+  float cm[6] = {0, 0, 0, 0, 0, 0};
+  if (MODE == M_RES && (int)threadIdx.x < CWc * CWc) {
+    const int I = i0 / 2 + threadIdx.x % CWc, J = j0 / 2 + threadIdx.x / CWc;
+    const int fi0 = min(2 * I, nx), fj0 = min(2 * J, ny);
+    if (I <= A.co.nx && J <= A.co.ny && fi0 >= i0 && fi0 < i0 + TS && fj0 >= j0 &&
+        fj0 < j0 + TS) {
+      cI = I;
+      cJ = J;
+      if (A.co.minv)
+        for (int q = 0; q < 6; q++) cm[q] = A.co.minv[6 * ((int64_t)J * (A.co.nx + 1) + I) + q];
+    }
+  }
+  // stage x (+ P xc, masked) on the tile and its halo
+  for (int idx = threadIdx.x; idx < T::NP; idx += 256) {
+    const int lx = idx % T::PW, ly = idx / T::PW;
+    const int hi = i0 - H + lx, hj = j0 - H + ly;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (hi >= 0 && hi <= nx && hj >= 0 && hj <= ny) {
+      const int64_t m = (int64_t)hj * (nx + 1) + hi;
+      for (int d = 0; d < 3; d++) v[d] = ldv<COH>(A.x + 3 * m + d);
+      if (PRO) {
+        const int q0 = hi / 2, q1 = (hi + 1) / 2, r0 = hj / 2, r1 = (hj + 1) / 2;
+        double sp[3] = {0.0, 0.0, 0.0};
+        const int ci[2] = {q0, q1}, cj[2] = {r0, r1};
+        const int ni = q0 == q1 ? 1 : 2, nj = r0 == r1 ? 1 : 2;
+        for (int bb = 0; bb < nj; bb++)
+          for (int aa = 0; aa < ni; aa++) {
+            const double w = w1(hi, ci[aa], nx) * w1(hj, cj[bb], ny);
+            const int64_t mc = (int64_t)cj[bb] * (A.pro.cnx + 1) + ci[aa];
+            for (int d = 0; d < 3; d++) sp[d] += w * ldv<COH>(A.pro.xc + 3 * mc + d);
+          }
+        const unsigned msk = A.L.mask[m];
+        for (int d = 0; d < 3; d++)
+          if (!(msk >> d & 1)) v[d] += sp[d];
+      }
+    }
+    for (int d = 0; d < 3; d++) sm.u[d * T::NP + idx] = v[d];
+  }
+  __syncthreads();
+  if (c < T::NC) {
+    if (!in) {
+      if (MODE == M_RES) {
+        for (int d = 0; d < 3; d++) sm.t[d * T::NC + c] = 0.0;
+        sm.m[c] = 0;
+      }
+    } else {
+      const int si = sy * T::PW + sx;
+      double Kx[3];
+      stencil_apply<T::PW, T::NP>(R, sx, sy, sm.u, Kx);
+      if (MODE == M_RES) {
+        for (int d = 0; d < 3; d++) sm.t[d * T::NC + c] = bv[d] - Kx[d];
+        sm.m[c] = (uint8_t)fix;
+      } else {
+        double r[3], z[3];
+        for (int d = 0; d < 3; d++) r[d] = bv[d] - Kx[d];
+        sym3_apply(mv, r, z);
+        for (int d = 0; d < 3; d++) {
+          const double xc = sm.u[d * T::NP + si];
+          const double dn = (A.c1 != 0.0 ? A.c1 * (xc - xov[d]) : 0.0) + A.c2 * z[d];
+          stv<COH>(A.xo + 3 * n + d, xc + dn);
+          if (PRO) stv<COH>(A.pro.xpro + 3 * n + d, xc);
+        }
+      }
+    }
+  }
+  if (MODE == M_RES) {
+    __syncthreads();
+    const Coarse& co = A.co;
+    if (cI >= 0) {
+      const int I = cI, J = cJ;
+      const int64_t nc = (int64_t)J * (co.nx + 1) + I;
+      double sr[3] = {0.0, 0.0, 0.0};
+      for (int bb = -1; bb <= 1; bb++) {
+        const int fj = 2 * J + bb;
+        if (fj < 0 || fj > ny) continue;
+        for (int aa = -1; aa <= 1; aa++) {
+          const int fi = 2 * I + aa;
+          if (fi < 0 || fi > nx) continue;
+          const double w = w1(fi, I, nx) * w1(fj, J, ny);
+          const int ci = (fj - (j0 - H + 1)) * T::CW + (fi - (i0 - H + 1));
+          const unsigned msk = sm.m[ci];
+          for (int d = 0; d < 3; d++)
+            if (!(msk >> d & 1)) sr[d] += w * sm.t[d * T::NC + ci];
+        }
+      }
+      for (int d = 0; d < 3; d++) stv<COH>(co.b + 3 * nc + d, sr[d]);
+      if (co.minv) {
+        double z[3];
+        sym3_apply(cm, sr, z);
+        for (int d = 0; d < 3; d++) {
+          stv<COH>(co.cur + 3 * nc + d, 0.0 + co.c2 * z[d]);
+          stv<COH>(co.prev + 3 * nc + d, 0.0);
+        }
+      }
+    }
+  }
+}
+
+// One launch per stencil-level phase.
+template <int TS, int H, int MODE, bool PRO>
+__global__ __launch_bounds__(256) void k_st(TileArgs A) {
+  __shared__ StSmem sm;
+  if (A.S->done) return;
+  st_tile<TS, H, MODE, PRO, false>(A, blockIdx.x, sm);
 }
 
 // x += alpha p, r -= alpha q with alpha = p.r / p.q, the partials of r.r,
@@ -689,7 +855,7 @@ int ntiles(int nx, int ny, int ts) { return ((nx + 1 + ts - 1) / ts) * ((ny + 1 
 // tile shapes: level-0 steps and K p on 15 x 15 nodes (16 x 16 elements: one
 // element per thread), level-0 residuals on 13 x 13 (a 2-node halo: again
 // 16 x 16 elements); stencil levels 16 x 16 and 14 x 14
-constexpr int kTs0 = 15, kTr0 = 13, kTsS = 16, kTrS = 14;
+constexpr int kTs0 = 15, kTr0 = 13, kTsS = kTsS_, kTrS = kTrS_;
 
 struct Coef {
   double c1, c2;
@@ -825,19 +991,19 @@ struct Fused {
     if (l == 0) {
       const int g = ntiles(nx, ny, kTs0);
       if (pro && zq)
-        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, true, true>), dim3(g), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_STEP, true, true>), dim3(g), dim3(256), 0, s, A);
       else if (pro)
-        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, true, false>), dim3(g), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_STEP, true, false>), dim3(g), dim3(256), 0, s, A);
       else if (zq)
-        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, false, true>), dim3(g), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_STEP, false, true>), dim3(g), dim3(256), 0, s, A);
       else
-        hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_STEP, false, false>), dim3(g), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_STEP, false, false>), dim3(g), dim3(256), 0, s, A);
     } else {
       const int g = ntiles(nx, ny, kTsS);
       if (pro)
-        hipLaunchKernelGGL((k_tile<false, kTsS, 1, 256, M_STEP, true, false>), dim3(g), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_st<kTsS, 1, M_STEP, true>), dim3(g), dim3(256), 0, s, A);
       else
-        hipLaunchKernelGGL((k_tile<false, kTsS, 1, 256, M_STEP, false, false>), dim3(g), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_st<kTsS, 1, M_STEP, false>), dim3(g), dim3(256), 0, s, A);
     }
     PP_HIP(hipGetLastError());
     return PP_OK;
@@ -860,12 +1026,13 @@ struct Fused {
       A.co.prev = C.buf[0];
     }
     const int nx = L[l].v.nx, ny = L[l].v.ny;
-    if (l == 0)
-      hipLaunchKernelGGL((k_tile<true, kTr0, 2, 256, M_RES, false, false>),
+    if (l == 0) {
+      hipLaunchKernelGGL((k_tile<kTr0, 2, 256, M_RES, false, false>),
                          dim3(ntiles(nx, ny, kTr0)), dim3(256), 0, s, A);
-    else
-      hipLaunchKernelGGL((k_tile<false, kTrS, 2, 256, M_RES, false, false>),
-                         dim3(ntiles(nx, ny, kTrS)), dim3(256), 0, s, A);
+    } else {
+      hipLaunchKernelGGL((k_st<kTrS, 2, M_RES, false>), dim3(ntiles(nx, ny, kTrS)), dim3(256), 0,
+                         s, A);
+    }
     PP_HIP(hipGetLastError());
     return PP_OK;
   }
@@ -930,7 +1097,7 @@ struct Fused {
       A.cin = part + 3 * (size_t)maxwg;
       A.cn = ntiles(L[0].v.nx, L[0].v.ny, kTs0);
     }
-    hipLaunchKernelGGL((k_tile<true, kTs0, 1, 256, M_KP, false, false>),
+    hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_KP, false, false>),
                        dim3(ntiles(L[0].v.nx, L[0].v.ny, kTs0)), dim3(256), 0, s, A);
     PP_HIP(hipGetLastError());
     return PP_OK;
@@ -1065,6 +1232,7 @@ int mgf_solve(pp_plan* pl, Mg* mg, const pp_solve_opts* o, pp_solve_stats* st) {
   Fused F;
   if ((rc = F.setup(pl, mg, S.p, part.p, mw))) return rc;
   F.consumer = mw <= kConsumerMax;
+
   PP_HIP(hipEventRecord(pl->ctx->ev0, s));
   hipLaunchKernelGGL(k_fill0, dim3(nblk(n, 256)), dim3(256), 0, s, n, pl->x.p);
   PP_HIP(hipMemcpyAsync(pl->r.p, pl->F.p, n * 8, hipMemcpyDeviceToDevice, s));

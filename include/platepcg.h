@@ -98,7 +98,7 @@ typedef struct {
   int32_t flags;           /* PP_FLAG_* */
   int32_t poly_degree;     /* PP_PRECOND_CHEBYSHEV: degree (<= 0: 8, at most 32);
                               PP_PRECOND_MULTIGRID: smoothing steps on the finest level
-                              (<= 0: 6 below 100,000 nodes, else 5; at most 8) */
+                              (<= 0: 5; at most 8) */
 } pp_solve_opts;
 
 /* Skip the FP64 |F - K x| / |F| recomputation after the solve, and with it
