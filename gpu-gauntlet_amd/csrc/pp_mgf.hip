@@ -46,12 +46,11 @@ namespace {
 struct MgfState {
   double alpha, beta, pq, rr, fnorm, tol;
   int it, done, converged, breakdown;
-  unsigned cnt[4];  // arrival counters of the fused reductions (reset by the last arrival)
 };
 
 // Progress record of one iteration, written by the device into coherent
-// pinned host memory (slot = iteration % ring) by the last workgroup of the
-// update (or, on a breakdown, of K p); seq = iteration + 1 is stored last.
+// pinned host memory (slot = iteration % ring) where the stopping rule (or
+// the breakdown rule) is applied; seq = iteration + 1 is stored last.
 struct MgfRec {
   double rr;
   int it, done, converged, breakdown;
@@ -93,13 +92,11 @@ struct Pro {
   double* xpro;      // out: the prolonged iterate on own nodes
 };
 
-// A fused reduction: per-workgroup partials [value][workgroup], plain
-// stores; with `last`, the last workgroup to arrive also finishes the sum
-// into the state (arrival counter `slot`).
+// A fused reduction: per-workgroup partials [value][workgroup] (plain
+// stores), summed by every workgroup of the consuming kernel (few
+// producers) or by k_finish (many).
 struct Red {
   double* part;
-  int slot;
-  int last;
 };
 
 struct TileArgs {
@@ -192,40 +189,6 @@ __device__ __forceinline__ void prolong_node(int fi, int fj, int fnx, int fny, c
       const int64_t m = (int64_t)cj[b] * (cnx + 1) + ci[a];
       for (int d = 0; d < 3; d++) s[d] += w * xc[3 * m + d];
     }
-}
-
-// Fixed-order grid sum of NV partials, finished by the last workgroup to
-// arrive.  v: this workgroup's partials (valid in thread 0).  Returns true
-// in every thread of the last workgroup, with the totals in v of thread 0.
-template <int NV, int NT>
-__device__ bool grid_finish(double (&v)[NV], double* part, unsigned* cnt, unsigned nwg,
-                            unsigned me, double* lds, int* flag) {
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++)
-      __hip_atomic_store(&part[(size_t)k * nwg + me], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned old =
-        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *flag = old == nwg - 1;
-  }
-  __syncthreads();
-  if (!*flag) return false;
-  double s[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) s[k] = 0.0;
-  for (unsigned w = threadIdx.x; w < nwg; w += NT)
-#pragma unroll
-    for (int k = 0; k < NV; k++)
-      s[k] += __hip_atomic_load(&part[(size_t)k * nwg + w], __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV, NT>(s, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = s[k];
-    __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return true;
 }
 
 // Stage src + beta y (+ P xc, masked) on the tile plus an H-node halo; zero
@@ -340,7 +303,6 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   __shared__ uint8_t m_s[MODE == M_RES ? T::NC : 1];
   __shared__ double red_s[2 * (NT / 64) + 2];
   __shared__ double in_s[2];
-  __shared__ int flag_s;
   if (A.S->done) return;
   const int nx = A.L.nx, ny = A.L.ny;
   const int ntx = (nx + 1 + TS - 1) / TS;
@@ -520,29 +482,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
     double v[NV];
     for (int k = 0; k < NV; k++) v[k] = acc[k];
     block_sum<NV, NT>(v, red_s);
-    const unsigned nwg = gridDim.x;
-    if (!A.red.last) {
-      if (threadIdx.x == 0)
-        for (int k = 0; k < NV; k++) A.red.part[(size_t)k * nwg + blockIdx.x] = v[k];
-      return;
-    }
-    if (!grid_finish<NV, NT>(v, A.red.part, &A.S->cnt[A.red.slot], nwg, blockIdx.x, red_s, &flag_s))
-      return;
-    if (threadIdx.x == 0) {
-      MgfState* S = A.S;
-      if (MODE == M_KP) {
-        S->pq = v[0];
-        if (fabs(v[0]) < 1e-30) {
-          S->breakdown = 1;
-          S->done = 1;
-          write_record(A.rec, A.nrec, S->it, S);
-        } else {
-          S->alpha = v[NV - 1] / v[0];
-        }
-      } else {
-        S->beta = -v[0] / S->pq;
-      }
-    }
+    if (threadIdx.x == 0)
+      for (int k = 0; k < NV; k++) A.red.part[(size_t)k * gridDim.x + blockIdx.x] = v[k];
   }
 }
 
@@ -716,9 +657,9 @@ __global__ __launch_bounds__(256) void k_st(TileArgs A) {
 // x += alpha p, r -= alpha q with alpha = p.r / p.q, the partials of r.r,
 // and the V-cycle's first level-0 step from x = 0 on the new r: cur = c2 B r,
 // prev = 0.  Consumer form (cin): every workgroup sums the K p partials
-// [p.q | p.r] itself and applies the breakdown rule (pcg.ts:154); the
-// kernel after this one applies the stopping rule.  Otherwise alpha comes
-// from the state and the last workgroup applies the stopping rule.
+// [p.q | p.r] itself and applies the breakdown rule (pcg.ts:154), and the
+// kernel after this one applies the stopping rule; otherwise alpha comes
+// from the state and k_finish applies the stopping rule.
 __global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r, const double* p,
                                                 const double* q, const float* minv, double c2,
                                                 double* cur, double* prev, const double* cin,
@@ -726,7 +667,6 @@ __global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r
                                                 int nrec) {
   __shared__ double red_s[8];
   __shared__ double in_s[2];
-  __shared__ int flag_s;
   if (S->done) return;
   const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, qv[3] = {0, 0, 0};
@@ -785,13 +725,35 @@ __global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r
     }
   }
   block_sum<1, 256>(v, red_s);
-  if (!red.last) {
-    if (threadIdx.x == 0) red.part[blockIdx.x] = v[0];
-    return;
-  }
-  if (!grid_finish<1, 256>(v, red.part, &S->cnt[red.slot], gridDim.x, blockIdx.x, red_s, &flag_s))
-    return;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) red.part[blockIdx.x] = v[0];
+}
+
+// The scalar step after a reduction with many producers, in one 1024-thread
+// workgroup: a fixed-order sum of the n partials, then
+//   kind 0 (after K p): alpha = p.r / p.q or the breakdown rule (pcg.ts:154);
+//   kind 1 (after the update): the stopping rule (pcg.ts:174), the iteration
+//           count and its progress record;
+//   kind 2 (after the last smoothing step): beta = -z.q / p.q.
+__global__ __launch_bounds__(1024) void k_finish(int kind, const double* part, int n, MgfState* S,
+                                                 MgfRec* rec, int nrec) {
+  __shared__ double lds[2 * 16];
+  if (S->done) return;
+  double v[2] = {0.0, 0.0};
+  const int nv = kind == 0 ? 2 : 1;
+  for (int i = threadIdx.x; i < n; i += 1024)
+    for (int k = 0; k < nv; k++) v[k] += part[(size_t)k * n + i];
+  block_sum<2, 1024>(v, lds);
+  if (threadIdx.x) return;
+  if (kind == 0) {
+    S->pq = v[0];
+    if (fabs(v[0]) < 1e-30) {
+      S->breakdown = 1;
+      S->done = 1;
+      write_record(rec, nrec, S->it, S);
+    } else {
+      S->alpha = v[1] / v[0];
+    }
+  } else if (kind == 1) {
     S->rr = v[0];
     S->it++;
     const double rn = sqrt(v[0]);
@@ -800,6 +762,8 @@ __global__ __launch_bounds__(256) void k_update(int64_t nn, double* x, double* r
       S->done = 1;
     }
     write_record(rec, nrec, S->it - 1, S);
+  } else {
+    S->beta = -v[0] / S->pq;
   }
 }
 
@@ -985,7 +949,7 @@ struct Fused {
     if (pro) A.pro = *pro;
     if (zq) {
       A.q = pl->Ap.p;
-      A.red = {part + 3 * (size_t)maxwg, 2, !consumer};
+      A.red = {part + 3 * (size_t)maxwg};
     }
     const int nx = L[l].v.nx, ny = L[l].v.ny;
     if (l == 0) {
@@ -998,6 +962,8 @@ struct Fused {
         hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_STEP, false, true>), dim3(g), dim3(256), 0, s, A);
       else
         hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_STEP, false, false>), dim3(g), dim3(256), 0, s, A);
+      PP_HIP(hipGetLastError());
+      if (zq && !consumer) return finish(2, A.red.part, g);
     } else {
       const int g = ntiles(nx, ny, kTsS);
       if (pro)
@@ -1092,13 +1058,20 @@ struct Fused {
     A.xo = pnew;
     A.b = pl->r.p;
     A.q = pl->Ap.p;
-    A.red = {part, 0, !consumer};
+    A.red = {part};
     if (consumer && !first) {
       A.cin = part + 3 * (size_t)maxwg;
       A.cn = ntiles(L[0].v.nx, L[0].v.ny, kTs0);
     }
-    hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_KP, false, false>),
-                       dim3(ntiles(L[0].v.nx, L[0].v.ny, kTs0)), dim3(256), 0, s, A);
+    const int g = ntiles(L[0].v.nx, L[0].v.ny, kTs0);
+    hipLaunchKernelGGL((k_tile<kTs0, 1, 256, M_KP, false, false>), dim3(g), dim3(256), 0, s, A);
+    PP_HIP(hipGetLastError());
+    return consumer ? PP_OK : finish(0, part, g);
+  }
+
+  // the scalar step of a reduction with many producers (k_finish)
+  int finish(int kind, const double* pt, int n) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, kind, pt, n, S, rec, nrec);
     PP_HIP(hipGetLastError());
     return PP_OK;
   }
@@ -1107,13 +1080,13 @@ struct Fused {
     const Lv& V = L[0];
     const double* cin = consumer ? part : nullptr;
     const int cn = ntiles(V.v.nx, V.v.ny, kTs0);
-    Red red{part + 2 * (size_t)maxwg, 1, !consumer};
+    Red red{part + 2 * (size_t)maxwg};
     hipLaunchKernelGGL(k_update, dim3(nblk(V.nn, 256)), dim3(256), 0, s, V.nn, pl->x.p, pl->r.p,
                        p, (const double*)pl->Ap.p, V.v.minv, V.cs[0].c2, V.buf[pc0], V.buf[pp0],
                        cin, cn, red, S, rec, nrec);
     PP_HIP(hipGetLastError());
     pending_check = consumer;
-    return PP_OK;
+    return consumer ? PP_OK : finish(1, red.part, nblk(V.nn, 256));
   }
 
   // z = V r from the current r (start of a run)

@@ -124,7 +124,7 @@ def test_restarts_near_the_fp64_floor(ctx):
 
 
 @pytest.mark.parametrize("case", [("c1", 6), ("c2", 1), ("c2", 3), ("c2", 5), ("c2", 6),
-                                  ("rect", 4)])
+                                  ("rect", 4), ("large", 5)])
 def test_fused_launches_match_launch_per_operation(ctx, case):
     """The fused iteration (19 launches on the 142^2 plate: K p with the
     direction update, x/r update with the first smoothing step, residual with
@@ -134,9 +134,18 @@ def test_fused_launches_match_launch_per_operation(ctx, case):
     count within one, and the same displacements to 1e-8 of max |w|.  Odd and
     even step counts (the buffer rotation ends in z either way), a plate with
     no stencil level (32^2: level 0 then the dense coarsest) and a
-    rectangular odd-sized plate."""
+    rectangular odd-sized plate; at 400 x 400 (729 tiles) the reductions are
+    finished by their own single-workgroup launch instead of by every
+    consuming workgroup."""
     name, steps = case
-    if name == "rect":
+    if name == "large":
+        n = 400
+        mesh = P.rect_mesh(n)
+        mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+        et = (0.1 * np.exp(0.4 * np.random.default_rng(3).standard_normal(n * n))).astype(np.float32)
+        plan = Plan(ctx, mesh, E, NU, 0.1, mask, element_t=et)
+        F = plan.pressure_load(1e4, np.array([[3e4, 4.0, 6.0, 1.0]]))
+    elif name == "rect":
         mesh = P.rect_mesh(90, 37, width=10.0, height=4.0)
         mask = P.edge_mask(90, 37, ["fixed", "pinned", "fixed", "pinned"])
         plan = Plan(ctx, mesh, E, NU, 0.05, mask)
