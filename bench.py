@@ -1,13 +1,20 @@
 """Benchmark: ms per plate solve to relative residual 1e-8 on one MI355X.
 
-Workload (default): 142 x 142 Q4 Mindlin plate, 61,347 DOF, clamped edges,
-E = 210 GPa, nu = 0.3, t = 0.1 m, 10 kPa + 50 kPa Gaussian patch (sigma 1 m,
-centre from SplitMix64 seed 1); FP64 block-Jacobi PCG, whole loop on the
-device.  A step is one full solve of the resident load vector.
-
-Prints one JSON line (rank 0).  Extra fields: the roofline of the persistent
-PCG kernel and the CPU baseline (the float32 restatement of solver.ts run in
-Node on a bounded number of iterations, extrapolated to its full count).
+Default run: the whole metric, "ms/solve to 1e-8 rel. residual at 61k & 3M
+DOF; learned vs block-Jacobi PCG".
+  value  142 x 142 Q4 Mindlin plate (61,347 DOF), clamped, E = 210 GPa,
+         nu = 0.3, t = 0.1 m, 10 kPa + 50 kPa Gaussian patch (sigma 1 m,
+         centre from SplitMix64 seed 1): FP64 block-Jacobi PCG, whole loop on
+         the device; a step is one full solve of the resident load vector.
+  c2     the same plate with the other preconditioners: Chebyshev-polynomial
+         block Jacobi, multigrid PCG, the learned preconditioner (trained
+         first, off the clock) and the same preconditioner with a zero network.
+  c5     1024 x 1024 plate (3,151,875 DOF) with a log-normal thickness
+         field: block-Jacobi PCG, multigrid PCG, learned and zero-network.
+Prints one JSON line.  roofline: the persistent block-Jacobi kernel of the
+value, with HBM traffic measured in the same run by two rocprofv3 --pmc child
+runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain copy;
+cpu_baseline: the float32 restatement of solver.ts in Node on the host.
 """
 import argparse
 import json
@@ -27,6 +34,9 @@ BYTES_PER_DOF_ITER_FP64 = 72
 BYTES_PER_DOF_ITER_MIXED = 48
 
 
+METRIC = "ms/solve to 1e-8 rel. residual at 61k & 3M DOF; learned vs block-Jacobi PCG"
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,32 +48,118 @@ def parse():
                          "learned preconditioner vs block Jacobi")
     ap.add_argument("--validate", action="store_true",
                     help="c4: compare every plate with the single-plate FP64 solve")
-    ap.add_argument("--train-loads", type=int, default=8,
-                    help="c5: load cases for the solver-generated training pairs")
-    ap.add_argument("--train-epochs", type=int, default=3, help="c5: training epochs")
-    ap.add_argument("--learned-max-iterations", type=int, default=40000,
-                    help="c5: iteration cap of the learned-preconditioner solve")
-    ap.add_argument("--learned-smoother", default="mg", choices=["mg", "bj"],
-                    help="c5: smoother around the network: Galerkin multigrid V-cycle (mg) or "
-                         "damped block Jacobi (bj)")
+    ap.add_argument("--train-loads", type=int, default=4,
+                    help="c5: load cases for the solver-generated training pairs (8 pairs each)")
+    ap.add_argument("--train-epochs", type=int, default=1, help="c5: training epochs")
     ap.add_argument("--cpu-sample-iters", type=int, default=-1,
                     help="iterations of the CPU baseline sample (0 = skip; default 1200, "
                          "150 for c3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (profiler calibration)")
+    ap.add_argument("--probe", choices=["copy", "pcg"],
+                    help="only run the traffic probe (a 256 MiB copy, or one 142 x 142 "
+                         "block-Jacobi solve) for the rocprofv3 counter runs")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs (traffic: null)")
+    ap.add_argument("--no-c5", action="store_true", help="default run without the 1024 x 1024 legs")
+    ap.add_argument("--c5-steps", type=int, default=2,
+                    help="timed solves per leg of the 1024 x 1024 plate (block Jacobi 3.7 s each)")
+    ap.add_argument("--c2-train-loads", type=int, default=32,
+                    help="load cases for the 142 x 142 network's training pairs")
     return ap.parse_args()
 
 
 def calibrate_copy(mib):
-    """A plain copy of known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
-    import torch
+    """A plain device-to-device copy of known size (the HIP runtime's copy
+    kernel), for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import ctypes as C
 
-    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = mib * 1024 * 1024
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), nbytes) or hip.hipMalloc(C.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, nbytes)
     for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 5}))
+        if hip.hipMemcpy(b, a, nbytes, 3):  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 5}))
+
+
+def probe_pcg():
+    """One 142 x 142 block-Jacobi solve (one launch of the persistent kernel)."""
+    from platepcg import Context, Plan, _lib
+    from platepcg import problem as P
+    w = P.workload("c2")
+    ctx = Context(0)
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    plan.set_rhs(plan.pressure_load(w["q0"], w["patches"]))
+    plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=400000,
+                                     flags=_lib.FLAG_NO_TRUE_RESIDUAL))
+    plan.close()
+    ctx.close()
+
+
+def measure_traffic(kernel_prefix="k_pcg_cg"):
+    """HBM bytes per launch of the persistent block-Jacobi kernel on the
+    142 x 142 plate: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), one child run per counter, each scaled by what the same counter
+    reads for a 256 MiB device-to-device copy (FETCH_SIZE reads 2x the copied
+    bytes, WRITE_SIZE 1x on MI355X).  Runs before this process touches the GPU.
+    Returns (bytes, details) or (None, reason)."""
+    import csv
+    import shutil
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, {"error": "rocprofv3 not found"}
+    out = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            for probe in ("copy", "pcg"):
+                tag = f"{ctr}_{probe}"
+                cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", os.path.join(d, tag),
+                       "-o", "r", "--", sys.executable, os.path.abspath(__file__), "--probe", probe]
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, cwd=ROOT)
+                if r.returncode != 0:
+                    return None, {"error": f"rocprofv3 {tag} exit {r.returncode}: "
+                                           f"{(r.stderr or '')[-300:]}"}
+                rows = []
+                for root, _, files in os.walk(os.path.join(d, tag)):
+                    for f in files:
+                        if f.endswith("counter_collection.csv"):
+                            with open(os.path.join(root, f)) as fh:
+                                rows += list(csv.DictReader(fh))
+                per = {}
+                for row in rows:
+                    if row.get("Counter_Name") != ctr:
+                        continue
+                    key = (row["Dispatch_Id"], row["Kernel_Name"])
+                    per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+                if probe == "copy":
+                    vals = [v for (_, k), v in per.items() if "copy" in k.lower()]
+                else:
+                    vals = [v for (_, k), v in per.items() if kernel_prefix in k]
+                if not vals:
+                    return None, {"error": f"no {tag} dispatch in the counter output"}
+                out[tag] = max(vals)
+    copy_kib = 256 * 1024
+    fs = out["FETCH_SIZE_copy"] / copy_kib
+    ws = out["WRITE_SIZE_copy"] / copy_kib
+    fetch = out["FETCH_SIZE_pcg"] / fs * 1024
+    write = out["WRITE_SIZE_pcg"] / ws * 1024
+    return int(fetch + write), {"fetch_bytes": int(fetch), "write_bytes": int(write),
+                                "fetch_scale": fs, "write_scale": ws,
+                                "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, one child run "
+                                          "each, scaled by the same counters on a 256 MiB copy"}
 
 
 def cpu_baseline(w, F, full_iters, sample_iters, basis="iterations it needs to reach 1e-8"):
@@ -78,12 +174,17 @@ def cpu_baseline(w, F, full_iters, sample_iters, basis="iterations it needs to r
                             [{"type": "fixed", "location": "all_edges"}], tol=1e-8,
                             max_iter=sample_iters)
     ms_per_iter = st["solveTimeMs"] / max(1, st["iterations"])
+    whole = st["iterations"] >= full_iters or st.get("converged")
     return {
-        "value": ms_per_iter * full_iters, "unit": "ms/solve", "cores": 1, "kind": "port",
-        "sample": (f"float32 solver.ts restatement in Node {st['node']} on "
-                   f"{st['cpuModel']}: {st['iterations']} PCG iterations timed "
-                   f"({st['solveTimeMs']:.0f} ms), extrapolated x {full_iters} {basis}"),
-        "ms_per_iteration": ms_per_iter,
+        "value": st["solveTimeMs"] if whole else ms_per_iter * full_iters, "unit": "ms/solve",
+        "cores": 1, "kind": "port", "host_cpus": st.get("cpus"),
+        "sample": (f"float32 solver.ts restatement in Node {st['node']} on {st['cpuModel']} "
+                   f"({st.get('cpus')} host CPUs, one thread used): "
+                   + (f"the whole solve, {st['iterations']} PCG iterations to 1e-8 "
+                      f"({st['solveTimeMs']:.0f} ms)" if whole else
+                      f"{st['iterations']} PCG iterations timed ({st['solveTimeMs']:.0f} ms), "
+                      f"extrapolated x {full_iters} {basis}")),
+        "ms_per_iteration": ms_per_iter, "iterations": st["iterations"],
     }
 
 
@@ -187,127 +288,143 @@ def bench_batch(a):
 
 
 def bench_learned(a):
-    """1024 x 1024 plate with a log-normal thickness field: block-Jacobi PCG
-    against flexible PCG with the learned preconditioner, both to 1e-8 true
-    relative residual.  The network is trained first (not timed) on pairs the
-    solver generates on independent load cases."""
+    """--workload c5: the 1024 x 1024 legs alone, the learned solve as value."""
+    res = c5_legs(a)
+    ln = res["learned"]
+    out = {
+        "metric": METRIC, "value": ln["ms"], "unit": "ms/solve", "n_gpus": 1,
+        "steps": 3, "warmup": 1, "ms_per_step": ln["ms"], "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64+bf16", "data": "synthetic",
+        "config": {"workload": res.pop("workload"), "dof": res["dof"],
+                   "precond": "learned U-net (16/32/64, BF16 MFMA) + Galerkin multigrid V-cycle "
+                              "smoothing, flexible PCG (FCG(4))",
+                   "precision": "fp64 outer, bf16 network"},
+    }
+    out.update(res)
+    print(json.dumps(out))
+
+
+def timed_leg(plan, o, warmup, steps):
+    """warmup + steps solves of the resident load with options o; wall and
+    device ms per solve, iterations, true residual of the last."""
     import numpy as np
 
-    from platepcg import Context, Net, Plan, _lib
+    for _ in range(warmup):
+        plan.solve_resident(o)
+    t0, dev = time.perf_counter(), []
+    for _ in range(steps):
+        st = plan.solve_resident(o)
+        dev.append(st.device_ms)
+    return {"ms": (time.perf_counter() - t0) * 1e3 / steps, "device_ms": float(np.mean(dev)),
+            "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
+            "converged": bool(st.converged), "sweeps": st.sweeps}
+
+
+def learned_legs(plan, F, train_loads, epochs, warmup, steps, x_ref, seed=5):
+    """The learned preconditioner (3-level U-net with a multigrid V-cycle
+    around it, flexible PCG) trained first on solver-generated pairs (block-
+    Jacobi PCG iterates of independent load cases, 8 per case; not timed), then
+    timed; and the same preconditioner with every network parameter zero."""
+    import numpy as np
+
+    from platepcg import Net, _lib
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    net = Net(plan, seed=seed, multigrid=True)
+    t0 = time.perf_counter()
+    first = last = None
+    for k in range(train_loads):  # one load case per call, so progress stays visible
+        tr = net.train(pairs=1, snapshots=8, epochs=epochs, seed0=1000 + k, lr=1e-3, lr_final=1e-5,
+                       max_grad_norm=1.0, source=_lib.TRAIN_SOLVER)
+        first = tr["loss_first"] if first is None else first
+        last = tr["loss_last"]
+        if (k + 1) % 8 == 0 or k + 1 == train_loads:
+            print(f"training {plan.n_dof} DOF: {k + 1}/{train_loads} load cases, loss "
+                  f"{tr['loss_first']:.3f} -> {tr['loss_last']:.3f}, "
+                  f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    train_s = time.perf_counter() - t0
+    plan.set_rhs(F)  # training replaced the resident load
+    net.attach()
+    lo = plan.options(tolerance=1e-8, max_iterations=40000, precond=_lib.PRECOND_LEARNED)
+    trained = net.params()
+    out = {}
+    for tag, params in (("learned", trained), ("zero_network", np.zeros_like(trained))):
+        net.set_params(params)
+        out[tag] = timed_leg(plan, lo, warmup, steps)
+        out[tag]["max_nodal_w_error_vs_block_jacobi"] = O.max_nodal_w_error(plan.solution()[0],
+                                                                            x_ref)
+        print(f"{tag} {plan.n_dof} DOF: {out[tag]['iterations']} iterations, "
+              f"{out[tag]['ms']:.1f} ms", file=sys.stderr, flush=True)
+    net.detach()
+    net.close()
+    ok = out["learned"]["converged"] and out["zero_network"]["converged"]
+    out["network_iteration_gain"] = (out["zero_network"]["iterations"] /
+                                     max(1, out["learned"]["iterations"]) if ok else None)
+    out["network_time_gain"] = (out["zero_network"]["ms"] / out["learned"]["ms"] if ok else None)
+    out["training"] = {"source": "block-Jacobi PCG iterates (x* - x_k) of independent seeded "
+                                 "pressure loads", "pairs": train_loads * 8, "epochs": epochs,
+                       "steps": train_loads * 8 * epochs, "seconds": train_s,
+                       "loss_first": first, "loss_last": last}
+    return out
+
+
+def c5_legs(a):
+    """The 1024 x 1024 variable-thickness plate (3.15M DOF): block-Jacobi PCG,
+    multigrid PCG, the learned preconditioner and its zero-network form, each
+    to 1e-8 true relative residual."""
+    import numpy as np
+
+    from platepcg import Context, Plan, _lib
     from platepcg import problem as P
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as O
     w = P.workload("c5")
     ctx = Context(0)
     plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"], element_t=w["element_t"])
-    mg = a.learned_smoother == "mg"
-    net = Net(plan, seed=5, multigrid=mg)
-    t0 = time.perf_counter()
-    tr = None
-    for k in range(a.train_loads):  # one load case per call: progress stays visible
-        tr = net.train(pairs=1, snapshots=8, epochs=a.train_epochs, seed0=1000 + k,
-                       lr=1e-3, lr_final=1e-5, max_grad_norm=1.0, source=_lib.TRAIN_SOLVER)
-        print(f"training: load case {k + 1}/{a.train_loads} done, loss {tr['loss_first']:.3f} -> "
-              f"{tr['loss_last']:.3f}, {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
-    train_s = time.perf_counter() - t0
     F = plan.pressure_load(w["q0"], w["patches"])
     plan.set_rhs(F)
-    bj = plan.options(tolerance=1e-8, max_iterations=1000000)
-    lo = plan.options(tolerance=1e-8, max_iterations=a.learned_max_iterations,
-                      precond=_lib.PRECOND_LEARNED)
-    net.attach()
-    res = {}
-    trained = net.params()
-    for tag, o in (("block_jacobi", bj), ("learned", lo), ("zero_network", lo)):
-        # zero_network: the same preconditioner with every parameter zero
-        net.set_params(np.zeros_like(trained) if tag == "zero_network" else trained)
-        for _ in range(a.warmup):
-            plan.solve_resident(o)
-        t1, dev = time.perf_counter(), []
-        for _ in range(a.steps):
-            st = plan.solve_resident(o)
-            dev.append(st.device_ms)
-        res[tag] = {"ms": (time.perf_counter() - t1) * 1e3 / a.steps, "device_ms": float(np.mean(dev)),
-                    "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
-                    "converged": bool(st.converged), "breakdown": bool(st.breakdown),
-                    # FP64 residual-replacement restarts: iterations and true residual of each
-                    "restarts": [[st.sweep_iterations[i], st.sweep_residual[i]]
-                                 for i in range(min(st.sweeps, _lib.MAX_SWEEPS))]}
-        print(f"{tag}: {st.iterations} iterations, {res[tag]['ms']:.0f} ms", file=sys.stderr,
-              flush=True)
-        res[tag]["x"] = plan.solution()[0]
-    # plain PCG with one multigrid V-cycle per iteration (PP_PRECOND_MULTIGRID, no
-    # network): a reported alternative, not the value; its hierarchy is built
-    # in the warm-up solve
-    om = plan.options(tolerance=1e-8, max_iterations=a.learned_max_iterations,
-                      precond=_lib.PRECOND_MULTIGRID)
-    for _ in range(max(1, a.warmup)):
-        plan.solve_resident(om)
-    t1, dev = time.perf_counter(), []
-    for _ in range(a.steps):
-        st = plan.solve_resident(om)
-        dev.append(st.device_ms)
-    res["multigrid"] = {"ms": (time.perf_counter() - t1) * 1e3 / a.steps,
-                        "device_ms": float(np.mean(dev)), "iterations": st.iterations,
-                        "true_rel_residual": st.true_rel_residual,
-                        "converged": bool(st.converged)}
-    res["multigrid"]["max_nodal_w_error_vs_bj"] = O.max_nodal_w_error(plan.solution()[0],
-                                                                       res["block_jacobi"]["x"])
-    print(f"multigrid: {st.iterations} iterations, {res['multigrid']['ms']:.0f} ms", file=sys.stderr,
-          flush=True)
-    err = O.max_nodal_w_error(res["learned"].pop("x"), res["block_jacobi"]["x"])
-    err0 = O.max_nodal_w_error(res["zero_network"].pop("x"), res["block_jacobi"].pop("x"))
+    res = {"workload": "c5: 1024 x 1024 Q4 plate, t_e = 0.1 exp(0.5 g), g unit Gaussian field "
+                       "with correlation length 0.5 m (seed 5), clamped, 10 kPa + 3 Gaussian "
+                       "patches", "dof": plan.n_dof}
+    bj = timed_leg(plan, plan.options(tolerance=1e-8, max_iterations=1000000), 1, a.c5_steps)
+    x_bj = plan.solution()[0]
+    res["block_jacobi"] = bj
+    print(f"block_jacobi 3.15M DOF: {bj['iterations']} iterations, {bj['ms']:.0f} ms",
+          file=sys.stderr, flush=True)
+    mg = timed_leg(plan, plan.options(tolerance=1e-8, max_iterations=40000,
+                                      precond=_lib.PRECOND_MULTIGRID), 1, 5)
+    mg["max_nodal_w_error_vs_block_jacobi"] = O.max_nodal_w_error(plan.solution()[0], x_bj)
+    res["multigrid"] = mg
+    print(f"multigrid 3.15M DOF: {mg['iterations']} iterations, {mg['ms']:.0f} ms",
+          file=sys.stderr, flush=True)
+    res.update(learned_legs(plan, F, a.train_loads, a.train_epochs, 1, 3, x_bj))
+    if res["learned"]["converged"]:
+        res["speedup_learned_vs_block_jacobi"] = bj["ms"] / res["learned"]["ms"]
+    res["speedup_multigrid_vs_block_jacobi"] = bj["ms"] / mg["ms"]
     n_dof = plan.n_dof
-    it_bj = res["block_jacobi"]["iterations"]
-    achieved = BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj / (res["block_jacobi"]["device_ms"] * 1e-3) / 1e9
-    out = {
-        "metric": "ms/solve to 1e-8 rel. residual (3.15M DOF, variable thickness): learned "
-                  "preconditioner in flexible PCG vs block-Jacobi PCG",
-        "value": res["learned"]["ms"], "unit": "ms/solve", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": res["learned"]["ms"], "higher_is_better": False,
-        "scaling": "strong", "vs_baseline": None, "dtype": "f64+bf16", "data": "synthetic",
-        "config": {"workload": "c5: 1024 x 1024 Q4 plate, t_e = 0.1 exp(0.5 g), g unit Gaussian "
-                   "field with correlation length 0.5 m (seed 5), clamped, 10 kPa + 3 Gaussian "
-                   "patches", "dof": n_dof, "precond": "learned U-net (16/32/64, BF16 MFMA) + " +
-                   ("Galerkin multigrid V-cycle smoothing (7 levels, FP32 coarse stencils)" if mg
-                    else "block-Jacobi smoothing") + (", flexible PCG (FCG(4))" if mg
-                                                      else ", flexible PCG (FCG(8))"),
-                   "precision": "fp64 outer, bf16 network"},
-        "learned": res["learned"], "block_jacobi": res["block_jacobi"],
-        "zero_network": res["zero_network"], "multigrid": res["multigrid"],
-        "network_iteration_gain": (res["zero_network"]["iterations"] / res["learned"]["iterations"]
-                                   if res["learned"]["converged"] and res["zero_network"]["converged"]
-                                   else None),
-        # only meaningful when the learned solve reached the tolerance
-        "speedup_vs_block_jacobi": (res["block_jacobi"]["ms"] / res["learned"]["ms"]
-                                    if res["learned"]["converged"] else None),
-        "iteration_ratio": (it_bj / max(1, res["learned"]["iterations"])
-                            if res["learned"]["converged"] else None),
-        "max_nodal_w_error_learned_vs_bj": err,
-        "max_nodal_w_error_zero_network_vs_bj": err0,
-        "training": {"pairs": a.train_loads * 8, "epochs": a.train_epochs, "seconds": train_s,
-                     "loss_first": tr["loss_first"], "loss_last": tr["loss_last"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pcg (block-Jacobi FP64 PCG, tiles strided; the comparison leg)",
-                     "algorithmic_bytes_per_launch": BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj},
-    }
+    it_bj = bj["iterations"]
+    byt = BYTES_PER_DOF_ITER_FP64 * n_dof * it_bj
+    ach = byt / (bj["device_ms"] * 1e-3) / 1e9
+    res["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "k_pcg (block-Jacobi FP64 PCG, tiles strided over the CUs)",
+                       "algorithmic_bytes_per_launch": byt}
     if a.cpu_sample_iters != 0:
         try:
             per, st_js = js_ms_per_dof_iteration()
-            out["cpu_baseline"] = {
+            res["cpu_baseline"] = {
                 "value": per * n_dof * it_bj, "unit": "ms/solve", "cores": 1, "kind": "port",
+                "host_cpus": st_js.get("cpus"),
                 "sample": (f"float32 solver.ts restatement in Node {st_js['node']} on "
                            f"{st_js['cpuModel']}: 300 PCG iterations of a 64 x 64 Q4 plate timed, "
                            f"{per * 1e6:.3f} ns per DOF-iteration, extrapolated x {n_dof} DOF x "
                            f"{it_bj} block-Jacobi iterations (the reference takes one thickness "
-                           f"per plate, so it cannot run this plate itself; extrapolation)")}
+                           f"per plate and cannot run this plate itself; extrapolation)")}
         except (OSError, subprocess.CalledProcessError) as e:
-            out["cpu_baseline"] = {"error": str(e)}
-    print(json.dumps(out))
-    net.close()
+            res["cpu_baseline"] = {"error": str(e)}
     plan.close()
     ctx.close()
+    return res
 
 
 def main():
@@ -318,12 +435,27 @@ def main():
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return
+    if a.probe == "copy":
+        calibrate_copy(256)
+        return
+    if a.probe == "pcg":
+        probe_pcg()
+        return
     if a.workload == "c4":
         bench_batch(a)
         return
     if a.workload == "c5":
         bench_learned(a)
         return
+    default_run = a.workload == "c2"
+    traffic, traffic_info = None, None
+    if default_run and not a.no_traffic:
+        # before this process initialises the GPU: the counter runs are children
+        try:
+            traffic, traffic_info = measure_traffic()
+        except (OSError, subprocess.SubprocessError, ValueError, KeyError) as e:
+            traffic, traffic_info = None, {"error": str(e)}
+        print(f"traffic: {traffic} ({traffic_info})", file=sys.stderr, flush=True)
     import numpy as np
 
     from platepcg import Context, Plan, _lib
@@ -331,7 +463,7 @@ def main():
 
     w = P.workload(a.workload)
     mixed = a.workload == "c3"
-    if a.cpu_sample_iters < 0:
+    if a.cpu_sample_iters < 0 and not default_run:
         a.cpu_sample_iters = 150 if mixed else 1200
     ctx = Context(0)
     plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
@@ -371,18 +503,11 @@ def main():
     bpd = BYTES_PER_DOF_ITER_MIXED if mixed else BYTES_PER_DOF_ITER_FP64
     bytes_per_launch = bpd * n_dof * it
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
-    traffic = None
-    tr_path = os.path.join(ROOT, "profiles", "pcg_fp64_142x142_traffic.md")
-    if a.workload == "c2" and os.path.exists(tr_path):
-        with open(tr_path) as f:
-            for line in f:
-                if line.startswith("hbm_bytes_per_launch:"):
-                    traffic = int(line.split(":")[1])
     if mixed:
         metric = "ms/solve to 1e-8 rel. residual (1.47M DOF, FP32-inner / FP64-refined PCG)"
         kernel = "k_pcg32 (persistent FP32 PCG, MFMA element apply) + FP64 refinement"
     else:
-        metric = "ms/solve to 1e-8 rel. residual (61k DOF, FP64 block-Jacobi PCG)"
+        metric = METRIC if default_run else "ms/solve to 1e-8 rel. residual (FP64 block-Jacobi PCG)"
         kernel = "k_pcg_cg (persistent single-reduction FP64 PCG)"
     out = {
         "metric": metric,
@@ -404,6 +529,8 @@ def main():
                      "kernel": kernel,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    if traffic_info is not None:
+        out["roofline"]["traffic_measurement"] = traffic_info
     if mixed:
         out["sweeps"] = chk.sweeps
         n = min(chk.sweeps, _lib.MAX_SWEEPS)
@@ -428,33 +555,38 @@ def main():
     # in the warm-up): reported alternatives, not the value above
     alts = (() if mixed else (("chebyshev8", _lib.PRECOND_CHEBYSHEV),)) + \
         (("multigrid", _lib.PRECOND_MULTIGRID),)
+    legs = {"block_jacobi": {"ms": ms_per_step, "device_ms": kern_ms, "iterations": it,
+                             "true_rel_residual": chk.true_rel_residual,
+                             "converged": bool(chk.converged)}}
     for key, pc in alts:
         try:
             oc = plan.options(tolerance=1e-8, max_iterations=400000, precond=pc,
                               poly_degree=8 if key == "chebyshev8" else 0)
-            for _ in range(max(1, a.warmup)):
-                plan.solve_resident(oc)
-            t3, dc = time.perf_counter(), []
-            for _ in range(a.steps):
-                sc = plan.solve_resident(oc)
-                dc.append(sc.device_ms)
-            out[key] = {"ms": (time.perf_counter() - t3) * 1e3 / a.steps,
-                        "device_ms": float(np.mean(dc)), "iterations": sc.iterations,
-                        "true_rel_residual": sc.true_rel_residual,
-                        "converged": bool(sc.converged)}
-            if key == "multigrid":
-                out[key]["max_nodal_w_error_vs_block_jacobi"] = O.max_nodal_w_error(
-                    plan.solution()[0], x_bj)
+            legs[key] = timed_leg(plan, oc, max(1, a.warmup), a.steps)
+            legs[key]["max_nodal_w_error_vs_block_jacobi"] = O.max_nodal_w_error(
+                plan.solution()[0], x_bj)
+            legs[key]["speedup_vs_block_jacobi"] = ms_per_step / legs[key]["ms"]
         except Exception as e:  # plates too large for the resident kernel / coarsest level
-            out[key] = {"error": str(e)}
-    if a.cpu_sample_iters > 0:
+            legs[key] = {"error": str(e)}
+    if default_run:
+        legs.update(learned_legs(plan, F, a.c2_train_loads, 2, max(1, a.warmup), a.steps, x_bj))
+        if legs["learned"]["converged"]:
+            legs["speedup_learned_vs_block_jacobi"] = ms_per_step / legs["learned"]["ms"]
+        out["c2"] = legs
+    else:
+        out.update({k: v for k, v in legs.items() if k != "block_jacobi"})
+    if a.cpu_sample_iters != 0:
+        if default_run and a.cpu_sample_iters < 0:
+            a.cpu_sample_iters = full  # the whole solve, nothing extrapolated
         try:
             out["cpu_baseline"] = cpu_baseline(w, F, full, a.cpu_sample_iters, basis)
         except (OSError, subprocess.CalledProcessError) as e:
             out["cpu_baseline"] = {"error": str(e)}
-    print(json.dumps(out))
     plan.close()
     ctx.close()
+    if default_run and not a.no_c5:
+        out["c5"] = c5_legs(a)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
