@@ -166,3 +166,75 @@ def max_nodal_w_error(x_test, x_ref, floor=0.01):
     wt, wr = np.asarray(x_test, np.float64)[0::3], np.asarray(x_ref, np.float64)[0::3]
     sel = np.abs(wr) > floor * np.abs(wr).max()
     return float(np.max(np.abs(wt[sel] - wr[sel]) / np.abs(wr[sel])))
+
+
+def unstructured_triangles(n, seed, width=10.0, jitter=0.35, balance=False):
+    """A genuinely unstructured triangle mesh of the width x width square for
+    the coloured path: n + 1 points per edge, interior points on an n x n grid
+    each moved by up to `jitter` of the spacing in both directions, Delaunay-
+    triangulated (scipy), every triangle counter-clockwise.  boundaryNodes are
+    the points on the edges (their order does not matter: solver.ts:92-134
+    puts them in a Set).
+
+    The triangle of element.ts:642-943 as written gives its THIRD vertex a
+    rank-2 diagonal 3x3 block, so a node that is the third vertex of every
+    triangle around it has a singular nodal block and the assembled stiffness
+    a zero-energy mode there.  balance=True rotates vertex orders (cyclically,
+    so every triangle stays counter-clockwise) until each node is the first or
+    second vertex of at least one triangle; the mesh is otherwise the same."""
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(seed)
+    h = width / n
+    s = np.arange(n + 1) * h
+    edge = [(x, 0.0) for x in s] + [(width, y) for y in s[1:]] + \
+        [(x, width) for x in s[-2::-1]] + [(0.0, y) for y in s[-2:0:-1]]
+    gx, gy = np.meshgrid(s[1:-1], s[1:-1])
+    inner = np.stack([gx.ravel(), gy.ravel()], 1)
+    inner = inner + rng.uniform(-jitter * h, jitter * h, inner.shape)
+    pts = np.vstack([np.array(edge), inner]).astype(np.float32)
+    tri = Delaunay(pts.astype(np.float64)).simplices.astype(np.int64)
+    p = pts.astype(np.float64)[tri]
+    cross = (p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) - \
+        (p[:, 1, 1] - p[:, 0, 1]) * (p[:, 2, 0] - p[:, 0, 0])
+    tri[cross < 0] = tri[cross < 0][:, [0, 2, 1]]
+    if balance:
+        tri = _balance_third_vertices(tri, len(pts))
+    return {"nodes": pts.ravel(), "elements": tri.astype(np.uint32).ravel(),
+            "boundaryNodes": np.arange(len(edge), dtype=np.uint32), "nodeCount": len(pts),
+            "elementCount": len(tri), "nodesPerElement": 3, "gridNx": 0, "gridNy": 0}
+
+
+def _balance_third_vertices(tri, n_nodes):
+    """Cyclic vertex rotations so that every node is the first or second
+    vertex of some triangle (greedy, deterministic)."""
+    tri = tri.copy()
+    inc = [[] for _ in range(n_nodes)]
+    for e, t in enumerate(tri):
+        for v in t:
+            inc[v].append(e)
+
+    def covered(v):
+        return any(tri[e][2] != v for e in inc[v])
+
+    for _ in range(100):
+        bare = [v for v in range(n_nodes) if not covered(v)]
+        if not bare:
+            return tri
+        for v in bare:
+            if covered(v):
+                continue
+            # rotate v to the front of a triangle around it, preferring one
+            # whose vertex that becomes third is still covered elsewhere
+            for e in inc[v]:
+                k = list(tri[e]).index(v)
+                new = np.roll(tri[e], -k)
+                old = tri[e].copy()
+                tri[e] = new
+                if covered(new[2]):
+                    break
+                tri[e] = old
+            else:
+                e = inc[v][0]
+                tri[e] = np.roll(tri[e], -list(tri[e]).index(v))
+    raise RuntimeError("could not balance third vertices")

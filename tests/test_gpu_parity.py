@@ -253,3 +253,70 @@ def test_malformed_colour_offsets_are_rejected():
         assert e.value.code == _lib.PP_ERR_INVALID
     Plan(ctx, tri, 210e9, 0.3, 0.1, mask, good).close()
     ctx.close()
+
+
+@pytest.mark.parametrize("balance", [False, True])
+def test_unstructured_triangulation_coloured_path(balance):
+    """A genuinely unstructured mesh on the coloured path: a jittered Delaunay
+    triangulation (irregular node valence, greedy colouring of
+    coloring.ts:92-177 with 8+ colours, node -> element lists of every
+    length).  K.x and each colour's apply equal the oracle's on the
+    triangulator's own vertex order, where the triangle as written leaves
+    some nodal blocks singular; there the block inverses agree wherever the
+    nodal block is well conditioned.  With vertex orders rotated so that no
+    node is only ever a third vertex, the block inverses, pressure load,
+    moments, first 12 PCG iterates and the converged solve all equal the
+    oracle's."""
+    ctx = Context(0)
+    mesh = O.unstructured_triangles(20, seed=11, jitter=0.3, balance=balance)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    col = P.colour_greedy(mesh)
+    plan = Plan(ctx, mesh, E, NU, 0.1, mask, col)
+    assert not plan.structured and col["colour_count"] >= 8
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask, order=col["elements_by_colour"])
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(plan.n_dof)
+    assert rel(plan.apply_k(x), op.apply_k(x)) < 1e-12
+    Ke = op.Ke.reshape(-1, 9, 9)
+    el = mesh["elements"].reshape(-1, 3).astype(np.int64)
+    off = col["colour_offsets"]
+    for c in range(col["colour_count"]):
+        y = np.zeros(plan.n_dof)
+        for e in col["elements_by_colour"][off[c]:off[c + 1]]:
+            dofs = (el[e][:, None] * 3 + np.arange(3)).ravel()
+            ye = Ke[e] @ x[dofs]
+            keep = op.cdof[dofs] == 0
+            y[dofs[keep]] += ye[keep]
+        assert rel(plan.apply_k_colour(c, x), y) < 1e-12
+    # nodal blocks before inversion, to tell the well-conditioned ones apart
+    raw = np.zeros((mesh["nodeCount"], 3, 3))
+    for i in range(3):
+        np.add.at(raw, el[:, i], Ke[:, 3 * i:3 * i + 3, 3 * i:3 * i + 3])
+    lam = np.linalg.eigvalsh(raw)
+    good = (lam[:, 0] > 1e-9 * lam[:, 2]) | (mask != 0)
+    assert good.all() == balance
+    b = plan.block_inverse().reshape(-1, 9)[good]
+    ob = op.blk.reshape(-1, 9)[good]
+    assert np.max(np.abs(b - ob) / np.abs(ob).max(axis=1, keepdims=True)) < 1e-10
+    if not balance:
+        plan.close()
+        ctx.close()
+        return
+    F = plan.pressure_load(1e4, np.array([[3e4, 4.0, 6.0, 1.5]]))
+    Fo = op.pressure_load(1e4, np.array([[3e4, 4.0, 6.0, 1.5]]))
+    np.testing.assert_allclose(F, Fo.astype(np.float32), rtol=2e-7, atol=0)
+    for k in (1, 4, 12):
+        xk, _, st = plan.solve(F, max_iterations=k)
+        xo, so = op.pcg(F.astype(np.float64), max_iter=k)
+        assert st["iterations"] == so["iterations"] == k
+        assert rel(xk, xo) < 1e-9, k
+    x64, _, st = plan.solve(F, tolerance=1e-8, max_iterations=5000)
+    xo, so = op.pcg(F.astype(np.float64), tol=1e-8)
+    assert st["converged"] and so["converged"]
+    assert abs(st["iterations"] - so["iterations"]) <= 2
+    assert op.true_rel_residual(x64, F.astype(np.float64)) <= 1e-8
+    assert O.max_nodal_w_error(x64, xo) < 1e-6
+    for got, want in zip(plan.moments(x64), op.moments(x64)):
+        np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-7 * np.abs(want).max())
+    plan.close()
+    ctx.close()

@@ -349,3 +349,60 @@ def test_js_float32_solver_matches_fp64_oracle(tri):
     if not tri:
         # the survey's independent FP64 probe of the reference math: 334 iterations
         assert st["iterations"] == 334
+
+
+def test_unstructured_triangulation_third_vertex_singularity():
+    """On a jittered Delaunay mesh (the reference's unstructured path meshes
+    with constrained Delaunay triangulation, mesher-unstructured.ts:198-243,
+    keeping the triangulator's vertex order) the triangle of
+    element.ts:642-943 as written gives its third vertex a rank-2 diagonal
+    3x3 block.  A node that is the third vertex of every triangle around it
+    then has a singular nodal block, and the free-DOF stiffness a zero-energy
+    mode: PCG does not converge.  On the split-quad grid every node is a first
+    or second vertex somewhere and the system is nonsingular; rotating vertex
+    orders (cyclically, still counter-clockwise) to the same effect makes the
+    Delaunay mesh nonsingular too, and PCG converges."""
+    mesh = O.unstructured_triangles(8, seed=3, jitter=0.2)
+    el = mesh["elements"].reshape(-1, 3).astype(np.int64)
+    pts = mesh["nodes"].reshape(-1, 2).astype(np.float64)
+    p = pts[el]
+    cross = (p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) - \
+        (p[:, 1, 1] - p[:, 0, 1]) * (p[:, 2, 0] - p[:, 0, 0])
+    assert np.all(cross > 0)
+    assert abs(cross.sum() / 2 - 100.0) < 1e-3          # the triangles tile the square
+    col = P.colour_greedy(mesh)
+    assert colour_valid(mesh, col) and col["colour_count"] >= 6
+    for e in el:
+        ke = O.ke_dkt(pts[e].ravel(), E, NU, 0.1)
+        lam = np.linalg.eigvalsh(ke)
+        assert lam.min() > -1e-12 * lam.max()
+        d = [np.linalg.eigvalsh(ke[3 * i:3 * i + 3, 3 * i:3 * i + 3]) for i in range(3)]
+        assert d[0][0] > 1e-4 * d[0][-1] and d[1][0] > 1e-5 * d[1][-1]
+        assert abs(d[2][0]) < 1e-12 * d[2][-1]
+    third_only = [v for v in range(mesh["nodeCount"]) if not np.any(el[:, :2] == v)]
+    interior = set(third_only) - set(mesh["boundaryNodes"].tolist())
+    assert interior
+
+    def free_spectrum(m):
+        msk = P.constrained_mask(m, [{"type": "fixed", "location": "all_edges"}])
+        op = O.OracleProblem(m, E, NU, 0.1, msk)
+        K = assembled(op).toarray()
+        free = np.nonzero(op.cdof == 0)[0]
+        lam = np.linalg.eigvalsh(K[np.ix_(free, free)])
+        return lam / lam[-1], op
+
+    lam, op = free_spectrum(mesh)
+    assert np.sum(np.abs(lam) < 1e-12) >= len(interior)
+    x, st = op.pcg(op.pressure_load(1e4), tol=1e-8, max_iter=2000)
+    assert not st["converged"]
+    bal = O.unstructured_triangles(8, seed=3, jitter=0.2, balance=True)
+    eb = bal["elements"].reshape(-1, 3).astype(np.int64)
+    assert np.array_equal(np.sort(eb, 1), np.sort(el, 1))   # same triangles
+    assert all(np.any(eb[:, :2] == v) for v in range(bal["nodeCount"]))
+    lam_b, opb = free_spectrum(bal)
+    assert lam_b[0] > 1e-3
+    x, st = opb.pcg(opb.pressure_load(1e4), tol=1e-8, max_iter=2000)
+    assert st["converged"] and st["iterations"] < 40
+    sq = P.split_quads(P.rect_mesh(8))
+    lam_sq, _ = free_spectrum(sq)
+    assert lam_sq[0] > 1e-3
