@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "pp_grid.h"
+#include "pp_load.h"
 
 namespace pp {
 
@@ -53,6 +54,8 @@ struct PlateDesc {
   double D11, D12, D33, Ds;  // Q4 material (bending D, nu D, (1 - nu) D / 2, shear)
   const double* ax;     // Q4: per column a, 1/a
   const double* by;     // Q4: per row b, 1/b
+  const double* xc;     // column x coordinates (nx + 1), for the loads
+  const double* yc;     // row y coordinates (ny + 1)
   const uint8_t* cmask;
   const uint16_t* ccls;  // column class per column
   const uint16_t* rcls;  // row class per row
@@ -496,6 +499,49 @@ __global__ void k_batch_block_inverse(const PlateDesc* descs, int plate, double*
   block_inverse_sym(A, D.cmask[n], minv + 6 * n);
 }
 
+// Consistent pressure loads of plates [p0, p1), one thread per node (nodes
+// node0 .. node0 + nodes - 1 of the pool): each plate's own q0 and Gaussian
+// patches (q0[p - p0], patches poff[p - p0] .. poff[p - p0 + 1]), Q4 cells or
+// split-quad triangles (pp_load.h).  The w load is rounded to float32 (the
+// reference builds a Float32Array) and written to the pool's FP64 right-hand
+// side Fb and/or to the float vector Ff (indexed from node0); rotation loads
+// and constrained w are zero (solver.ts:400-407).
+__global__ __launch_bounds__(256) void k_batch_pressure(const PlateDesc* descs,
+                                                        const int64_t* dof_off, int p0, int p1,
+                                                        int64_t node0, int64_t nodes,
+                                                        const double* q0, const double* pt,
+                                                        const int32_t* poff, double* Fb,
+                                                        float* Ff) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= nodes) return;
+  const int64_t ng = node0 + g;
+  int lo = p0, hi = p1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (dof_off[mid] / 3 <= ng) lo = mid;
+    else hi = mid - 1;
+  }
+  const PlateDesc& D = descs[lo];
+  const int64_t n = ng - dof_off[lo] / 3;
+  const int i = (int)(n % (D.nx + 1)), j = (int)(n / (D.nx + 1));
+  const int k = lo - p0;
+  const double* pp = pt + 4 * (int64_t)poff[k];
+  const int np = poff[k + 1] - poff[k];
+  const double f = D.dkt ? grid_node_load_split(D.xc, D.yc, D.nx, D.ny, i, j, q0[k], pp, np)
+                         : grid_node_load_q4(D.xc, D.yc, D.nx, D.ny, i, j, q0[k], pp, np);
+  const float fw = (D.cmask[n] & PP_FIX_W) ? 0.0f : (float)f;
+  if (Fb) {
+    Fb[3 * ng] = (double)fw;
+    Fb[3 * ng + 1] = 0.0;
+    Fb[3 * ng + 2] = 0.0;
+  }
+  if (Ff) {
+    Ff[3 * g] = fw;
+    Ff[3 * g + 1] = 0.0f;
+    Ff[3 * g + 2] = 0.0f;
+  }
+}
+
 __global__ void k_batch_zero(GridSync* gs, DevStats* st, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -514,15 +560,13 @@ using namespace pp;
 
 struct BatchPlate {
   int nx = 0, ny = 0, npe = 4;
-  std::vector<float> nodes;
-  std::vector<uint32_t> elems;
   double E = 0, nu = 0, t = 0;
   int ncc = 0, ncr = 0;
   std::vector<uint16_t> ccls, rcls;
   std::vector<uint8_t> cmask;
   std::vector<double> ax, by;  // per column (a, 1/a), per row (b, 1/b)
   std::vector<double> kcell;
-  int64_t dof_off = 0, node_off = 0, cls_off = 0, k_off = 0, geo_off = 0;
+  int64_t dof_off = 0, node_off = 0, cls_off = 0, k_off = 0, geo_off = 0, coord_off = 0;
   int shape = 0, nwg = 1, last_iterations = 0, last_converged = 0;
 };
 
@@ -530,7 +574,7 @@ struct pp_batch {
   pp_ctx* ctx = nullptr;
   std::vector<BatchPlate> P;
   int64_t total_dof = 0, total_nodes = 0;
-  DevBuf<double> F, x, r, z, p0, p1, Ap, tmp, minv, kcell, geo, part, norms;
+  DevBuf<double> F, x, r, z, p0, p1, Ap, tmp, minv, kcell, geo, coord, part, norms;
   DevBuf<uint8_t> cmask;
   DevBuf<uint16_t> cls;
   DevBuf<PlateDesc> descs;
@@ -845,7 +889,7 @@ int pp_batch_create(pp_ctx* ctx, int32_t n, const pp_mesh* meshes, const pp_mate
   b->P.resize(n);
   std::vector<uint8_t> hmask;
   std::vector<uint16_t> hcls;
-  std::vector<double> hk, hgeo;
+  std::vector<double> hk, hgeo, hcoord;
   std::vector<int64_t> hoff(n + 1, 0);
   int rc = PP_OK;
   for (int p = 0; p < n && rc == PP_OK; p++) {
@@ -887,13 +931,14 @@ int pp_batch_create(pp_ctx* ctx, int32_t n, const pp_mesh* meshes, const pp_mate
     q.geo_off = (int64_t)hgeo.size();
     hgeo.insert(hgeo.end(), q.ax.begin(), q.ax.end());
     hgeo.insert(hgeo.end(), q.by.begin(), q.by.end());
+    q.coord_off = (int64_t)hcoord.size();
+    for (int i = 0; i <= q.nx; i++) hcoord.push_back((double)m->nodes[2 * i]);
+    for (int j = 0; j <= q.ny; j++) hcoord.push_back((double)m->nodes[2 * ((size_t)j * W) + 1]);
     q.kcell.resize((size_t)q.ncc * q.ncr * 144);
     for (int cr = 0; cr < q.ncr; cr++)
       for (int cc = 0; cc < q.ncc; cc++)
         cell_matrix(q.npe, va[cc], vb[cr], q.E, q.nu, q.t, &q.kcell[(size_t)(cr * q.ncc + cc) * 144]);
     q.cmask.assign(masks[p], masks[p] + m->node_count);
-    q.nodes.assign(m->nodes, m->nodes + 2 * (size_t)m->node_count);
-    q.elems.assign(m->elements, m->elements + (size_t)m->element_count * m->nodes_per_element);
     q.dof_off = b->total_dof;
     q.node_off = b->total_nodes;
     q.cls_off = (int64_t)hcls.size();
@@ -912,7 +957,7 @@ int pp_batch_create(pp_ctx* ctx, int32_t n, const pp_mesh* meshes, const pp_mate
            (rc = b->z.alloc(nd)) || (rc = b->p0.alloc(nd)) || (rc = b->p1.alloc(nd)) ||
            (rc = b->Ap.alloc(nd)) || (rc = b->tmp.alloc(nd)) ||
            (rc = b->minv.alloc(6 * b->total_nodes)) || (rc = b->kcell.alloc(hk.size())) ||
-           (rc = b->geo.alloc(hgeo.size())) ||
+           (rc = b->geo.alloc(hgeo.size())) || (rc = b->coord.alloc(hcoord.size())) ||
            (rc = b->cmask.alloc(hmask.size())) || (rc = b->cls.alloc(hcls.size())) ||
            (rc = b->descs.alloc(n)) || (rc = b->gs.alloc(n)) || (rc = b->st.alloc(n)) ||
            (rc = b->norms.alloc(2 * n)) || (rc = b->dof_off.alloc(n + 1)));
@@ -929,6 +974,7 @@ int pp_batch_create(pp_ctx* ctx, int32_t n, const pp_mesh* meshes, const pp_mate
       up(b->cmask.p, hmask.data(), hmask.size()) != hipSuccess ||
       up(b->cls.p, hcls.data(), hcls.size() * 2) != hipSuccess ||
       up(b->geo.p, hgeo.data(), hgeo.size() * 8) != hipSuccess ||
+      up(b->coord.p, hcoord.data(), hcoord.size() * 8) != hipSuccess ||
       up(b->dof_off.p, hoff.data(), hoff.size() * 8) != hipSuccess ||
       hipMemsetAsync(b->F.p, 0, nd * 8, s) != hipSuccess) {
     pp_batch_destroy(b);
@@ -951,6 +997,8 @@ int pp_batch_create(pp_ctx* ctx, int32_t n, const pp_mesh* meshes, const pp_mate
     D.Ds = (5.0 / 6.0) * q.E / (2.0 * (1.0 + q.nu)) * q.t;
     D.ax = b->geo.p + q.geo_off;
     D.by = b->geo.p + q.geo_off + 2 * q.nx;
+    D.xc = b->coord.p + q.coord_off;
+    D.yc = b->coord.p + q.coord_off + q.nx + 1;
     D.cmask = b->cmask.p + q.node_off;
     D.ccls = b->cls.p + q.cls_off;
     D.rcls = b->cls.p + q.cls_off + q.nx;
@@ -1029,10 +1077,61 @@ int pp_batch_pressure_load(pp_batch* b, int32_t plate, double q0, const double* 
       (n_patches > 0 && !patches))
     return fail(PP_ERR_INVALID, "bad argument");
   PP_LOCK(b->ctx);
+  PP_HIP(hipSetDevice(b->ctx->device));
   const BatchPlate& q = b->P[plate];
   const int64_t nn = (int64_t)(q.nx + 1) * (q.ny + 1);
-  return mesh_pressure(q.nodes.data(), q.elems.data(), q.npe, nn, (int64_t)q.elems.size() / q.npe,
-                       q.cmask.data(), q0, patches, n_patches, F);
+  std::vector<double> hv(1 + 4 * (size_t)n_patches);
+  hv[0] = q0;
+  if (n_patches) memcpy(hv.data() + 1, patches, 4 * (size_t)n_patches * sizeof(double));
+  const int32_t hpo[2] = {0, n_patches};
+  DevBuf<double> v;
+  DevBuf<int32_t> po;
+  DevBuf<float> f;
+  int rc;
+  if ((rc = v.alloc(hv.size())) || (rc = po.alloc(2)) || (rc = f.alloc(3 * nn))) return rc;
+  hipStream_t s = b->ctx->stream;
+  PP_HIP(hipMemcpyAsync(v.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(po.p, hpo, sizeof(hpo), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_batch_pressure, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                     (const PlateDesc*)b->descs.p, (const int64_t*)b->dof_off.p, plate, plate + 1,
+                     q.node_off, nn, (const double*)v.p, (const double*)v.p + 1,
+                     (const int32_t*)po.p, (double*)nullptr, f.p);
+  PP_HIP(hipGetLastError());
+  PP_HIP(hipMemcpyAsync(F, f.p, 3 * nn * sizeof(float), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+int pp_batch_pressure_loads(pp_batch* b, const double* q0, const double* patches,
+                            const int32_t* patch_offsets) {
+  if (!b || !q0 || !patch_offsets) return fail(PP_ERR_INVALID, "NULL argument");
+  const int n = (int)b->P.size();
+  if (patch_offsets[0] != 0) return fail(PP_ERR_INVALID, "patch_offsets[0] must be 0");
+  for (int p = 0; p < n; p++)
+    if (patch_offsets[p + 1] < patch_offsets[p])
+      return fail(PP_ERR_INVALID, "patch_offsets must be nondecreasing");
+  const int32_t total = patch_offsets[n];
+  if (total > 0 && !patches) return fail(PP_ERR_INVALID, "patches is NULL");
+  PP_LOCK(b->ctx);
+  PP_HIP(hipSetDevice(b->ctx->device));
+  std::vector<double> hv((size_t)n + 4 * (size_t)total);
+  memcpy(hv.data(), q0, (size_t)n * sizeof(double));
+  if (total) memcpy(hv.data() + n, patches, 4 * (size_t)total * sizeof(double));
+  DevBuf<double> v;
+  DevBuf<int32_t> po;
+  int rc;
+  if ((rc = v.alloc(hv.size())) || (rc = po.alloc((size_t)n + 1))) return rc;
+  hipStream_t s = b->ctx->stream;
+  PP_HIP(hipMemcpyAsync(v.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(po.p, patch_offsets, ((size_t)n + 1) * sizeof(int32_t),
+                        hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_batch_pressure, dim3((unsigned)((b->total_nodes + 255) / 256)), dim3(256),
+                     0, s, (const PlateDesc*)b->descs.p, (const int64_t*)b->dof_off.p, 0, n,
+                     (int64_t)0, b->total_nodes, (const double*)v.p, (const double*)v.p + n,
+                     (const int32_t*)po.p, b->F.p, (float*)nullptr);
+  PP_HIP(hipGetLastError());
+  PP_HIP(hipStreamSynchronize(s));
+  return PP_OK;
 }
 
 int pp_batch_set_schedule(pp_batch* b, const pp_batch_schedule* s) {

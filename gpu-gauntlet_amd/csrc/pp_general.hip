@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_block_csr(int64_t nn, int npe, const in
 // Consistent nodal pressure load, one thread per node: the node's share of
 // every adjacent element's quadrature (Q4 2x2 Gauss with |J|, triangles the
 // 3-point rule with linear N and area/3), summed in element order and then
-// quadrature order (the order of mesh_pressure's element loop).  Rotation
+// quadrature order (the order of the oracle's element loop).  Rotation
 // loads and constrained w are zero.
 __device__ double pressure_value(double x, double y, double q0, const double* pt, int np) {
   double p = q0;
@@ -560,65 +560,6 @@ int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) 
   st->device_ms = ms;
   st->sweeps = 1;
   st->sweep_iterations[0] = st->iterations;
-  return PP_OK;
-}
-
-// Consistent pressure load on a general mesh, on the host (set-up work).
-// Consistent nodal pressure load of a mesh (host, FP64 sums, float out).
-int mesh_pressure(const float* h_nodes, const uint32_t* h_elems, int npe, int64_t n_nodes,
-                  int64_t n_elems, const uint8_t* h_cmask, double q0, const double* pt, int np,
-                  float* F) {
-  std::vector<double> f((size_t)(3 * n_nodes), 0.0);
-  auto pres = [&](double x, double y) {
-    double p = q0;
-    for (int k = 0; k < np; k++) {
-      const double dx = x - pt[4 * k + 1], dy = y - pt[4 * k + 2], s = pt[4 * k + 3];
-      p += pt[4 * k] * exp(-(dx * dx + dy * dy) / (2 * s * s));
-    }
-    return p;
-  };
-  for (int64_t e = 0; e < n_elems; e++) {
-    double xy[8];
-    uint32_t nodes[4];
-    for (int i = 0; i < npe; i++) {
-      nodes[i] = h_elems[(size_t)e * npe + i];
-      xy[2 * i] = h_nodes[2 * (size_t)nodes[i]];
-      xy[2 * i + 1] = h_nodes[2 * (size_t)nodes[i] + 1];
-    }
-    if (npe == 4) {
-      const double g = 0.5773502691896257;
-      const double gp[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
-      for (int q = 0; q < 4; q++) {
-        const double xi = gp[q][0], eta = gp[q][1];
-        const double N[4] = {(1 - xi) * (1 - eta) / 4, (1 + xi) * (1 - eta) / 4,
-                             (1 + xi) * (1 + eta) / 4, (1 - xi) * (1 + eta) / 4};
-        const double dxi[4] = {-(1 - eta) / 4, (1 - eta) / 4, (1 + eta) / 4, -(1 + eta) / 4};
-        const double deta[4] = {-(1 - xi) / 4, -(1 + xi) / 4, (1 + xi) / 4, (1 - xi) / 4};
-        double J11 = 0, J12 = 0, J21 = 0, J22 = 0, px = 0, py = 0;
-        for (int i = 0; i < 4; i++) {
-          J11 += dxi[i] * xy[2 * i]; J12 += dxi[i] * xy[2 * i + 1];
-          J21 += deta[i] * xy[2 * i]; J22 += deta[i] * xy[2 * i + 1];
-          px += N[i] * xy[2 * i]; py += N[i] * xy[2 * i + 1];
-        }
-        const double w = pres(px, py) * fabs(J11 * J22 - J12 * J21);
-        for (int i = 0; i < 4; i++) f[3 * (size_t)nodes[i]] += N[i] * w;
-      }
-    } else {
-      const double gL[3][3] = {{1.0 / 6, 1.0 / 6, 2.0 / 3}, {2.0 / 3, 1.0 / 6, 1.0 / 6},
-                               {1.0 / 6, 2.0 / 3, 1.0 / 6}};
-      const double twoA = (xy[0] - xy[2]) * (xy[5] - xy[1]) - (xy[4] - xy[0]) * (xy[1] - xy[3]);
-      const double area = fabs(twoA) / 2;
-      for (int q = 0; q < 3; q++) {
-        const double px = gL[q][0] * xy[0] + gL[q][1] * xy[2] + gL[q][2] * xy[4];
-        const double py = gL[q][0] * xy[1] + gL[q][1] * xy[3] + gL[q][2] * xy[5];
-        const double w = pres(px, py) * area / 3.0;
-        for (int i = 0; i < 3; i++) f[3 * (size_t)nodes[i]] += gL[q][i] * w;
-      }
-    }
-  }
-  for (int64_t n = 0; n < n_nodes; n++)
-    for (int d = 0; d < 3; d++)
-      F[3 * n + d] = (d == 0 && !(h_cmask[n] & PP_FIX_W)) ? (float)f[3 * n] : 0.0f;
   return PP_OK;
 }
 

@@ -188,9 +188,6 @@ int general_apply_precond(pp_plan* pl, const double* dr, double* dz);
 int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st);
 int general_true_residual(pp_plan* pl, const double* dx, double* rel);
 int general_pressure(pp_plan* pl, double q0, const double* d_patches, int np, float* dF);
-int mesh_pressure(const float* nodes, const uint32_t* elems, int npe, int64_t n_nodes,
-                  int64_t n_elems, const uint8_t* cmask, double q0, const double* patches, int np,
-                  float* F);
 // host FP64 element stiffness: Q4 12x12 (xy: 4 corners), DKT 9x9 (xy: 3 corners)
 void element_ke_q4(const double* xy, double E, double nu, double t, double* Ke);
 void element_ke_dkt(const double* xy, double E, double nu, double t, double* Ke);

@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "pp_grid.h"
+#include "pp_load.h"
 
 namespace pp {
 
@@ -168,42 +169,15 @@ __global__ void k_precond(int64_t nn, const double* minv, const double* r, doubl
 
 // ------------------------------------------------------- pressure loads
 
-__device__ double pressure_at(double x, double y, double q0, const double* pt, int np) {
-  double p = q0;
-  for (int k = 0; k < np; k++) {
-    const double dx = x - pt[4 * k + 1], dy = y - pt[4 * k + 2], s = pt[4 * k + 3];
-    p += pt[4 * k] * exp(-(dx * dx + dy * dy) / (2.0 * s * s));
-  }
-  return p;
-}
-
-// F_w(node) = sum over the adjacent elements of the 2x2 Gauss rule of
-// N_corner * p * |J| (the rule of element.ts:29-41); rotation loads are zero
-// and constrained DOFs get zero (solver.ts:400-407).
+// F_w(node) = the node's share of the 2x2 Gauss rule of every adjacent cell
+// (grid_node_load_q4); rotation loads are zero and constrained DOFs get zero
+// (solver.ts:400-407).
 __global__ void k_pressure(KGrid G, double q0, const double* pt, int np, float* F) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nn = (int64_t)(G.nx + 1) * (G.ny + 1);
   if (n >= nn) return;
   const int i = (int)(n % (G.nx + 1)), j = (int)(n / (G.nx + 1));
-  const int ei[4] = {i - 1, i, i, i - 1}, ej[4] = {j - 1, j - 1, j, j};
-  const int corner[4] = {2, 3, 0, 1};
-  const double g = 0.5773502691896257;
-  const double gxi[4] = {-g, g, g, -g}, geta[4] = {-g, -g, g, g};
-  const double cxi[4] = {-1, 1, 1, -1}, ceta[4] = {-1, -1, 1, 1};
-  double f = 0;
-  for (int k = 0; k < 4; k++) {
-    const int gi = ei[k], gj = ej[k];
-    if (gi < 0 || gi >= G.nx || gj < 0 || gj >= G.ny) continue;
-    const double x0 = G.xc[gi], x1 = G.xc[gi + 1], y0 = G.yc[gj], y1 = G.yc[gj + 1];
-    const double detJ = 0.25 * (x1 - x0) * (y1 - y0);
-    for (int q = 0; q < 4; q++) {
-      const double xi = gxi[q], eta = geta[q];
-      const double px = 0.5 * (1 - xi) * x0 + 0.5 * (1 + xi) * x1;
-      const double py = 0.5 * (1 - eta) * y0 + 0.5 * (1 + eta) * y1;
-      const double N = 0.25 * (1 + cxi[corner[k]] * xi) * (1 + ceta[corner[k]] * eta);
-      f += N * pressure_at(px, py, q0, pt, np) * fabs(detJ);
-    }
-  }
+  const double f = grid_node_load_q4(G.xc, G.yc, G.nx, G.ny, i, j, q0, pt, np);
   const unsigned fix = G.cmask[n];
   F[3 * n] = (fix & PP_FIX_W) ? 0.0f : (float)f;
   F[3 * n + 1] = 0.0f;

@@ -5,7 +5,7 @@ package binds its C ABI and builds problem inputs.
 """
 from . import _lib, problem
 from ._lib import PlatePCGError, load
-from .solver import Batch, Context, Net, Plan, solve_gpu, stats_dict
+from .solver import Batch, Context, Net, Plan, lognormal_thickness, solve_gpu, stats_dict
 
-__all__ = ["Batch", "Context", "Net", "Plan", "solve_gpu", "stats_dict", "PlatePCGError", "load", "problem",
+__all__ = ["Batch", "Context", "Net", "Plan", "lognormal_thickness", "solve_gpu", "stats_dict", "PlatePCGError", "load", "problem",
            "_lib"]

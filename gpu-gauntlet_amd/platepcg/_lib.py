@@ -119,6 +119,9 @@ SIGNATURES = {
     "pp_apply_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_get_block_inverse": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_pressure_load": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_void_p]),
+    "pp_lognormal_thickness": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                         C.c_double, C.c_uint64, C.c_double, C.c_double,
+                                         C.c_void_p, C.c_void_p]),
     "pp_moments": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_max_displacement": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
@@ -143,6 +146,7 @@ SIGNATURES = {
     "pp_batch_plate_count": (C.c_int32, [C.c_void_p]),
     "pp_batch_pressure_load": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
                                          C.c_int32, C.c_void_p]),
+    "pp_batch_pressure_loads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_batch_set_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "pp_batch_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule)]),
     "pp_batch_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule), C.c_void_p]),

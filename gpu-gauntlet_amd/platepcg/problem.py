@@ -230,9 +230,11 @@ def batch_plates(count=256, seed=4, n_min=16, n_max=200):
     return plates
 
 
-def workload(name, seed=None):
+def workload(name, seed=None, element_t=None):
     """The named synthetic workloads.  Returns a dict with mesh, material,
-    mask, q0 and patches (A, cx, cy, sigma per Gaussian patch)."""
+    mask, q0 and patches (A, cx, cy, sigma per Gaussian patch).  For "c5",
+    element_t may supply the thickness field already made (on the device by
+    platepcg.lognormal_thickness) instead of the host synthesis here."""
     if name == "c1":
         mesh = rect_mesh(32)
         return dict(name=name, mesh=mesh, E=E_STEEL, nu=NU_STEEL, t=0.1,
@@ -250,12 +252,14 @@ def workload(name, seed=None):
         # length 0.5 m (seed 5); 10 kPa + three Gaussian patches
         n = 1024
         mesh = rect_mesh(n)
-        g = gaussian_field(n, n, 10.0, 10.0, 0.5, 5 if seed is None else seed)
+        if element_t is None:
+            g = gaussian_field(n, n, 10.0, 10.0, 0.5, 5 if seed is None else seed)
+            element_t = (0.1 * np.exp(0.5 * g)).astype(np.float32).ravel()
         rng = SplitMix64(1005 if seed is None else seed + 1000)
         patches = np.array([[rng.uniform(10e3, 50e3), rng.uniform(0, 10), rng.uniform(0, 10), 1.0]
                             for _ in range(3)])
         return dict(name=name, mesh=mesh, E=E_STEEL, nu=NU_STEEL, t=0.1,
-                    element_t=(0.1 * np.exp(0.5 * g)).astype(np.float32).ravel(),
+                    element_t=np.ascontiguousarray(element_t, np.float32).ravel(),
                     mask=constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}]),
                     q0=10e3, patches=patches)
     if name == "c3":

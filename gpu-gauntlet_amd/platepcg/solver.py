@@ -40,6 +40,18 @@ class Context:
             pass
 
 
+def lognormal_thickness(ctx, nx, ny, width, height, corr_len, seed, t_median, sigma_log):
+    """(g, t): the unit-variance Gaussian field (FP64) and the float32
+    element thickness t_median exp(sigma_log g) of an nx x ny grid,
+    synthesised on the device; both (ny, nx), x fastest."""
+    g = np.empty((ny, nx), np.float64)
+    t = np.empty((ny, nx), np.float32)
+    check(ctx.lib.pp_lognormal_thickness(ctx.h, int(nx), int(ny), float(width), float(height),
+                                         float(corr_len), int(seed), float(t_median),
+                                         float(sigma_log), _ptr(g), _ptr(t)))
+    return g, t
+
+
 class Plan:
     """Set-up for one mesh/material/support set; solve many load cases."""
 
@@ -311,6 +323,20 @@ class Batch:
         check(self.lib.pp_batch_pressure_load(self.h, plate, float(q0), _ptr(pt) if pt.size else None,
                                               int(pt.shape[0]), _ptr(F)))
         return F
+
+    def pressure_loads(self, q0s, patches_list):
+        """Every plate's load computed on the device straight into the
+        batch's right-hand sides (plate p: uniform q0s[p] plus the (k, 4)
+        array patches_list[p])."""
+        q = np.ascontiguousarray(q0s, np.float64)
+        if q.size != self.n or len(patches_list) != self.n:
+            raise ValueError(f"need {self.n} loads")
+        pts = [np.zeros((0, 4)) if p is None else np.asarray(p, np.float64).reshape(-1, 4)
+               for p in patches_list]
+        off = np.zeros(self.n + 1, np.int32)
+        off[1:] = np.cumsum([len(p) for p in pts])
+        flat = np.ascontiguousarray(np.concatenate(pts) if off[-1] else np.zeros(4), np.float64)
+        check(self.lib.pp_batch_pressure_loads(self.h, _ptr(q), _ptr(flat), _ptr(off)))
 
     def set_rhs(self, plate, F):
         F = np.ascontiguousarray(F, np.float32)

@@ -228,6 +228,20 @@ int pp_get_block_inverse(pp_plan* plan, double* blocks);
  * counterpart: the reference has only point loads (solver.ts:363-392). */
 int pp_pressure_load(pp_plan* plan, double q0, const double* patches, int n_patches, float* F);
 
+/* Log-normal element thickness of the learned-preconditioner workload,
+ * synthesised on the device: g is a unit-variance Gaussian random field on
+ * the element centres of an nx x ny grid of a width x height plate with
+ * squared-exponential covariance exp(-d^2 / (2 corr_len^2)) (spectral
+ * synthesis from the SplitMix64 normals of `seed` on a grid padded to
+ * 2nx x 2ny, cropped, centred and scaled to unit sample variance), and
+ * t_e = t_median exp(sigma_log g_e) as float32.  field (nx * ny doubles) or
+ * element_t (nx * ny floats) may be NULL, not both; both are host pointers,
+ * row-major with x fastest (element j * nx + i).  No reference counterpart:
+ * the reference's plates have uniform thickness (types.ts PlateMaterial). */
+int pp_lognormal_thickness(pp_ctx* ctx, int32_t nx, int32_t ny, double width, double height,
+                           double corr_len, uint64_t seed, double t_median, double sigma_log,
+                           double* field, float* element_t);
+
 /* Bending moments at the nodes: replaces computeMoments
  * (postprocess.ts:40-270).  Curvatures at each element's centroid (Q4: the
  * isoparametric derivatives at xi = eta = 0; triangles: linear shape
@@ -386,9 +400,21 @@ int pp_batch_create(pp_ctx* ctx, int32_t n_plates, const pp_mesh* meshes,
                     pp_batch** out);
 int pp_batch_destroy(pp_batch* batch);
 int32_t pp_batch_plate_count(const pp_batch* batch);
-/* Consistent nodal pressure load of one plate, as pp_pressure_load. */
+/* Consistent nodal pressure load of one plate, as pp_pressure_load
+ * (computed on the device; Q4 plates by the 2x2 Gauss rule, DKT plates by the
+ * triangles' 3-point rule).  Replaces buildLoadVector + applyBCsToRHS
+ * (solver.ts:363-407) for one plate of the batch. */
 int pp_batch_pressure_load(pp_batch* batch, int32_t plate, double q0, const double* patches,
                            int32_t n_patches, float* F);
+/* Every plate's pressure load at once, on the device, written straight into
+ * the batch's right-hand sides (what pp_batch_pressure_load followed by
+ * pp_batch_set_rhs does for each plate, without the host round trip).
+ * q0: n_plates uniform pressures; plate p's Gaussian patches are
+ * patches[4 * patch_offsets[p] .. 4 * patch_offsets[p + 1]) as (A, cx, cy, s);
+ * patch_offsets has n_plates + 1 nondecreasing entries from 0.  The loop over
+ * plates of solver.ts:363-407, batched. */
+int pp_batch_pressure_loads(pp_batch* batch, const double* q0, const double* patches,
+                            const int32_t* patch_offsets);
 /* F of one plate (3 * node_count floats). */
 int pp_batch_set_rhs(pp_batch* batch, int32_t plate, const float* F);
 int pp_batch_set_schedule(pp_batch* batch, const pp_batch_schedule* schedule);
