@@ -56,9 +56,10 @@ def parse():
                          "150 for c3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (profiler calibration)")
-    ap.add_argument("--probe", choices=["copy", "pcg"],
-                    help="only run the traffic probe (a 256 MiB copy, or one 142 x 142 "
-                         "block-Jacobi solve) for the rocprofv3 counter runs")
+    ap.add_argument("--probe", choices=["copy", "pcg", "pcg1024", "mg1024"],
+                    help="only run a probe for the rocprofv3 runs: a 256 MiB copy, one "
+                         "142 x 142 block-Jacobi solve, 1,500 block-Jacobi iterations on the "
+                         "1024 x 1024 plate, or its multigrid-preconditioned solve")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs (traffic: null)")
     ap.add_argument("--no-c5", action="store_true", help="default run without the 1024 x 1024 legs")
@@ -107,13 +108,35 @@ def probe_pcg():
     ctx.close()
 
 
-def measure_traffic(kernel_prefix="k_pcg_cg"):
-    """HBM bytes per launch of the persistent block-Jacobi kernel on the
-    142 x 142 plate: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch), one child run per counter, each scaled by what the same counter
-    reads for a 256 MiB device-to-device copy (FETCH_SIZE reads 2x the copied
-    bytes, WRITE_SIZE 1x on MI355X).  Runs before this process touches the GPU.
-    Returns (bytes, details) or (None, reason)."""
+def probe_1024(multigrid):
+    """The 1024 x 1024 plate (thickness field made on the device): 1,500
+    iterations of the strided block-Jacobi kernel in one launch, or the whole
+    multigrid-preconditioned solve to 1e-8."""
+    from platepcg import Context, Plan, _lib, lognormal_thickness
+    from platepcg import problem as P
+    ctx = Context(0)
+    _, t_e = lognormal_thickness(ctx, 1024, 1024, 10.0, 10.0, 0.5, 5, 0.1, 0.5)
+    w = P.workload("c5", element_t=t_e)
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"], element_t=w["element_t"])
+    plan.set_rhs(plan.pressure_load(w["q0"], w["patches"]))
+    if multigrid:
+        plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=5000,
+                                         precond=_lib.PRECOND_MULTIGRID))
+    else:
+        plan.solve_resident(plan.options(tolerance=1e-30, max_iterations=1500,
+                                         flags=_lib.FLAG_NO_TRUE_RESIDUAL))
+    plan.close()
+    ctx.close()
+
+
+def measure_traffic(kernel_prefix="k_pcg_cg", probe_name="pcg"):
+    """HBM bytes per launch of a kernel in a probe run (default: the
+    persistent block-Jacobi kernel on the 142 x 142 plate): rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one child run per counter,
+    each scaled by what the same counter reads for a 256 MiB device-to-device
+    copy (FETCH_SIZE reads 2x the copied bytes, WRITE_SIZE 1x on MI355X).
+    Runs before this process touches the GPU.  Returns (bytes, details) or
+    (None, reason)."""
     import csv
     import shutil
     import tempfile
@@ -124,11 +147,11 @@ def measure_traffic(kernel_prefix="k_pcg_cg"):
     out = {}
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-            for probe in ("copy", "pcg"):
+            for probe in ("copy", probe_name):
                 tag = f"{ctr}_{probe}"
                 cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", os.path.join(d, tag),
                        "-o", "r", "--", sys.executable, os.path.abspath(__file__), "--probe", probe]
-                r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, cwd=ROOT)
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=ROOT)
                 if r.returncode != 0:
                     return None, {"error": f"rocprofv3 {tag} exit {r.returncode}: "
                                            f"{(r.stderr or '')[-300:]}"}
@@ -154,8 +177,8 @@ def measure_traffic(kernel_prefix="k_pcg_cg"):
     copy_kib = 256 * 1024
     fs = out["FETCH_SIZE_copy"] / copy_kib
     ws = out["WRITE_SIZE_copy"] / copy_kib
-    fetch = out["FETCH_SIZE_pcg"] / fs * 1024
-    write = out["WRITE_SIZE_pcg"] / ws * 1024
+    fetch = out["FETCH_SIZE_" + probe_name] / fs * 1024
+    write = out["WRITE_SIZE_" + probe_name] / ws * 1024
     return int(fetch + write), {"fetch_bytes": int(fetch), "write_bytes": int(write),
                                 "fetch_scale": fs, "write_scale": ws,
                                 "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, one child run "
@@ -414,6 +437,16 @@ def c5_legs(a):
                        "frac": ach / HBM_PEAK_GBS, "traffic": None,
                        "kernel": "k_pcg (block-Jacobi FP64 PCG, tiles strided over the CUs)",
                        "algorithmic_bytes_per_launch": byt}
+    tr = getattr(a, "c5_traffic", None)
+    if tr and tr[0]:
+        # the probe launch runs 1,500 iterations of the same plate; its
+        # counter bytes per iteration times this solve's iterations
+        res["roofline"]["traffic"] = tr[0] / 1500 * it_bj
+        res["roofline"]["traffic_measurement"] = dict(tr[1], probe="1,500 iterations in one "
+                                                      "launch (bench.py --probe pcg1024)",
+                                                      bytes_per_iteration=tr[0] / 1500)
+    elif tr:
+        res["roofline"]["traffic_measurement"] = tr[1]
     if a.cpu_sample_iters != 0:
         try:
             per, st_js = js_ms_per_dof_iteration()
@@ -446,6 +479,9 @@ def main():
     if a.probe == "pcg":
         probe_pcg()
         return
+    if a.probe in ("pcg1024", "mg1024"):
+        probe_1024(a.probe == "mg1024")
+        return
     if a.workload == "c4":
         bench_batch(a)
         return
@@ -461,6 +497,12 @@ def main():
         except (OSError, subprocess.SubprocessError, ValueError, KeyError) as e:
             traffic, traffic_info = None, {"error": str(e)}
         print(f"traffic: {traffic} ({traffic_info})", file=sys.stderr, flush=True)
+        if not a.no_c5:
+            try:
+                a.c5_traffic = measure_traffic("k_pcg<", "pcg1024")
+            except (OSError, subprocess.SubprocessError, ValueError, KeyError) as e:
+                a.c5_traffic = (None, {"error": str(e)})
+            print(f"c5 traffic: {a.c5_traffic}", file=sys.stderr, flush=True)
     import numpy as np
 
     from platepcg import Context, Plan, _lib

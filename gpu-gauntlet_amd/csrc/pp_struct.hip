@@ -224,6 +224,7 @@ struct PcgArgs {
 template <class C>
 __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
   static_assert(C::NE <= C::NT, "one element per thread");
+  static_assert(6 * C::NT <= 12 * C::NE, "phase B stages two chunk vectors in ye_s");
   constexpr int SPT = (C::NP + C::NT - 1) / C::NT;  // staged nodes per thread
   __shared__ double p_s[3 * C::NP];
   __shared__ double ye_s[12 * C::NE];
@@ -360,16 +361,33 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
         for (int k = 0; k < 12; k++) ye_s[k * C::NE + le] = y[k];
       }
       __syncthreads();
+      double v[3] = {0, 0, 0};
+      if (mine) node_sum<C, double>(tx, ty, ye_s, v);
+      __syncthreads();  // every node sum has read ye_s
       if (mine) {
-        double v[3];
-        node_sum<C, double>(tx, ty, ye_s, v);
         const int c = (ty + 1) * C::PW + tx + 1;
         for (int d = 0; d < 3; d++) {
           const double pd = p_s[d * C::NP + c];
           const double ad = ((fix >> d) & 1u) ? pd : v[d];
-          st_coh(&pnew[3 * n + d], pd);
           pap[0] += pd * ad;
-          st_coh(&A.Ap[3 * n + d], ad);  // read by the node range's owner
+          ye_s[3 * (ty * C::TX + tx) + d] = ad;
+        }
+      }
+      __syncthreads();
+      // p and K p of the tile's own nodes go out row by row with consecutive
+      // lanes on consecutive doubles: a device-coherent store is written
+      // through per instruction, so node-strided stores (lane = node,
+      // one instruction per component) would write each line three times
+      {
+        const int rw = min(C::TX, G.nx + 1 - i0), rows = min(C::TY, G.ny + 1 - j0);
+        for (int t = threadIdx.x; t < 3 * C::TX * rows; t += C::NT) {
+          const int r = t / (3 * C::TX), k = t % (3 * C::TX);
+          if (k < 3 * rw) {
+            const int lx = k / 3, d = k % 3;
+            const size_t g = 3 * ((size_t)(j0 + r) * (G.nx + 1) + i0) + k;
+            st_coh(&pnew[g], p_s[d * C::NP + (r + 1) * C::PW + lx + 1]);
+            st_coh(&A.Ap[g], ye_s[3 * (r * C::TX + lx) + d]);  // read by the node range's owner
+          }
         }
       }
       __syncthreads();
@@ -381,23 +399,38 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     const double alpha = rz / pap[0];
 
     // ---------------- phase B: x, r, z updates, r.z, r.r
+    // Chunks of NT nodes: p and K p come in and z goes out through LDS with
+    // consecutive lanes on consecutive doubles (whole lines per coherent
+    // access); thread k of a chunk owns node nb + k as before.
     acc[0] = 0; acc[1] = 0;
-    for (int64_t n = lo + threadIdx.x; n < hi; n += C::NT) {
-      {
+    double* lb = ye_s;  // [3 NT] p then z, [3 NT] K p
+    for (int64_t nb = lo; nb < hi; nb += C::NT) {
+      const int cnt = (int)min((int64_t)C::NT, hi - nb);
+      for (int t = threadIdx.x; t < 3 * cnt; t += C::NT) {
+        lb[t] = ld_coh(&pnew[3 * nb + t]);
+        lb[3 * C::NT + t] = ld_coh(&A.Ap[3 * nb + t]);
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) {
+        const int64_t n = nb + threadIdx.x;
+        double* own = lb + 3 * threadIdx.x;
         double rv[3], zv[3], m[6];
         for (int k = 0; k < 6; k++) m[k] = A.minv[6 * n + k];
         for (int d = 0; d < 3; d++) {
-          A.x[3 * n + d] += alpha * ld_coh(&pnew[3 * n + d]);
-          rv[d] = A.r[3 * n + d] - alpha * ld_coh(&A.Ap[3 * n + d]);
+          A.x[3 * n + d] += alpha * own[d];
+          rv[d] = A.r[3 * n + d] - alpha * own[3 * C::NT + d];
           A.r[3 * n + d] = rv[d];
         }
         sym3_apply(m, rv, zv);
         for (int d = 0; d < 3; d++) {
-          st_coh(&A.z[3 * n + d], zv[d]);
+          own[d] = zv[d];  // this thread's own entries only
           acc[0] += rv[d] * zv[d];
           acc[1] += rv[d] * rv[d];
         }
       }
+      __syncthreads();
+      for (int t = threadIdx.x; t < 3 * cnt; t += C::NT) st_coh(&A.z[3 * nb + t], lb[t]);
+      __syncthreads();  // before the next chunk overwrites lb
     }
     if (timing) t3 = wall_clock64();
     if (!allreduce(acc)) return;
