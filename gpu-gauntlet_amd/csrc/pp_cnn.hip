@@ -270,6 +270,284 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   }
 }
 
+// Forward convolution with the shapes fixed at compile time: dilation 1,
+// input channels in chunks of CC (16, 32 or 48), NB blocks of 16 output
+// channels, an output tile of MB rows x 64 pixels (4 waves, wave w owning the
+// 16-pixel blocks w MB .. w MB + MB - 1, row-major, 4 per row).  The same
+// implicit GEMM as k_conv, but every staging split is a division by a
+// constant, the input and the weights come in 16-byte loads (8 channels),
+// all loads of a chunk are issued before the first LDS store, and the tap
+// loop is unrolled so every LDS read of the K loop has a constant offset
+// from the lane's base.  Weight rows in LDS are padded by 4 channels so the
+// 16 rows an MFMA reads fall in distinct banks.  Tiles are dealt to the
+// XCDs in row-major bands (workgroup b runs on XCD b % 8), so the halo rows
+// two vertically adjacent tiles share are fetched into one L2.  Inputs and
+// weights need channel strides and offsets that are multiples of 8 (the
+// launch checks).
+template <int CC, int NB, int MB>
+__global__ __launch_bounds__(256) void k_convf(ConvArgs A, int ntiles) {
+  constexpr int TW = 66, TH = MB + 2, CS = CC + 4, KC = 9 * CC, KP = KC + 4;
+  constexpr int Q8 = CC / 8;                                // 16-byte units per pixel
+  constexpr int NIN = TH * TW * Q8, NW = NB * 16 * KC / 8;  // 16-byte units per chunk
+  constexpr int UIN = (NIN + 255) / 256, UW = (NW + 255) / 256;
+  static_assert(CC % 16 == 0 && KC % 16 == 0, "16-wide K steps within one tap");
+  extern __shared__ uint16_t smem[];
+  uint16_t* w_s = smem;
+  uint16_t* in_s = smem + NB * 16 * KP;
+  const int band = (ntiles + 7) / 8;
+  const int tile = (int)(blockIdx.x % 8) * band + (int)(blockIdx.x / 8);
+  if (tile >= ntiles) return;
+  const int tiles_x = (A.W + 63) / 64;
+  const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  int pbase[MB];
+#pragma unroll
+  for (int j = 0; j < MB; j++) {
+    const int mb = wave * MB + j;
+    pbase[j] = ((mb >> 2) * TW + (mb & 3) * 16 + m) * CS + 4 * kq;
+  }
+  f4 acc[MB][NB];
+#pragma unroll
+  for (int j = 0; j < MB; j++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[j][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int c0 = 0; c0 < A.cin; c0 += CC) {
+    if (c0) __syncthreads();  // the previous chunk's reads are done
+    uint4 vw[UW], vi[UIN];
+#pragma unroll
+    for (int u = 0; u < UW; u++) {
+      const int i = threadIdx.x + 256 * u;
+      vw[u] = uint4{0u, 0u, 0u, 0u};
+      if (i < NW) {
+        const int co = i / (KC / 8), kk = (i % (KC / 8)) * 8;
+        const int tap = kk / CC, ci = kk % CC;
+        vw[u] = *(const uint4*)&A.w[(size_t)co * A.kp + tap * A.cin + c0 + ci];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UIN; u++) {
+      const int i = threadIdx.x + 256 * u;
+      vi[u] = uint4{0u, 0u, 0u, 0u};
+      if (i < NIN) {
+        const int pix = i / Q8, part = i % Q8;
+        const int gx = tx0 - 1 + pix % TW, gy = ty0 - 1 + pix / TW;
+        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+          vi[u] = *(const uint4*)&A.in[(gy * A.W + gx) * A.in_cs + A.in_c0 + c0 + 8 * part];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UW; u++) {
+      const int i = threadIdx.x + 256 * u;
+      if (i < NW) {
+        uint16_t* d = &w_s[(i / (KC / 8)) * KP + (i % (KC / 8)) * 8];
+        *(uint2*)d = uint2{vw[u].x, vw[u].y};
+        *(uint2*)(d + 4) = uint2{vw[u].z, vw[u].w};
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UIN; u++) {
+      const int i = threadIdx.x + 256 * u;
+      if (i < NIN) {
+        uint16_t* d = &in_s[(i / Q8) * CS + 8 * (i % Q8)];
+        *(uint2*)d = uint2{vi[u].x, vi[u].y};
+        *(uint2*)(d + 4) = uint2{vi[u].z, vi[u].w};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+      const int toff = ((tap / 3) * TW + tap % 3) * CS;
+#pragma unroll
+      for (int c16 = 0; c16 < CC; c16 += 16) {
+        const int kb = tap * CC + c16 + 4 * kq;
+        s4 b[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KP + kb];
+        s4 a[MB];
+#pragma unroll
+        for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff + c16];
+#pragma unroll
+        for (int j = 0; j < MB; j++)
+#pragma unroll
+          for (int nb = 0; nb < NB; nb++)
+            acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MB; j++) {
+    const int mb = wave * MB + j;
+    const int oy = ty0 + (mb >> 2), ox = tx0 + (mb & 3) * 16 + m;
+    if (oy >= A.H || ox >= A.W) continue;
+    const int op = oy * A.W + ox;  // 32-bit offsets: the launch checks they fit
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+      const int co = nb * 16 + 4 * kq;
+      if (co >= A.cout) continue;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = acc[j][nb][i] + (A.bias && co + i < A.cout ? A.bias[co + i] : 0.0f);
+        if (A.relu) v[i] = fmaxf(v[i], 0.0f);
+      }
+      if (co + 4 <= A.cout) {
+        if (A.out32) *(float4*)&A.out32[op * A.out32_cs + co] = make_float4(v[0], v[1], v[2], v[3]);
+        else st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
+      } else {
+        for (int i = 0; i < 4 && co + i < A.cout; i++) {
+          if (A.out32) A.out32[op * A.out32_cs + co + i] = v[i];
+          else A.out[op * A.out_cs + A.out_c0 + co + i] = f2bf(v[i]);
+        }
+      }
+    }
+  }
+}
+
+// Persistent form of k_convf: each workgroup stages all the layer's weights
+// once ([NB * 16][9 cin + 4], the K order of the packed weights), then walks
+// the tiles of its XCD's band (tile = band start + local index, stepping by
+// the workgroups on that XCD) chunk by chunk (CC input channels at a time),
+// loading the next (tile, chunk) input into registers while the current one
+// is multiplied, so the load latency of every stage but the first is hidden
+// behind the previous stage's work.
+template <int CC, int NB, int MB>
+__global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
+  constexpr int TW = 66, TH = MB + 2, CS = CC + 4;
+  constexpr int Q8 = CC / 8;
+  constexpr int NIN = TH * TW * Q8, UIN = (NIN + 255) / 256;
+  static_assert(CC % 16 == 0, "16-wide K steps within one tap");
+  extern __shared__ uint16_t smem[];
+  const int cin = A.cin, nch = cin / CC, K9 = 9 * cin, KP = K9 + 4;
+  uint16_t* w_s = smem;
+  uint16_t* in_s = smem + NB * 16 * KP;
+  const int band = (ntiles + 7) / 8, xcd = (int)(blockIdx.x % 8);
+  const int per_xcd = (int)(gridDim.x / 8), local0 = (int)(blockIdx.x / 8);
+  const int tile_lo = xcd * band, tile_hi = min(ntiles, tile_lo + band);
+  const int tiles_x = (A.W + 63) / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  int pbase[MB];
+#pragma unroll
+  for (int j = 0; j < MB; j++) {
+    const int mb = wave * MB + j;
+    pbase[j] = ((mb >> 2) * TW + (mb & 3) * 16 + m) * CS + 4 * kq;
+  }
+  // weights, once: row co holds k = tap * cin + ci, 9 cin entries; kStage
+  // loads in flight per thread before their LDS stores
+  {
+    const int units = K9 / 8, nw = NB * 16 * units;
+    for (int i0 = threadIdx.x; i0 < nw; i0 += 256 * kStage) {
+      uint4 v[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; u++) {
+        const int i = i0 + 256 * u;
+        if (i < nw) {
+          const int co = i / units, kk = (i - co * units) * 8;
+          v[u] = *(const uint4*)&A.w[(size_t)co * A.kp + kk];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; u++) {
+        const int i = i0 + 256 * u;
+        if (i < nw) {
+          const int co = i / units, kk = (i - co * units) * 8;
+          uint16_t* d = &w_s[co * KP + kk];
+          *(uint2*)d = uint2{v[u].x, v[u].y};
+          *(uint2*)(d + 4) = uint2{v[u].z, v[u].w};
+        }
+      }
+    }
+  }
+  uint4 vi[UIN];
+  auto load_stage = [&](int tile, int c0) {
+    const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
+#pragma unroll
+    for (int u = 0; u < UIN; u++) {
+      const int i = threadIdx.x + 256 * u;
+      vi[u] = uint4{0u, 0u, 0u, 0u};
+      if (i < NIN) {
+        const int pix = i / Q8, part = i % Q8;
+        const int gx = tx0 - 1 + pix % TW, gy = ty0 - 1 + pix / TW;
+        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+          vi[u] = *(const uint4*)&A.in[(gy * A.W + gx) * A.in_cs + A.in_c0 + c0 + 8 * part];
+      }
+    }
+  };
+  int tile = tile_lo + local0;
+  if (tile < tile_hi) load_stage(tile, 0);
+  for (; tile < tile_hi; tile += per_xcd) {
+    f4 acc[MB][NB];
+#pragma unroll
+    for (int j = 0; j < MB; j++)
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) acc[j][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ch = 0; ch < nch; ch++) {
+      __syncthreads();  // the previous stage's LDS reads are done
+#pragma unroll
+      for (int u = 0; u < UIN; u++) {
+        const int i = threadIdx.x + 256 * u;
+        if (i < NIN) {
+          uint16_t* d = &in_s[(i / Q8) * CS + 8 * (i % Q8)];
+          *(uint2*)d = uint2{vi[u].x, vi[u].y};
+          *(uint2*)(d + 4) = uint2{vi[u].z, vi[u].w};
+        }
+      }
+      __syncthreads();
+      if (ch + 1 < nch) load_stage(tile, (ch + 1) * CC);
+      else if (tile + per_xcd < tile_hi) load_stage(tile + per_xcd, 0);
+      const int kc0 = ch * CC + 4 * kq;
+#pragma unroll
+      for (int tap = 0; tap < 9; tap++) {
+        const int toff = ((tap / 3) * TW + tap % 3) * CS;
+#pragma unroll
+        for (int c16 = 0; c16 < CC; c16 += 16) {
+          const int kb = tap * cin + kc0 + c16;
+          s4 b[NB];
+#pragma unroll
+          for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KP + kb];
+          s4 a[MB];
+#pragma unroll
+          for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff + c16];
+#pragma unroll
+          for (int j = 0; j < MB; j++)
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+              acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+        }
+      }
+    }
+    const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+      const int mb = wave * MB + j;
+      const int oy = ty0 + (mb >> 2), ox = tx0 + (mb & 3) * 16 + m;
+      if (oy >= A.H || ox >= A.W) continue;
+      const int op = oy * A.W + ox;
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        const int co = nb * 16 + 4 * kq;
+        if (co >= A.cout) continue;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          v[i] = acc[j][nb][i] + (A.bias && co + i < A.cout ? A.bias[co + i] : 0.0f);
+          if (A.relu) v[i] = fmaxf(v[i], 0.0f);
+        }
+        if (co + 4 <= A.cout) {
+          if (A.out32) *(float4*)&A.out32[op * A.out32_cs + co] = make_float4(v[0], v[1], v[2], v[3]);
+          else st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
+        } else {
+          for (int i = 0; i < 4 && co + i < A.cout; i++) {
+            if (A.out32) A.out32[op * A.out32_cs + co + i] = v[i];
+            else A.out[op * A.out_cs + A.out_c0 + co + i] = f2bf(v[i]);
+          }
+        }
+      }
+    }
+  }
+}
+
 // Weight gradient dW[co][tap*cin + ci] = sum_p dout[p][co] in[p + off(tap)][ci]:
 // a GEMM over pixels (K), M = output channels, N = (tap, ci).  One row
 // segment of 64 pixels at a time is staged transposed in LDS (dout as
@@ -799,6 +1077,64 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
   if (mask) {
     A.mask = mask->p; A.mask_cs = mask->cs; A.mask_c0 = mask->c0; A.mask_from = mask_from;
   }
+  const int64_t cs_max = std::max({(int64_t)A.in_cs, (int64_t)A.out_cs, (int64_t)A.out32_cs,
+                                    (int64_t)A.mask_cs});
+  if ((int64_t)H[lev] * W[lev] * cs_max + cs_max >= (1LL << 31))
+    return fail(PP_ERR_INVALID, "conv: tensor too large for 32-bit offsets");
+  hipStream_t s = stream;
+  // forward passes of the network's own shapes: the compile-time kernels
+  if (!backward && !mask && A.dil == 1 && cin % 16 == 0 && A.in_cs % 8 == 0 &&
+      A.in_c0 % 8 == 0) {
+    // input channels in chunks of 32 or 16; the persistent kernel holds all
+    // the weights, the tile-per-workgroup kernel one chunk of them
+    const int cc = cin % 32 == 0 ? 32 : 16;
+    A.cc = cc;
+    {
+      auto tiles_p = [&](int mb) { return ((W[lev] + 63) / 64) * ((H[lev] + mb - 1) / mb); };
+      auto lds_p = [&](int nbk, int mb) {
+        return ((size_t)nbk * 16 * (9 * cin + 4) + (size_t)(mb + 2) * 66 * (cc + 4)) * 2;
+      };
+      // workgroups: what stays resident (kernel occupancy x CUs), whole XCD
+      // groups, at most one per tile
+#define PP_CONVP(C, N, R)                                                                  \
+  if (cc == C && nb == N && lds_p(N, R) <= cap) {                                          \
+    const int nt = tiles_p(R);                                                             \
+    int occ = 0;                                                                           \
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_convp<C, N, R>, 256,       \
+                                                        lds_p(N, R)));                     \
+    const int g = 8 * std::max(1, std::min((nt + 7) / 8, std::max(1, occ) * cu / 8));      \
+    hipLaunchKernelGGL((k_convp<C, N, R>), dim3(g), dim3(256), lds_p(N, R), s, A, nt);     \
+    PP_HIP(hipGetLastError());                                                             \
+    return PP_OK;                                                                          \
+  }
+      // the first shape that leaves room for two workgroups per CU, else one
+      for (size_t cap : {lds_max / 2, lds_max}) {
+        PP_CONVP(16, 1, 4)
+        PP_CONVP(16, 2, 4)
+        PP_CONVP(32, 2, 4)
+        PP_CONVP(32, 2, 2)
+        PP_CONVP(32, 4, 2)
+      }
+#undef PP_CONVP
+    }
+    auto tiles_of = [&](int mb) { return ((W[lev] + 63) / 64) * ((H[lev] + mb - 1) / mb); };
+    auto lds_of = [&](int nbk, int mb) {
+      return ((size_t)nbk * 16 * (9 * cc + 4) + (size_t)(mb + 2) * 66 * (cc + 4)) * 2;
+    };
+#define PP_CONVF(C, N, R)                                                                \
+  if (cc == C && nb == N && lds_of(N, R) <= lds_max) {                                   \
+    const int nt = tiles_of(R);                                                          \
+    hipLaunchKernelGGL((k_convf<C, N, R>), dim3(8 * ((nt + 7) / 8)), dim3(256), lds_of(N, R), \
+                       s, A, nt);                                                        \
+    PP_HIP(hipGetLastError());                                                           \
+    return PP_OK;                                                                        \
+  }
+    PP_CONVF(16, 1, 8)
+    PP_CONVF(16, 2, 4)
+    PP_CONVF(32, 2, 4)
+    PP_CONVF(32, 4, 2)
+#undef PP_CONVF
+  }
   // input channels per chunk: all of them up to 32, else 32 or 16
   A.cc = cin <= 32 ? cin : cin % 32 == 0 ? 32 : cin % 16 == 0 ? 16 : 4;
   const int kc = (9 * A.cc + 15) / 16 * 16;
@@ -812,16 +1148,11 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
     for (int mb : {4, 2})
       if (!MB && lds_of(mb) <= cap) MB = mb;
   if (!MB || cin % 4) return fail(PP_ERR_INVALID, "conv: layer too wide for LDS");
-  const int64_t cs_max = std::max({(int64_t)A.in_cs, (int64_t)A.out_cs, (int64_t)A.out32_cs,
-                                    (int64_t)A.mask_cs});
-  if ((int64_t)H[lev] * W[lev] * cs_max + cs_max >= (1LL << 31))
-    return fail(PP_ERR_INVALID, "conv: tensor too large for 32-bit offsets");
   if (64 + 2 * A.dil > 72 || (MB + 2 * A.dil) * (64 + 2 * A.dil) * (A.cc / 4) >= 65536 ||
       nb * 16 * (kc / 4) >= 65536 || kc / 4 > 72)
     return fail(PP_ERR_INVALID, "conv: input tile beyond the staging index range");
   const size_t lds = lds_of(MB);
   const int grid = ((W[lev] + 63) / 64) * ((H[lev] + MB - 1) / MB);
-  hipStream_t s = stream;
   const bool u16 = A.cc % 16 == 0;
   if (!u16 && nb != 1) return fail(PP_ERR_INVALID, "conv: narrow input chunks need one N block");
 #define PP_CONV(N, R, U) hipLaunchKernelGGL((k_conv<N, R, U>), dim3(grid), dim3(256), lds, s, A)
@@ -1020,6 +1351,13 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
                       (const void*)k_conv<4, 2, true>,  (const void*)k_conv<6, 2, true>,
                       (const void*)k_conv<1, 4, false>, (const void*)k_conv<1, 2, false>};
   for (auto k : ks)
+    PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+  const void* kf[] = {(const void*)k_convf<16, 1, 8>, (const void*)k_convf<16, 2, 4>,
+                      (const void*)k_convf<32, 2, 4>, (const void*)k_convf<32, 4, 2>,
+                      (const void*)k_convp<16, 1, 4>, (const void*)k_convp<16, 2, 4>,
+                      (const void*)k_convp<32, 2, 4>, (const void*)k_convp<32, 2, 2>,
+                      (const void*)k_convp<32, 4, 2>};
+  for (auto k : kf)
     PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
   size_t mw = 0;
   for (auto& L : layers)

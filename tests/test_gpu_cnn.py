@@ -57,10 +57,12 @@ def test_training_error_matches_oracle(ctx):
     net.close()
 
 
-@pytest.mark.parametrize("n,relu", [(30, False), (32, False), (32, True)])
+@pytest.mark.parametrize("n,relu", [(30, False), (32, False), (32, True), (130, False),
+                                    (130, True)])
 def test_forward_matches_oracle(ctx, n, relu):
     """M(r) on the GPU (BF16 MFMA, FP32 accumulate) against the oracle with
-    the same BF16 storage points; n = 30 exercises the padded grid."""
+    the same BF16 storage points; n = 30 exercises the padded grid, n = 130
+    several 64-pixel tiles per row on every level and partial row tiles."""
     mesh, mask, et, plan, op = plate(ctx, n)
     net = Net(plan, seed=1, relu=relu)
     ref = U.UNetOracle(n, n, op.blk, mask, 0.1, et, quantize=True, relu=relu)
