@@ -56,10 +56,12 @@ def parse():
                          "150 for c3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (profiler calibration)")
-    ap.add_argument("--probe", choices=["copy", "pcg", "pcg1024", "mg1024"],
+    ap.add_argument("--probe", choices=["copy", "pcg", "mg142", "pcg1024", "mg1024", "unet1024"],
                     help="only run a probe for the rocprofv3 runs: a 256 MiB copy, one "
-                         "142 x 142 block-Jacobi solve, 1,500 block-Jacobi iterations on the "
-                         "1024 x 1024 plate, or its multigrid-preconditioned solve")
+                         "142 x 142 block-Jacobi solve, five 142 x 142 multigrid-preconditioned "
+                         "solves, 1,500 block-Jacobi iterations on the 1024 x 1024 plate, its "
+                         "multigrid-preconditioned solve, or 20 applications of the learned "
+                         "preconditioner there (U-net forward + block-Jacobi smoothing)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs (traffic: null)")
     ap.add_argument("--no-c5", action="store_true", help="default run without the 1024 x 1024 legs")
@@ -104,6 +106,41 @@ def probe_pcg():
     plan.set_rhs(plan.pressure_load(w["q0"], w["patches"]))
     plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=400000,
                                      flags=_lib.FLAG_NO_TRUE_RESIDUAL))
+    plan.close()
+    ctx.close()
+
+
+def probe_mg142():
+    """Five multigrid-preconditioned solves of the 142 x 142 plate to 1e-8."""
+    from platepcg import Context, Plan, _lib
+    from platepcg import problem as P
+    w = P.workload("c2")
+    ctx = Context(0)
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"])
+    plan.set_rhs(plan.pressure_load(w["q0"], w["patches"]))
+    for _ in range(5):
+        plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=5000,
+                                         precond=_lib.PRECOND_MULTIGRID))
+    plan.close()
+    ctx.close()
+
+
+def probe_unet1024():
+    """20 applications of the learned preconditioner (block-Jacobi smoothing
+    around the U-net, random parameters) on the 1024 x 1024 plate."""
+    import numpy as np
+
+    from platepcg import Context, Net, Plan, lognormal_thickness
+    from platepcg import problem as P
+    ctx = Context(0)
+    _, t_e = lognormal_thickness(ctx, 1024, 1024, 10.0, 10.0, 0.5, 5, 0.1, 0.5)
+    w = P.workload("c5", element_t=t_e)
+    plan = Plan(ctx, w["mesh"], w["E"], w["nu"], w["t"], w["mask"], element_t=w["element_t"])
+    net = Net(plan, seed=5)
+    r = np.random.default_rng(1).standard_normal(plan.n_dof)
+    for _ in range(20):
+        net.apply(r)
+    net.close()
     plan.close()
     ctx.close()
 
@@ -481,6 +518,12 @@ def main():
         return
     if a.probe in ("pcg1024", "mg1024"):
         probe_1024(a.probe == "mg1024")
+        return
+    if a.probe == "mg142":
+        probe_mg142()
+        return
+    if a.probe == "unet1024":
+        probe_unet1024()
         return
     if a.workload == "c4":
         bench_batch(a)
