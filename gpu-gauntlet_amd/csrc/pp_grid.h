@@ -37,6 +37,23 @@ struct Tile {
   static_assert(TX * TY <= NT, "one node per thread");
 };
 
+// Tiles of workgroup me among nwg strided workgroups.  Workgroup me runs on
+// XCD me % 8 (the dispatcher deals workgroups to the eight XCDs in turn), so
+// the tiles are cut into eight row-major bands, one per XCD, and the
+// workgroups of an XCD take their band's tiles in turn: neighbouring tiles,
+// which share halo rows and the partial 128-byte lines at the ends of every
+// staged row, are then fetched into one L2.
+struct TileBand {
+  int first, last, step;
+};
+__device__ __forceinline__ TileBand xcd_tile_band(unsigned me, unsigned nwg, int ntiles) {
+  if (nwg < 8u) return TileBand{(int)me, ntiles, (int)nwg};  // not every XCD has a workgroup
+  const int x = (int)(me & 7u), rank = (int)(me >> 3);
+  const int on_x = (int)(nwg >> 3) + (x < (int)(nwg & 7u) ? 1 : 0);  // workgroups on XCD x
+  const int lo = (int)((int64_t)ntiles * x / 8), hi = (int)((int64_t)ntiles * (x + 1) / 8);
+  return TileBand{lo + rank, hi, on_x};
+}
+
 template <class C>
 __device__ __forceinline__ int tiles_x(const KGrid& G) {
   return (G.nx + 1 + C::TX - 1) / C::TX;

@@ -158,6 +158,18 @@ __device__ __forceinline__ bool stop_rule(double rr, MgfState* S, MgfRec* rec, i
 
 enum { M_STEP = 0, M_RES = 1, M_KP = 2 };
 
+// Tile of workgroup b in a launch of G workgroups, one per tile.  Workgroup
+// b runs on XCD b % 8 (the dispatcher deals workgroups to the eight XCDs in
+// turn), so XCD x is given the contiguous row-major tiles
+// [x (G / 8) + min(x, G % 8), ...): horizontally adjacent tiles, and most
+// vertically adjacent ones, then share one L2 for their halo rows and for the
+// partial 128-byte lines at the ends of every staged row.  A bijection of
+// [0, G), so per-workgroup partials keep their slots.
+__device__ __forceinline__ int xcd_band_tile(int b, int G) {
+  const int x = b & 7, q = G >> 3, r = G & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // stencil-level tile sides: steps 16 x 16 nodes, residuals 14 x 14 (a
 // 2-node halo keeps the computed nodes at 16 x 16: one per thread)
 constexpr int kTsS_ = 16, kTrS_ = 14;
@@ -306,7 +318,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   if (A.S->done) return;
   const int nx = A.L.nx, ny = A.L.ny;
   const int ntx = (nx + 1 + TS - 1) / TS;
-  const int i0 = (blockIdx.x % ntx) * TS, j0 = (blockIdx.x / ntx) * TS;
+  const int tile = xcd_band_tile((int)blockIdx.x, (int)gridDim.x);
+  const int i0 = (tile % ntx) * TS, j0 = (tile / ntx) * TS;
   // ---- this thread's computed node and its operands
   const int c = threadIdx.x;
   const int cx = c % T::CW, cy = c / T::CW;
@@ -651,7 +664,7 @@ template <int TS, int H, int MODE, bool PRO>
 __global__ __launch_bounds__(256) void k_st(TileArgs A) {
   __shared__ StSmem sm;
   if (A.S->done) return;
-  st_tile<TS, H, MODE, PRO, false>(A, blockIdx.x, sm);
+  st_tile<TS, H, MODE, PRO, false>(A, xcd_band_tile((int)blockIdx.x, (int)gridDim.x), sm);
 }
 
 // x += alpha p, r -= alpha q with alpha = p.r / p.q, the partials of r.r,

@@ -139,6 +139,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
   const unsigned me = blockIdx.x, nwg = gridDim.x;
   const KGrid& G = A.G;
   const int ntx = tiles_x<C>(G), ntiles = tiles_total<C>(G);
+  const TileBand band = xcd_tile_band(me, nwg, ntiles);
   const int tx = threadIdx.x % C::TX, ty = threadIdx.x / C::TX;
   const bool node_thread = threadIdx.x < C::TX * C::TY;
   unsigned epoch = 0;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
 
   // ---------------- init: r = (float) r64, e = 0, z = M^-1 r, r.z, r.r
   double acc[2] = {0, 0};
-  for (int tile = me; tile < ntiles; tile += nwg) {
+  for (int tile = band.first; tile < band.last; tile += band.step) {
     const int gi = (tile % ntx) * C::TX + tx, gj = (tile / ntx) * C::TY + ty;
     if (node_thread && gi <= G.nx && gj <= G.ny) {
       const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
@@ -233,8 +234,8 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
       // ---------------- phase A: p = z + beta pold, Ap = K~ p, p.Ap
       double pap[1] = {0};
       const float* po = fresh ? nullptr : pold;
-      if (me < (unsigned)ntiles) stage_load(me, po);
-      for (int tile = me; tile < ntiles; tile += nwg) {
+      if (band.first < band.last) stage_load(band.first, po);
+      for (int tile = band.first; tile < band.last; tile += band.step) {
         const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
         float s1v[4], s3v[4];
         elem_scales<C>(G, i0, j0, A.uniform != 0, s1v, s3v);
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
         const int64_t nm = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
         const unsigned fixm = mine ? G.cmask[nm] : 0u;
         stage_store(po != nullptr, (float)beta);
-        if (tile + (int)nwg < ntiles) stage_load(tile + nwg, po);
+        if (tile + band.step < band.last) stage_load(tile + band.step, po);
         __syncthreads();
         tile_elements_mfma<C>(G, i0, j0, p_s, ye_s, ab, as, A.uniform != 0, s1v, s3v);
         __syncthreads();
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
       const float alpha = (float)(rz / pap[0]);
       // ---------------- phase B: e += alpha p, r -= alpha Ap, z = M^-1 r
       acc[0] = 0; acc[1] = 0;
-      for (int tile = me; tile < ntiles; tile += nwg) {
+      for (int tile = band.first; tile < band.last; tile += band.step) {
         const int gi = (tile % ntx) * C::TX + tx, gj = (tile / ntx) * C::TY + ty;
         if (node_thread && gi <= G.nx && gj <= G.ny) {
           const int64_t n = (int64_t)gj * (G.nx + 1) + gi;

@@ -249,6 +249,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
   // touched by their range's owner.
   const int64_t nn = (int64_t)(G.nx + 1) * (G.ny + 1);
   const int64_t lo = nn * me / nwg, hi = nn * (me + 1) / nwg;
+  const TileBand band = xcd_tile_band(me, nwg, ntiles);
 
   // ---------------- init: r = F (or r0 on a restart), z = M^-1 r, r.z, r.r
   const bool restart = A.r0 != nullptr;
@@ -322,8 +323,8 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
     // ---------------- phase A: p, Ap = K p, p.Ap
     if (timing) t0 = wall_clock64();
     double pap[1] = {0};
-    if (me < (unsigned)ntiles) stage_load(me, pold);
-    for (int tile = me; tile < ntiles; tile += nwg) {
+    if (band.first < band.last) stage_load(band.first, pold);
+    for (int tile = band.first; tile < band.last; tile += band.step) {
       const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
       // this tile's element data and node mask, then its staged block, then
       // the next tile's staging loads
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
       const int64_t n = mine ? (int64_t)gj * (G.nx + 1) + gi : 0;
       const unsigned fix = mine ? G.cmask[n] : 0u;
       stage_store(pold != nullptr, beta);
-      if (tile + (int)nwg < ntiles) stage_load(tile + nwg, pold);
+      if (tile + band.step < band.last) stage_load(tile + band.step, pold);
       __syncthreads();
       if (le < C::NE) {
         double y[12];
