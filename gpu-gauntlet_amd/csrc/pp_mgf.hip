@@ -158,17 +158,6 @@ __device__ __forceinline__ bool stop_rule(double rr, MgfState* S, MgfRec* rec, i
 
 enum { M_STEP = 0, M_RES = 1, M_KP = 2 };
 
-// Tile of workgroup b in a launch of G workgroups, one per tile.  Workgroup
-// b runs on XCD b % 8 (the dispatcher deals workgroups to the eight XCDs in
-// turn), so XCD x is given the contiguous row-major tiles
-// [x (G / 8) + min(x, G % 8), ...): horizontally adjacent tiles, and most
-// vertically adjacent ones, then share one L2 for their halo rows and for the
-// partial 128-byte lines at the ends of every staged row.  A bijection of
-// [0, G), so per-workgroup partials keep their slots.
-__device__ __forceinline__ int xcd_band_tile(int b, int G) {
-  const int x = b & 7, q = G >> 3, r = G & 7;
-  return x * q + min(x, r) + (b >> 3);
-}
 
 // stencil-level tile sides: steps 16 x 16 nodes, residuals 14 x 14 (a
 // 2-node halo keeps the computed nodes at 16 x 16: one per thread)

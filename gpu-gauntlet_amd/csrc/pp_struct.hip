@@ -55,7 +55,8 @@ __global__ __launch_bounds__(CfgS::NT) void k_apply(KGrid G, const double* x, do
   using C = CfgS;
   __shared__ double p_s[3 * C::NP];
   __shared__ double ye_s[12 * C::NE];
-  const int tix = blockIdx.x % tiles_x<C>(G), tiy = blockIdx.x / tiles_x<C>(G);
+  const int tile = xcd_band_tile((int)blockIdx.x, (int)gridDim.x);
+  const int tix = tile % tiles_x<C>(G), tiy = tile / tiles_x<C>(G);
   const int i0 = tix * C::TX, j0 = tiy * C::TY;
   stage_tile<C, double, double>(G, i0, j0, x, nullptr, 0.0, p_s);
   __syncthreads();
@@ -88,7 +89,8 @@ __global__ __launch_bounds__(CfgS::NT) void k_apply_sweep(KGrid G, const double*
   using C = CfgS;
   __shared__ double p_s[3 * C::NP];
   __shared__ double ye_s[12 * C::NE];
-  const int tix = blockIdx.x % tiles_x<C>(G), tiy = blockIdx.x / tiles_x<C>(G);
+  const int tile = xcd_band_tile((int)blockIdx.x, (int)gridDim.x);
+  const int tix = tile % tiles_x<C>(G), tiy = tile / tiles_x<C>(G);
   const int i0 = tix * C::TX, j0 = tiy * C::TY;
   stage_tile<C, double, double>(G, i0, j0, x, nullptr, 0.0, p_s);
   __syncthreads();
