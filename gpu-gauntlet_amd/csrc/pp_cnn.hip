@@ -257,7 +257,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
         for (int i = 0; i < 4; i++)
           if (!(q[i] > 0.0f)) v[i] = 0.0f;
       }
-      if (co + 4 <= A.cout) {
+      // a float output with room for four channels takes the zero pad channels too
+      if (co + 4 <= A.cout || (A.out32 && co + 4 <= A.out32_cs)) {
         if (A.out32) *(float4*)&A.out32[op * A.out32_cs + co] = make_float4(v[0], v[1], v[2], v[3]);
         else st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
       } else {
@@ -391,7 +392,8 @@ __global__ __launch_bounds__(256) void k_convf(ConvArgs A, int ntiles) {
         v[i] = acc[j][nb][i] + (A.bias && co + i < A.cout ? A.bias[co + i] : 0.0f);
         if (A.relu) v[i] = fmaxf(v[i], 0.0f);
       }
-      if (co + 4 <= A.cout) {
+      // a float output with room for four channels takes the zero pad channels too
+      if (co + 4 <= A.cout || (A.out32 && co + 4 <= A.out32_cs)) {
         if (A.out32) *(float4*)&A.out32[op * A.out32_cs + co] = make_float4(v[0], v[1], v[2], v[3]);
         else st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
       } else {
@@ -534,7 +536,8 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
           v[i] = acc[j][nb][i] + (A.bias && co + i < A.cout ? A.bias[co + i] : 0.0f);
           if (A.relu) v[i] = fmaxf(v[i], 0.0f);
         }
-        if (co + 4 <= A.cout) {
+        // a float output with room for four channels takes the zero pad channels too
+      if (co + 4 <= A.cout || (A.out32 && co + 4 <= A.out32_cs)) {
           if (A.out32) *(float4*)&A.out32[op * A.out32_cs + co] = make_float4(v[0], v[1], v[2], v[3]);
           else st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
         } else {
