@@ -68,6 +68,13 @@ struct Mg {
   int estimate_omega(int l);
 };
 
+// One multigrid V-cycle x = V b with the fused kernels of pp_mgf.hip, for use
+// outside PCG (the learned preconditioner's smoother); the hierarchy is mg's.
+struct MgfCycle;
+int mgf_cycle_create(pp_plan* pl, Mg* mg, MgfCycle** out);
+int mgf_cycle_apply(MgfCycle* c, const double* b, double* x);  // b and x must not alias
+void mgf_cycle_destroy(MgfCycle* c);
+
 struct Cnn {
   pp_plan* pl = nullptr;
   hipStream_t stream = nullptr;
@@ -85,6 +92,7 @@ struct Cnn {
   int line_search = 1;  // energy-optimal step on the network's correction (else unit step)
   int fcg_window = 8;   // directions kept by flexible PCG (FCG(m))
   Mg* mg = nullptr;     // V-cycle smoother in place of w B (null: w B)
+  MgfCycle* mgc = nullptr;  // the same V-cycle on the fused kernels
 
   DevBuf<float> params, grads, adam_m, adam_v;
   DevBuf<uint16_t> wf, wt;

@@ -1377,7 +1377,7 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   if (cfg && cfg->multigrid) {
     mg = new Mg();
     if ((rc = mg->setup(pl, omega, cfg->mg_sweeps, cfg->mg_coarse_sweeps, cfg->mg_jacobi != 0)) ||
-        (rc = mgz.alloc(pl->n_dof)))
+        (rc = mgz.alloc(pl->n_dof)) || (rc = mgf_cycle_create(pl, mg, &mgc)))
       return rc;
   }
   PP_HIP(hipStreamSynchronize(stream));
@@ -1490,9 +1490,9 @@ int Cnn::smooth(const double* in, const double* base, double* out) {
     PP_HIP(hipGetLastError());
     return PP_OK;
   }
-  if (!base) return mg->vcycle(in, out);
+  if (!base) return mgf_cycle_apply(mgc, in, out);
   int rc;
-  if ((rc = mg->vcycle(in, mgz.p))) return rc;
+  if ((rc = mgf_cycle_apply(mgc, in, mgz.p))) return rc;
   const int64_t nd = pl->n_dof;
   hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, base, 1.0,
                      (const double*)mgz.p, out);
@@ -1649,6 +1649,7 @@ int Cnn::adam_step(float lr, float max_norm) {
 }
 
 Cnn::~Cnn() {
+  mgf_cycle_destroy(mgc);
   delete mg;
   mg = nullptr;
   if (!pl) return;

@@ -503,7 +503,7 @@ int pp_cnn_mg_cycle(pp_cnn* c, const double* b, double* x) {
   pp_plan* pl = n.pl;
   PP_HIP(hipSetDevice(pl->ctx->device));
   PP_HIP(hipMemcpyAsync(n.evec.p, b, pl->n_dof * 8, hipMemcpyHostToDevice, n.stream));
-  int rc = n.mg->vcycle(n.evec.p, n.rvec.p);
+  int rc = mgf_cycle_apply(n.mgc, n.evec.p, n.rvec.p);  // the preconditioner's own cycle
   if (rc) return rc;
   PP_HIP(hipMemcpyAsync(x, n.rvec.p, pl->n_dof * 8, hipMemcpyDeviceToHost, n.stream));
   PP_HIP(hipStreamSynchronize(n.stream));

@@ -1111,6 +1111,50 @@ int max_workgroups(const pp_plan* pl) {
 
 }  // namespace
 
+// ------------------------------------------------ V-cycle outside PCG
+
+struct MgfCycle {
+  Fused F;
+  DevBuf<MgfState> S;  // never stops: only the V-cycle kernels read it
+  DevBuf<double> part;
+};
+
+int mgf_cycle_create(pp_plan* pl, Mg* mg, MgfCycle** out) {
+  *out = nullptr;
+  MgfCycle* c = new MgfCycle();
+  int rc;
+  if ((rc = c->S.alloc(1)) || (rc = c->part.alloc(4)) || (rc = c->F.setup(pl, mg, c->S.p, c->part.p, 1))) {
+    delete c;
+    return rc;
+  }
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, pl->ctx->stream, c->S.p, 1.0, 0.0);
+  if (hipGetLastError() != hipSuccess) {
+    delete c;
+    return fail(PP_ERR_HIP, "multigrid: state initialisation failed");
+  }
+  *out = c;
+  return PP_OK;
+}
+
+// x = V b: the pointwise first step from x = 0, then the fused cycle; level 0
+// reads b and leaves its result in x (the cycle's buffers are chosen so that
+// it ends in buf[0]), using the hierarchy's own level-0 x2 and t as the
+// other two iterates.
+int mgf_cycle_apply(MgfCycle* c, const double* b, double* x) {
+  Fused& F = c->F;
+  Fused::Lv& V = F.L[0];
+  if (b == x || x == V.buf[1] || x == V.buf[2] || b == V.buf[1] || b == V.buf[2])
+    return fail(PP_ERR_INVALID, "multigrid: V-cycle vectors alias");
+  V.b = const_cast<double*>(b);
+  V.buf[0] = x;
+  int rc = F.first_cycle();
+  if (rc) return rc;
+  if (V.out != x) return fail(PP_ERR_INVALID, "multigrid: V-cycle did not end in its output");
+  return PP_OK;
+}
+
+void mgf_cycle_destroy(MgfCycle* c) { delete c; }
+
 constexpr int kRecRing = 8;
 
 // FCG(1) with the fused V-cycle from x (zero on a cold start) with r = F - K x
