@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--train-loads", type=int, default=4,
                     help="c5: load cases for the solver-generated training pairs (8 pairs each)")
     ap.add_argument("--train-epochs", type=int, default=1, help="c5: training epochs")
+    ap.add_argument("--train-source", default="solver", choices=["solver", "learned"],
+                    help="c5: training pairs from block-Jacobi PCG iterates or from flexible "
+                         "PCG with the network's current preconditioner")
     ap.add_argument("--cpu-sample-iters", type=int, default=-1,
                     help="iterations of the CPU baseline sample (0 = skip; default 1200, "
                          "150 for c3)")
@@ -381,26 +384,38 @@ def timed_leg(plan, o, warmup, steps):
             "converged": bool(st.converged), "sweeps": st.sweeps}
 
 
-def learned_legs(plan, F, train_loads, epochs, warmup, steps, x_ref, seed=5):
+def learned_legs(plan, F, train_loads, epochs, warmup, steps, x_ref, seed=5, source="solver"):
     """The learned preconditioner (3-level U-net with a multigrid V-cycle
-    around it, flexible PCG) trained first on solver-generated pairs (block-
-    Jacobi PCG iterates of independent load cases, 8 per case; not timed), then
-    timed; and the same preconditioner with every network parameter zero."""
+    around it, flexible PCG) trained first on solver-generated pairs (8 per
+    independent load case, not timed): iterates of block-Jacobi PCG
+    (source "solver") or of flexible PCG with the network's current
+    preconditioner (source "learned"); then timed, and the same
+    preconditioner with every network parameter zero."""
+    import math
+
     import numpy as np
 
     from platepcg import Net, _lib
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as O
     net = Net(plan, seed=seed, multigrid=True)
+    src = {"solver": _lib.TRAIN_SOLVER, "learned": _lib.TRAIN_LEARNED}[source]
     t0 = time.perf_counter()
     first = last = None
-    for k in range(train_loads):  # one load case per call, so progress stays visible
-        tr = net.train(pairs=1, snapshots=8, epochs=epochs, seed0=1000 + k, lr=1e-3, lr_final=1e-5,
-                       max_grad_norm=1.0, source=_lib.TRAIN_SOLVER)
+    # a few load cases per call, so progress stays visible; each call runs one
+    # piece of a single cosine learning-rate decay 1e-3 -> 1e-5 over all calls
+    chunk = 1 if source == "solver" else 4
+    calls = (train_loads + chunk - 1) // chunk
+    lr_at = lambda f: 1e-5 + 0.5 * (1e-3 - 1e-5) * (1.0 + math.cos(math.pi * f))
+    for c in range(calls):
+        k0, k1 = c * chunk, min(train_loads, (c + 1) * chunk)
+        tr = net.train(pairs=k1 - k0, snapshots=8, epochs=epochs, seed0=1000 + k0,
+                       lr=lr_at(c / calls), lr_final=lr_at((c + 1) / calls), max_grad_norm=1.0,
+                       source=src)
         first = tr["loss_first"] if first is None else first
         last = tr["loss_last"]
-        if (k + 1) % 8 == 0 or k + 1 == train_loads:
-            print(f"training {plan.n_dof} DOF: {k + 1}/{train_loads} load cases, loss "
+        if k1 % 8 == 0 or k1 == train_loads:
+            print(f"training {plan.n_dof} DOF: {k1}/{train_loads} load cases, loss "
                   f"{tr['loss_first']:.3f} -> {tr['loss_last']:.3f}, "
                   f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     train_s = time.perf_counter() - t0
@@ -422,8 +437,10 @@ def learned_legs(plan, F, train_loads, epochs, warmup, steps, x_ref, seed=5):
     out["network_iteration_gain"] = (out["zero_network"]["iterations"] /
                                      max(1, out["learned"]["iterations"]) if ok else None)
     out["network_time_gain"] = (out["zero_network"]["ms"] / out["learned"]["ms"] if ok else None)
-    out["training"] = {"source": "block-Jacobi PCG iterates (x* - x_k) of independent seeded "
-                                 "pressure loads", "pairs": train_loads * 8, "epochs": epochs,
+    out["training"] = {"source": ("block-Jacobi PCG" if source == "solver" else
+                                  "flexible PCG (the network's current preconditioner)") +
+                                 " iterates (x* - x_k) of independent seeded pressure loads",
+                       "pairs": train_loads * 8, "epochs": epochs,
                        "steps": train_loads * 8 * epochs, "seconds": train_s,
                        "loss_first": first, "loss_last": last}
     return out
@@ -462,7 +479,8 @@ def c5_legs(a):
     res["multigrid"] = mg
     print(f"multigrid 3.15M DOF: {mg['iterations']} iterations, {mg['ms']:.0f} ms",
           file=sys.stderr, flush=True)
-    res.update(learned_legs(plan, F, a.train_loads, a.train_epochs, 1, 3, x_bj))
+    res.update(learned_legs(plan, F, a.train_loads, a.train_epochs, 1, 3, x_bj,
+                            source=a.train_source))
     if res["learned"]["converged"]:
         res["speedup_learned_vs_block_jacobi"] = bj["ms"] / res["learned"]["ms"]
     res["speedup_multigrid_vs_block_jacobi"] = bj["ms"] / mg["ms"]

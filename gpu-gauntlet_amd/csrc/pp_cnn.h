@@ -100,7 +100,7 @@ struct Cnn {
   DevBuf<uint16_t> g_c0b, g_c0, g_cat0, g_d1, g_c1, g_cat1, g_bb, g_a2, g_r2, g_a1, g_r1, g_a0,
       g_out;
   DevBuf<float> out32, wpart, bpart, logt;
-  DevBuf<double> zbj, mout, dvec, kvec, evec, rvec, scal, dotp, bumps, z0v, z1v, r1v, mgz;
+  DevBuf<double> zbj, mout, dvec, kvec, evec, rvec, scal, dotp, bumps, z0v, z1v, r1v, mgz, kbv;
 
   const TV* mk(const TV& v) const { return linear ? nullptr : &v; }  // ReLU mask or none
   int setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed);

@@ -1040,13 +1040,61 @@ __global__ void k_axpy_dev(int64_t n, const double* a, const double* g, double s
   if (i < n) out[i] = a[i] + sign * g[0] * b[i];
 }
 
-// g = u.r1 / u.K u over the partials (0 when u.K u <= 0): the energy-optimal
-// length of the network's correction
-__global__ void k_gamma(const double* part, int np, double* g) {
-  const double a = block_total(part, np, 2);
+// Partials of b.Kb, b.Ku, u.Ku, b.r, u.r, five per block in a fixed order.
+__global__ __launch_bounds__(256) void k_dot5(int64_t n, const double* b, const double* kb,
+                                              const double* u, const double* ku, const double* r,
+                                              double* part) {
+  __shared__ double red[5][256];
+  double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double bi = b[i], ui = u[i], ri = r[i];
+    s[0] += bi * kb[i];
+    s[1] += bi * ku[i];
+    s[2] += ui * ku[i];
+    s[3] += bi * ri;
+    s[4] += ui * ri;
+  }
+  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = s[k];
   __syncthreads();
-  const double b = block_total(part + 1, np, 2);
-  if (threadIdx.x == 0) g[0] = b > 0.0 ? a / b : 0.0;
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 5; k++) part[5 * (size_t)blockIdx.x + k] = red[k][0];
+}
+
+// (g0, g1) minimising the energy error of z0 + g0 b + g1 u (b = B r1, the
+// block-Jacobi part of the network's output u): the 2 x 2 normal equations
+// [b.Kb b.Ku; b.Ku u.Ku] g = [b.r1; u.r1].  When b and u are parallel in
+// the energy inner product (1 - cos^2 < 1e-10, as for a zero network, where
+// u = b) the step is along u alone, g1 = u.r1 / u.Ku.
+__global__ void k_gamma2(const double* part, int np, double* g) {
+  double a[5];
+  for (int k = 0; k < 5; k++) {
+    a[k] = block_total(part + k, np, 5);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double bb = a[0], bu = a[1], uu = a[2], br = a[3], ur = a[4];
+    const double det = bb * uu - bu * bu;
+    if (bb > 0.0 && uu > 0.0 && det > 1e-10 * bb * uu) {
+      g[0] = (br * uu - ur * bu) / det;
+      g[1] = (bb * ur - bu * br) / det;
+    } else {
+      g[0] = 0.0;
+      g[1] = uu > 0.0 ? ur / uu : 0.0;
+    }
+  }
+}
+
+// out = a + sign (g[0] b + g[1] c) (g on the device)
+__global__ void k_axpy2_dev(int64_t n, const double* a, const double* g, double sign,
+                            const double* b, const double* c, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + sign * (g[0] * b[i] + g[1] * c[i]);
 }
 
 // v /= |v| (host-provided norm), for the power iteration
@@ -1326,7 +1374,8 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
       (rc = dvec.alloc(pl->n_dof)) || (rc = kvec.alloc(pl->n_dof)) ||
       (rc = evec.alloc(pl->n_dof)) || (rc = rvec.alloc(pl->n_dof)) ||
       (rc = logt.alloc(pl->n_nodes)) || (rc = scal.alloc(16)) ||
-      (rc = dotp.alloc(2 * kDotBlocks)) || (rc = bumps.alloc(4 * 64)))
+      (rc = dotp.alloc(5 * kDotBlocks)) || (rc = bumps.alloc(4 * 64)) ||
+      (rc = kbv.alloc(pl->n_dof)))
     return rc;
   // weight-gradient partials: the largest layer x its chunk count
   int64_t wp = 0, bp = 0;
@@ -1518,10 +1567,20 @@ int Cnn::precondition(const double* r, double* out, bool line_search) {
   if ((rc = struct_apply_k(pl, mout.p, kvec.p))) return rc;  // K u
   double* g = scal.p + 10;
   if (line_search) {
-    hipLaunchKernelGGL(k_dot2, dim3(kDotBlocks), dim3(256), 0, stream, nd, (const double*)mout.p,
-                       (const double*)r1v.p, (const double*)mout.p, (const double*)kvec.p,
-                       dotp.p);
-    hipLaunchKernelGGL(k_gamma, dim3(1), dim3(256), 0, stream, (const double*)dotp.p, kDotBlocks, g);
+    // two directions, the network's block-Jacobi part b = zbj and its whole
+    // output u, each with its own energy-optimal length: an application is
+    // then never worse than the same preconditioner with a zero network
+    if ((rc = struct_apply_k(pl, zbj.p, kbv.p))) return rc;  // K b
+    hipLaunchKernelGGL(k_dot5, dim3(kDotBlocks), dim3(256), 0, stream, nd, (const double*)zbj.p,
+                       (const double*)kbv.p, (const double*)mout.p, (const double*)kvec.p,
+                       (const double*)r1v.p, dotp.p);
+    hipLaunchKernelGGL(k_gamma2, dim3(1), dim3(256), 0, stream, (const double*)dotp.p, kDotBlocks, g);
+    hipLaunchKernelGGL(k_axpy2_dev, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)z0v.p,
+                       (const double*)g, 1.0, (const double*)zbj.p, (const double*)mout.p, z1v.p);
+    hipLaunchKernelGGL(k_axpy2_dev, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)r1v.p,
+                       (const double*)g, -1.0, (const double*)kbv.p, (const double*)kvec.p, r1v.p);
+    PP_HIP(hipGetLastError());
+    return smooth(r1v.p, z1v.p, out);
   } else {
     hipLaunchKernelGGL(k_lin, dim3(1), dim3(1), 0, stream, (int64_t)1, (const double*)scal.p + 14,
                        1.0, (const double*)scal.p + 13, g);  // g = 0 + 1

@@ -462,6 +462,20 @@ int pp_cnn_apply(pp_cnn* c, const double* r, double* z) {
   return PP_OK;
 }
 
+int pp_cnn_forward(pp_cnn* c, const double* v, double* u) {
+  if (!c || !v || !u) return fail(PP_ERR_INVALID, "NULL argument");
+  PP_LOCK(c->net.pl->ctx);
+  Cnn& n = c->net;
+  pp_plan* pl = n.pl;
+  PP_HIP(hipSetDevice(pl->ctx->device));
+  PP_HIP(hipMemcpyAsync(n.evec.p, v, pl->n_dof * 8, hipMemcpyHostToDevice, n.stream));
+  int rc = n.apply(n.evec.p, n.rvec.p);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(u, n.rvec.p, pl->n_dof * 8, hipMemcpyDeviceToHost, n.stream));
+  PP_HIP(hipStreamSynchronize(n.stream));
+  return PP_OK;
+}
+
 int pp_cnn_mg_levels(const pp_cnn* c, int32_t* nx, int32_t* ny, double* omega, int32_t cap) {
   if (!c) return fail(PP_ERR_INVALID, "NULL argument");
   const Mg* mg = c->net.mg;

@@ -133,6 +133,7 @@ SIGNATURES = {
     "pp_cnn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_cnn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pp_cnn_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pp_cnn_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pp_cnn_mg_levels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     "pp_cnn_mg_apply": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "pp_cnn_mg_cycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

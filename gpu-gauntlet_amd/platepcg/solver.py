@@ -227,6 +227,13 @@ class Net:
         check(self.lib.pp_cnn_apply(self.h, _ptr(r), _ptr(z)))
         return z
 
+    def forward(self, v):
+        """The network alone: N(v) = B v + its learned correction."""
+        v = np.ascontiguousarray(v, np.float64)
+        u = np.empty_like(v)
+        check(self.lib.pp_cnn_forward(self.h, _ptr(v), _ptr(u)))
+        return u
+
     def mg_levels(self):
         """[(nx, ny, omega)] per multigrid level ([] without multigrid)."""
         nx, ny = (C.c_int32 * 16)(), (C.c_int32 * 16)()

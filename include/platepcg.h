@@ -343,6 +343,10 @@ double pp_cnn_smoother_weight(const pp_cnn* net);
  * (w B, B the 3x3 block-Jacobi inverse, or with multigrid = 1 one V-cycle),
  * N the network (N(v) = B v + its learned correction). */
 int pp_cnn_apply(pp_cnn* net, const double* r, double* z);
+/* The network alone: u = N(v) = B v + its learned correction (host vectors
+ * of 3 * node_count doubles); for parity tests and for building other
+ * preconditioners around it.  No reference counterpart. */
+int pp_cnn_forward(pp_cnn* net, const double* v, double* u);
 /* Multigrid smoother diagnostics (config multigrid = 1; no reference
  * counterpart).  Levels: returns the level count (0 without multigrid) and
  * fills up to cap entries of the element counts per side and the smoothing

@@ -80,8 +80,14 @@ def test_forward_matches_oracle(ctx, n, relu):
     corr_gpu, corr_ref = m_gpu - m_bj, m_ref - m_bj
     err = np.max(np.abs(corr_gpu - corr_ref)) / np.max(np.abs(corr_ref))
     assert err < 2e-2, err
+    # the network alone (pp_cnn_forward): N(v) = B v + the learned correction
+    bv = np.einsum("nij,nj->ni", ref.blk, r.reshape(-1, 3)).reshape(-1)
+    u_gpu, u_ref = net.forward(r), ref.forward(p, r)
+    err = np.max(np.abs((u_gpu - bv) - (u_ref - bv))) / np.max(np.abs(u_ref - bv))
+    assert err < 2e-2, err
     net.set_params(zero)
     np.testing.assert_allclose(net.apply(r), m_bj, rtol=1e-9, atol=1e-12 * np.abs(m_bj).max())
+    np.testing.assert_allclose(net.forward(r), bv, rtol=1e-12, atol=1e-14 * np.abs(bv).max())
     net.close()
 
 
