@@ -97,6 +97,10 @@ struct Cnn {
   DevBuf<float> params, grads, adam_m, adam_v;
   DevBuf<uint16_t> wf, wt;
   DevBuf<uint16_t> xin, a0, cat0, r1, a1, cat1, r2, a2, bb, c1b, d1, c0, c0b;
+  // the skip and prolonged tensors stored alone, for the forward pass without
+  // backward (the decoder's first layers read their two input parts in place)
+  DevBuf<uint16_t> s0a, s1a, p0a, p1a;
+  bool split_fwd = false;  // the network's shapes allow that forward pass
   DevBuf<uint16_t> g_c0b, g_c0, g_cat0, g_d1, g_c1, g_cat1, g_bb, g_a2, g_r2, g_a1, g_r1, g_a0,
       g_out;
   DevBuf<float> out32, wpart, bpart, logt;
@@ -105,8 +109,10 @@ struct Cnn {
   const TV* mk(const TV& v) const { return linear ? nullptr : &v; }  // ReLU mask or none
   int setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed);
   int pack_weights();
-  int apply(const double* r, double* out);         // network alone, device vectors
-  int precondition(const double* r, double* out, bool line_search);  // smoothing + network + smoothing
+  // network alone, device vectors; keep = the activations the backward pass reads
+  int apply(const double* r, double* out, bool keep = false);
+  // smoothing + network + smoothing
+  int precondition(const double* r, double* out, bool line_search, bool keep = false);
   int smooth(const double* in, const double* base, double* out);  // out = base + S in (base may be null)
   int estimate_omega();
   int backward();
@@ -114,7 +120,10 @@ struct Cnn {
   int gen_error(uint64_t seed, double* e);
   int adam_step(float lr, float max_norm);
   int conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int out32_cs, int lev,
-               bool backward, const TV* mask = nullptr, int mask_from = 0);
+               bool backward, const TV* mask = nullptr, int mask_from = 0,
+               const TV* first = nullptr, int first_ch = 0);
+  int convp_pick(int cin, int nb) const;   // k_convp shape index or -1
+  size_t convp_lds(int cin, int k) const;  // its dynamic LDS bytes
   int conv_wgrad(const Layer& L, const TV& in, const TV& dout, int lev);
   int resample(bool restrict_op, const TV& in, int lin, const TV& out, int lout, int C,
                float scale, const TV* mask, bool accumulate);

@@ -90,7 +90,29 @@ struct ConvArgs {
   int H, W, dil, relu;
   const uint16_t* mask;  // out[c] *= (mask[pixel][c] > 0) for c >= mask_from
   int mask_cs, mask_c0, mask_from;
+  // two-part input (k_convp): channels [0, p_ch) come from pin (channel stride
+  // p_cs, offset p_c0), channels [p_ch, cin) from in at in_c0 + c - p_ch; so a
+  // concatenation of two tensors stored apart is read in place
+  const uint16_t* pin;
+  int p_cs, p_c0, p_ch;
 };
+
+// ((a + b) + (c + d)) / 4 per BF16 channel, in FP32, rounded to BF16: the
+// bilinear value of a fine node from its 1, 2 or 4 coarse neighbours, a
+// missing neighbour repeating the one beside it (k_prolong and k_prolong8 sum
+// in this order, so both give the same bits).
+__device__ __forceinline__ uint32_t avg4_bf2(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  const float lo = ((__uint_as_float(a << 16) + __uint_as_float(b << 16)) +
+                    (__uint_as_float(c << 16) + __uint_as_float(d << 16))) * 0.25f;
+  const float hi = ((__uint_as_float(a & 0xffff0000u) + __uint_as_float(b & 0xffff0000u)) +
+                    (__uint_as_float(c & 0xffff0000u) + __uint_as_float(d & 0xffff0000u))) * 0.25f;
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+__device__ __forceinline__ uint4 avg4_bf8(const uint4& a, const uint4& b, const uint4& c,
+                                          const uint4& d) {
+  return uint4{avg4_bf2(a.x, b.x, c.x, d.x), avg4_bf2(a.y, b.y, c.y, d.y),
+               avg4_bf2(a.z, b.z, c.z, d.z), avg4_bf2(a.w, b.w, c.w, d.w)};
+}
 
 // One workgroup per output tile of MB rows x 64 pixels (MB = 2 or 4).  The input tile
 // with its halo, (MB + 2 dil) x (64 + 2 dil) pixels x cin channels, is
@@ -412,7 +434,8 @@ __global__ __launch_bounds__(256) void k_convf(ConvArgs A, int ntiles) {
 // the workgroups on that XCD) chunk by chunk (CC input channels at a time),
 // loading the next (tile, chunk) input into registers while the current one
 // is multiplied, so the load latency of every stage but the first is hidden
-// behind the previous stage's work.
+// behind the previous stage's work.  Chunks below A.p_ch come from A.pin (a
+// concatenated input whose parts are stored apart).
 template <int CC, int NB, int MB>
 __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
   constexpr int TW = 66, TH = MB + 2, CS = CC + 4;
@@ -464,6 +487,9 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
   uint4 vi[UIN];
   auto load_stage = [&](int tile, int c0) {
     const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
+    const bool first = c0 < A.p_ch;
+    const uint16_t* src = first ? A.pin + A.p_c0 + c0 : A.in + A.in_c0 + (c0 - A.p_ch);
+    const int cs = first ? A.p_cs : A.in_cs;
 #pragma unroll
     for (int u = 0; u < UIN; u++) {
       const int i = threadIdx.x + 256 * u;
@@ -472,7 +498,7 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
         const int pix = i / Q8, part = i % Q8;
         const int gx = tx0 - 1 + pix % TW, gy = ty0 - 1 + pix / TW;
         if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
-          vi[u] = *(const uint4*)&A.in[(gy * A.W + gx) * A.in_cs + A.in_c0 + c0 + 8 * part];
+          vi[u] = *(const uint4*)&src[(gy * A.W + gx) * cs + 8 * part];
       }
     }
   };
@@ -739,6 +765,60 @@ __global__ void k_restrict(const uint16_t* in, int in_cs, int in_c0, int Hf, int
                  mask ? &mask[(size_t)pix * mask_cs + mask_c0 + c] : nullptr, accumulate);
 }
 
+// Full-weighting restriction of a tensor stored alone (channel stride C, a
+// multiple of 8), eight channels per thread in 16-byte accesses, for the
+// network's forward pass: the sums of k_restrict with scale 1, in the same
+// order, so the same bits.
+__global__ void k_restrict8(const uint16_t* in, int Hf, int Wf, uint16_t* out, int Hc, int Wc,
+                            int C) {
+  const int C8 = C >> 3;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Hc * Wc * C8) return;
+  const int c = (i % C8) * 8, pix = i / C8;
+  const int X = pix % Wc, Y = pix / Wc;
+  float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (int b = -1; b <= 1; b++) {
+    const int fy = 2 * Y + b;
+    if (fy < 0 || fy >= Hf) continue;
+    const float wy = b == 0 ? 2.0f : 1.0f;
+    for (int a = -1; a <= 1; a++) {
+      const int fx = 2 * X + a;
+      if (fx < 0 || fx >= Wf) continue;
+      const float w = wy * (a == 0 ? 2.0f : 1.0f);
+      const uint4 u = *(const uint4*)&in[((size_t)fy * Wf + fx) * C + c];
+      const uint32_t q[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        s[2 * k] += w * __uint_as_float(q[k] << 16);
+        s[2 * k + 1] += w * __uint_as_float(q[k] & 0xffff0000u);
+      }
+    }
+  }
+  uint32_t o[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    o[k] = (uint32_t)f2bf(0.0625f * s[2 * k]) | ((uint32_t)f2bf(0.0625f * s[2 * k + 1]) << 16);
+  *(uint4*)&out[(size_t)pix * C + c] = uint4{o[0], o[1], o[2], o[3]};
+}
+
+// Bilinear prolongation into a tensor stored alone (channel stride C, a
+// multiple of 8) for the network's forward pass: eight channels per thread in
+// 16-byte accesses, the value k_prolong gives with scale 1.
+__global__ void k_prolong8(const uint16_t* in, int Hc, int Wc, uint16_t* out, int Hf, int Wf,
+                           int C) {
+  const int C8 = C >> 3;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Hf * Wf * C8) return;
+  const int c = (i % C8) * 8, pix = i / C8;
+  const int x = pix % Wf, y = pix / Wf;
+  const int x0 = x >> 1, y0 = y >> 1, x1 = x0 + (x & 1), y1 = y0 + (y & 1);
+  const uint4 a = *(const uint4*)&in[((size_t)y0 * Wc + x0) * C + c];
+  const uint4 b = *(const uint4*)&in[((size_t)y0 * Wc + x1) * C + c];
+  const uint4 d = *(const uint4*)&in[((size_t)y1 * Wc + x0) * C + c];
+  const uint4 e = *(const uint4*)&in[((size_t)y1 * Wc + x1) * C + c];
+  *(uint4*)&out[(size_t)pix * C + c] = avg4_bf8(a, b, d, e);
+}
+
 // out[x,y][c] = scale * bilinear(in) (vertex-centred, Hf = 2 Hc - 1), four
 // channels per thread.
 __global__ void k_prolong(const uint16_t* in, int in_cs, int in_c0, int Hc, int Wc, uint16_t* out,
@@ -750,18 +830,15 @@ __global__ void k_prolong(const uint16_t* in, int in_cs, int in_c0, int Hc, int 
   const int c = (i % C4) * 4, pix = i / C4;
   const int x = pix % Wf, y = pix / Wf;
   const int x0 = x >> 1, y0 = y >> 1;
-  const int x1 = (x & 1) ? x0 + 1 : x0, y1 = (y & 1) ? y0 + 1 : y0;
-  const float w = ((x & 1) ? 0.5f : 1.0f) * ((y & 1) ? 0.5f : 1.0f);
-  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  auto add = [&](int X, int Y) {
-    float v[4];
-    ld4(&in[((size_t)Y * Wc + X) * in_cs + in_c0 + c], v);
-    for (int k = 0; k < 4; k++) s[k] += w * v[k];
-  };
-  add(x0, y0);
-  if (x & 1) add(x1, y0);
-  if (y & 1) add(x0, y1);
-  if ((x & 1) && (y & 1)) add(x1, y1);
+  const int x1 = x0 + (x & 1), y1 = y0 + (y & 1);
+  // ((v00 + v01) + (v10 + v11)) / 4 with a missing neighbour repeating the
+  // one beside it: the bilinear weights 1, 1/2, 1/4, summed as avg4_bf2 does
+  float v00[4], v01[4], v10[4], v11[4], s[4];
+  ld4(&in[((size_t)y0 * Wc + x0) * in_cs + in_c0 + c], v00);
+  ld4(&in[((size_t)y0 * Wc + x1) * in_cs + in_c0 + c], v01);
+  ld4(&in[((size_t)y1 * Wc + x0) * in_cs + in_c0 + c], v10);
+  ld4(&in[((size_t)y1 * Wc + x1) * in_cs + in_c0 + c], v11);
+  for (int k = 0; k < 4; k++) s[k] = ((v00[k] + v01[k]) + (v10[k] + v11[k])) * 0.25f;
   resample_store(s, scale, &out[(size_t)pix * out_cs + out_c0 + c],
                  mask ? &mask[(size_t)pix * mask_cs + mask_c0 + c] : nullptr, accumulate);
 }
@@ -1110,8 +1187,33 @@ constexpr int kRowsPerChunk = 8;
 
 // ------------------------------------------------------------- network
 
+// The persistent kernel's compile-time shapes (input channels per chunk,
+// 16-channel output blocks, rows per tile), in the order they are tried.
+static const int kConvpShapes[5][3] = {{16, 1, 4}, {16, 2, 4}, {32, 2, 4}, {32, 2, 2}, {32, 4, 2}};
+
+// The k_convp shape for cin input channels and nb output blocks: the first
+// whose weights and tile leave room for two workgroups per CU, else one; -1
+// if none fits.
+int Cnn::convp_pick(int cin, int nb) const {
+  if (cin % 16) return -1;
+  const int cc = cin % 32 == 0 ? 32 : 16;
+  for (size_t cap : {lds_max / 2, lds_max})
+    for (int k = 0; k < 5; k++) {
+      if (cc == kConvpShapes[k][0] && nb == kConvpShapes[k][1] && convp_lds(cin, k) <= cap)
+        return k;
+    }
+  return -1;
+}
+
+// the weights and the input tile
+size_t Cnn::convp_lds(int cin, int k) const {
+  const int C = kConvpShapes[k][0], N = kConvpShapes[k][1], R = kConvpShapes[k][2];
+  return ((size_t)N * 16 * (9 * cin + 4) + (size_t)(R + 2) * 66 * (C + 4)) * 2;
+}
+
 int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int out32_cs,
-                  int lev, bool backward, const TV* mask, int mask_from) {
+                  int lev, bool backward, const TV* mask, int mask_from, const TV* first,
+                  int first_ch) {
   ConvArgs A;
   memset(&A, 0, sizeof(A));
   const int cin = backward ? L.coutA : L.cin;
@@ -1128,8 +1230,18 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
   if (mask) {
     A.mask = mask->p; A.mask_cs = mask->cs; A.mask_c0 = mask->c0; A.mask_from = mask_from;
   }
-  const int64_t cs_max = std::max({(int64_t)A.in_cs, (int64_t)A.out_cs, (int64_t)A.out32_cs,
-                                    (int64_t)A.mask_cs});
+  int64_t cs_max = std::max({(int64_t)A.in_cs, (int64_t)A.out_cs, (int64_t)A.out32_cs,
+                             (int64_t)A.mask_cs});
+  if (first) {
+    // a two-part input: the persistent kernel only, whole chunks per part
+    const int shape = convp_pick(cin, nb);
+    if (backward || mask || A.dil != 1 || shape < 0 || first_ch <= 0 || first_ch >= cin ||
+        first_ch % kConvpShapes[shape][0] || first->cs % 8 || first->c0 % 8 ||
+        A.in_cs % 8 || A.in_c0 % 8)
+      return fail(PP_ERR_INVALID, "conv: no two-part input for this layer");
+    A.pin = first->p; A.p_cs = first->cs; A.p_c0 = first->c0; A.p_ch = first_ch;
+    cs_max = std::max(cs_max, (int64_t)first->cs);
+  }
   if ((int64_t)H[lev] * W[lev] * cs_max + cs_max >= (1LL << 31))
     return fail(PP_ERR_INVALID, "conv: tensor too large for 32-bit offsets");
   hipStream_t s = stream;
@@ -1140,32 +1252,27 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
     // the weights, the tile-per-workgroup kernel one chunk of them
     const int cc = cin % 32 == 0 ? 32 : 16;
     A.cc = cc;
-    {
-      auto tiles_p = [&](int mb) { return ((W[lev] + 63) / 64) * ((H[lev] + mb - 1) / mb); };
-      auto lds_p = [&](int nbk, int mb) {
-        return ((size_t)nbk * 16 * (9 * cin + 4) + (size_t)(mb + 2) * 66 * (cc + 4)) * 2;
-      };
+    const int shape = convp_pick(cin, nb);
+    if (shape >= 0) {
+      const int R = kConvpShapes[shape][2];
+      const int nt = ((W[lev] + 63) / 64) * ((H[lev] + R - 1) / R);
+      const size_t lds = convp_lds(cin, shape);
       // workgroups: what stays resident (kernel occupancy x CUs), whole XCD
       // groups, at most one per tile
-#define PP_CONVP(C, N, R)                                                                  \
-  if (cc == C && nb == N && lds_p(N, R) <= cap) {                                          \
-    const int nt = tiles_p(R);                                                             \
+#define PP_CONVP(K, C, N, R)                                                             \
+  if (shape == K) {                                                                        \
     int occ = 0;                                                                           \
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_convp<C, N, R>, 256,       \
-                                                        lds_p(N, R)));                     \
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_convp<C, N, R>, 256, lds)); \
     const int g = 8 * std::max(1, std::min((nt + 7) / 8, std::max(1, occ) * cu / 8));      \
-    hipLaunchKernelGGL((k_convp<C, N, R>), dim3(g), dim3(256), lds_p(N, R), s, A, nt);     \
+    hipLaunchKernelGGL((k_convp<C, N, R>), dim3(g), dim3(256), lds, s, A, nt);             \
     PP_HIP(hipGetLastError());                                                             \
     return PP_OK;                                                                          \
   }
-      // the first shape that leaves room for two workgroups per CU, else one
-      for (size_t cap : {lds_max / 2, lds_max}) {
-        PP_CONVP(16, 1, 4)
-        PP_CONVP(16, 2, 4)
-        PP_CONVP(32, 2, 4)
-        PP_CONVP(32, 2, 2)
-        PP_CONVP(32, 4, 2)
-      }
+      PP_CONVP(0, 16, 1, 4)
+      PP_CONVP(1, 16, 2, 4)
+      PP_CONVP(2, 32, 2, 4)
+      PP_CONVP(3, 32, 2, 2)
+      PP_CONVP(4, 32, 4, 2)
 #undef PP_CONVP
     }
     auto tiles_of = [&](int mb) { return ((W[lev] + 63) / 64) * ((H[lev] + mb - 1) / mb); };
@@ -1364,7 +1471,8 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
               {&c0b, P0 * C0},    {&g_c0b, P0 * C0},    {&g_c0, P0 * C0},     {&g_cat0, P0 * cat0C},
               {&g_d1, P1 * C1},   {&g_c1, P1 * C1},     {&g_cat1, P1 * cat1C}, {&g_bb, P2 * C2},
               {&g_a2, P2 * C2},   {&g_r2, P2 * C1},     {&g_a1, P1 * C1},     {&g_r1, P1 * C0},
-              {&g_a0, P0 * C0},   {&g_out, P0 * 4}};
+              {&g_a0, P0 * C0},   {&g_out, P0 * 4},     {&s0a, P0 * C0},      {&s1a, P1 * C1},
+              {&p0a, P0 * C1},    {&p1a, P1 * C2}};
   for (auto& b : bufs)
     if ((rc = b.b->alloc(b.n))) return rc;
   for (auto& b : bufs) PP_HIP(hipMemsetAsync(b.b->p, 0, b.n * 2, stream));
@@ -1411,6 +1519,15 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
                       (const void*)k_convp<32, 4, 2>};
   for (auto k : kf)
     PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+  // the forward pass without backward reads d1a's and d0a's concatenated
+  // inputs in two parts when both run on the persistent kernel with whole
+  // chunks per part
+  split_fwd = C0 % 16 == 0 && C1 % 16 == 0 && C2 % 16 == 0;
+  for (int l : {6, 8}) {
+    const Layer& Lf = layers[l];
+    const int shape = convp_pick(Lf.cin, Lf.coutP / 16), pch = l == 6 ? C2 : C1;
+    if (Lf.dil != 1 || shape < 0 || pch % kConvpShapes[shape][0]) split_fwd = false;
+  }
   size_t mw = 0;
   for (auto& L : layers)
     mw = std::max(mw, ((size_t)L.cin * 3 * (64 + 2 * L.dil) + (size_t)L.coutP * 64) * 2);
@@ -1445,7 +1562,7 @@ int Cnn::pack_weights() {
 }
 
 // M(r) into out (FP64 DOF vector); r on the device.
-int Cnn::apply(const double* r, double* out) {
+int Cnn::apply(const double* r, double* out, bool keep) {
   int rc;
   hipLaunchKernelGGL(k_bj_norm, dim3(kDotBlocks), dim3(256), 0, stream, (int64_t)pl->n_nodes,
                      (const double*)pl->minv.p, r, zbj.p, sw, st, dotp.p);
@@ -1463,18 +1580,44 @@ int Cnn::apply(const double* r, double* out) {
       vd1{d1.p, C1, 0}, vpd1{cat0.p, cat0C, 0}, vcat0{cat0.p, cat0C, 0}, vc0{c0.p, C0, 0},
       vc0b{c0b.p, C0, 0};
   if ((rc = conv_fwd(L[0], vx, va0, nullptr, 0, 0, false))) return rc;
-  if ((rc = conv_fwd(L[1], va0, vs0, nullptr, 0, 0, false))) return rc;
-  if ((rc = resample(true, vs0, 0, vr1, 1, C0, 1.0f, nullptr, false))) return rc;
-  if ((rc = conv_fwd(L[2], vr1, va1, nullptr, 0, 1, false))) return rc;
-  if ((rc = conv_fwd(L[3], va1, vs1, nullptr, 0, 1, false))) return rc;
-  if ((rc = resample(true, vs1, 1, vr2, 2, C1, 1.0f, nullptr, false))) return rc;
-  if ((rc = conv_fwd(L[4], vr2, va2, nullptr, 0, 2, false))) return rc;
-  if ((rc = conv_fwd(L[5], va2, vb, nullptr, 0, 2, false))) return rc;
-  if ((rc = resample(false, vb, 2, vpb, 1, C2, 1.0f, nullptr, false))) return rc;
-  if ((rc = conv_fwd(L[6], vcat1, vc1, nullptr, 0, 1, false))) return rc;
-  if ((rc = conv_fwd(L[7], vc1, vd1, nullptr, 0, 1, false))) return rc;
-  if ((rc = resample(false, vd1, 1, vpd1, 0, C1, 1.0f, nullptr, false))) return rc;
-  if ((rc = conv_fwd(L[8], vcat0, vc0, nullptr, 0, 0, false))) return rc;
+  if (!keep && split_fwd) {
+    // skip and prolonged tensors stored alone (whole cache lines), resampled
+    // in 16-byte accesses; the decoder's first layers read the two parts of
+    // their concatenated input in place: the concatenations are never written
+    const TV vs0a{s0a.p, C0, 0}, vs1a{s1a.p, C1, 0}, vp0a{p0a.p, C1, 0}, vp1a{p1a.p, C2, 0};
+    if ((rc = conv_fwd(L[1], va0, vs0a, nullptr, 0, 0, false))) return rc;
+    hipLaunchKernelGGL(k_restrict8, dim3(nblk((int64_t)H[1] * W[1] * (C0 / 8))), dim3(256), 0,
+                       stream, (const uint16_t*)s0a.p, H[0], W[0], r1.p, H[1], W[1], C0);
+    if ((rc = conv_fwd(L[2], vr1, va1, nullptr, 0, 1, false))) return rc;
+    if ((rc = conv_fwd(L[3], va1, vs1a, nullptr, 0, 1, false))) return rc;
+    hipLaunchKernelGGL(k_restrict8, dim3(nblk((int64_t)H[2] * W[2] * (C1 / 8))), dim3(256), 0,
+                       stream, (const uint16_t*)s1a.p, H[1], W[1], r2.p, H[2], W[2], C1);
+    PP_HIP(hipGetLastError());
+    if ((rc = conv_fwd(L[4], vr2, va2, nullptr, 0, 2, false))) return rc;
+    if ((rc = conv_fwd(L[5], va2, vb, nullptr, 0, 2, false))) return rc;
+    hipLaunchKernelGGL(k_prolong8, dim3(nblk((int64_t)H[1] * W[1] * (C2 / 8))), dim3(256), 0,
+                       stream, (const uint16_t*)bb.p, H[2], W[2], p1a.p, H[1], W[1], C2);
+    PP_HIP(hipGetLastError());
+    if ((rc = conv_fwd(L[6], vs1a, vc1, nullptr, 0, 1, false, nullptr, 0, &vp1a, C2))) return rc;
+    if ((rc = conv_fwd(L[7], vc1, vd1, nullptr, 0, 1, false))) return rc;
+    hipLaunchKernelGGL(k_prolong8, dim3(nblk((int64_t)H[0] * W[0] * (C1 / 8))), dim3(256), 0,
+                       stream, (const uint16_t*)d1.p, H[1], W[1], p0a.p, H[0], W[0], C1);
+    PP_HIP(hipGetLastError());
+    if ((rc = conv_fwd(L[8], vs0a, vc0, nullptr, 0, 0, false, nullptr, 0, &vp0a, C1))) return rc;
+  } else {
+    if ((rc = conv_fwd(L[1], va0, vs0, nullptr, 0, 0, false))) return rc;
+    if ((rc = resample(true, vs0, 0, vr1, 1, C0, 1.0f, nullptr, false))) return rc;
+    if ((rc = conv_fwd(L[2], vr1, va1, nullptr, 0, 1, false))) return rc;
+    if ((rc = conv_fwd(L[3], va1, vs1, nullptr, 0, 1, false))) return rc;
+    if ((rc = resample(true, vs1, 1, vr2, 2, C1, 1.0f, nullptr, false))) return rc;
+    if ((rc = conv_fwd(L[4], vr2, va2, nullptr, 0, 2, false))) return rc;
+    if ((rc = conv_fwd(L[5], va2, vb, nullptr, 0, 2, false))) return rc;
+    if ((rc = resample(false, vb, 2, vpb, 1, C2, 1.0f, nullptr, false))) return rc;
+    if ((rc = conv_fwd(L[6], vcat1, vc1, nullptr, 0, 1, false))) return rc;
+    if ((rc = conv_fwd(L[7], vc1, vd1, nullptr, 0, 1, false))) return rc;
+    if ((rc = resample(false, vd1, 1, vpd1, 0, C1, 1.0f, nullptr, false))) return rc;
+    if ((rc = conv_fwd(L[8], vcat0, vc0, nullptr, 0, 0, false))) return rc;
+  }
   if ((rc = conv_fwd(L[9], vc0, vc0b, nullptr, 0, 0, false))) return rc;
   if ((rc = conv_fwd(L[10], vc0b, TV{nullptr, 0, 0}, out32.p, 4, 0, false))) return rc;
   hipLaunchKernelGGL(k_unpack_out, dim3(nblk(pl->n_nodes)), dim3(256), 0, stream, nx, ny, W[0],
@@ -1556,14 +1699,14 @@ int Cnn::smooth(const double* in, const double* base, double* out) {
 // step g = u.r1 / u.K u minimises the energy error of z1 along u, so an
 // application is never worse than the smoothing alone; in training g = 1
 // (the loss sees the network's own output).
-int Cnn::precondition(const double* r, double* out, bool line_search) {
+int Cnn::precondition(const double* r, double* out, bool line_search, bool keep) {
   int rc;
   const int64_t nd = pl->n_dof;
   if ((rc = smooth(r, nullptr, z0v.p))) return rc;
   if ((rc = struct_apply_k(pl, z0v.p, kvec.p))) return rc;
   hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, r, -1.0,
                      (const double*)kvec.p, r1v.p);
-  if ((rc = apply(r1v.p, mout.p))) return rc;
+  if ((rc = apply(r1v.p, mout.p, keep))) return rc;
   if ((rc = struct_apply_k(pl, mout.p, kvec.p))) return rc;  // K u
   double* g = scal.p + 10;
   if (line_search) {
@@ -1601,7 +1744,7 @@ int Cnn::loss_grad(const double* e, double* loss_dev) {
   int rc;
   const int64_t nd = pl->n_dof;
   if ((rc = struct_apply_k(pl, e, rvec.p))) return rc;
-  if ((rc = precondition(rvec.p, dvec.p, false))) return rc;  // z (scal[0] = s of the network input)
+  if ((rc = precondition(rvec.p, dvec.p, false, true))) return rc;  // z (scal[0] = s of the input)
   hipLaunchKernelGGL(k_sub, dim3(nblk(nd)), dim3(256), 0, stream, nd, e, (const double*)dvec.p,
                      dvec.p);  // e_f
   if ((rc = struct_apply_k(pl, dvec.p, kvec.p))) return rc;  // q1
