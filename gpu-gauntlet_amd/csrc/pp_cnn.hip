@@ -51,6 +51,15 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// Eight BF16 values at an 8-byte aligned LDS address, as two 8-byte reads
+// (one ds_read2_b64): the K = 32 MFMA operand of one lane.
+__device__ __forceinline__ bf8 lds8(const uint16_t* p) {
+  const s4 lo = *(const s4*)p, hi = *(const s4*)(p + 4);
+  return __builtin_bit_cast(bf8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // round to nearest even (NaN kept quiet): one v_cvt_pk_bf16_f32 on gfx950
@@ -452,11 +461,14 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
   const int tiles_x = (A.W + 63) / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
+  // 32-channel chunks multiply on v_mfma_f32_16x16x32_bf16 (lane kq holds K
+  // 8 kq .. 8 kq + 7 of A and B), 16-channel chunks on the K = 16 form (4 kq ..)
+  constexpr int KQ = CC == 32 ? 8 : 4;
   int pbase[MB];
 #pragma unroll
   for (int j = 0; j < MB; j++) {
     const int mb = wave * MB + j;
-    pbase[j] = ((mb >> 2) * TW + (mb & 3) * 16 + m) * CS + 4 * kq;
+    pbase[j] = ((mb >> 2) * TW + (mb & 3) * 16 + m) * CS + KQ * kq;
   }
   // weights, once: row co holds k = tap * cin + ci, 9 cin entries; kStage
   // loads in flight per thread before their LDS stores
@@ -524,24 +536,39 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
       __syncthreads();
       if (ch + 1 < nch) load_stage(tile, (ch + 1) * CC);
       else if (tile + per_xcd < tile_hi) load_stage(tile + per_xcd, 0);
-      const int kc0 = ch * CC + 4 * kq;
+      const int kc0 = ch * CC + KQ * kq;
 #pragma unroll
       for (int tap = 0; tap < 9; tap++) {
         const int toff = ((tap / 3) * TW + tap % 3) * CS;
+        if constexpr (CC == 32) {
+          const int kb = tap * cin + kc0;
+          bf8 b[NB];
 #pragma unroll
-        for (int c16 = 0; c16 < CC; c16 += 16) {
-          const int kb = tap * cin + kc0 + c16;
-          s4 b[NB];
+          for (int nb = 0; nb < NB; nb++) b[nb] = lds8(&w_s[(nb * 16 + m) * KP + kb]);
+          bf8 a[MB];
 #pragma unroll
-          for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KP + kb];
-          s4 a[MB];
-#pragma unroll
-          for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff + c16];
+          for (int j = 0; j < MB; j++) a[j] = lds8(&in_s[pbase[j] + toff]);
 #pragma unroll
           for (int j = 0; j < MB; j++)
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
-              acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+              acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[nb], a[j], acc[j][nb], 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int c16 = 0; c16 < CC; c16 += 16) {
+            const int kb = tap * cin + kc0 + c16;
+            s4 b[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4*)&w_s[(nb * 16 + m) * KP + kb];
+            s4 a[MB];
+#pragma unroll
+            for (int j = 0; j < MB; j++) a[j] = *(const s4*)&in_s[pbase[j] + toff + c16];
+#pragma unroll
+            for (int j = 0; j < MB; j++)
+#pragma unroll
+              for (int nb = 0; nb < NB; nb++)
+                acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[nb], a[j], acc[j][nb], 0, 0, 0);
+          }
         }
       }
     }
