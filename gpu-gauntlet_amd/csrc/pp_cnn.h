@@ -122,6 +122,12 @@ struct Cnn {
   int conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int out32_cs, int lev,
                bool backward, const TV* mask = nullptr, int mask_from = 0,
                const TV* first = nullptr, int first_ch = 0);
+  // the first two convolutions (4 -> 16 -> 16 channels) in one launch, for
+  // the forward pass without backward (k_conv2)
+  static constexpr int kConv2Rows = 4;
+  bool fuse_pairs = true;
+  bool conv2_ok(const Layer& La, const Layer& Lb) const;
+  int conv2_fwd(const Layer& La, const Layer& Lb, const TV& in, const TV& out, int lev);
   int convp_pick(int cin, int nb) const;   // k_convp shape index or -1
   size_t convp_lds(int cin, int k) const;  // its dynamic LDS bytes
   int conv_wgrad(const Layer& L, const TV& in, const TV& dout, int lev);

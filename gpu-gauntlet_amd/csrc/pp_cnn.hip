@@ -604,6 +604,164 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
   }
 }
 
+// The network's first two convolutions in one kernel: e0a (4 -> 16 channels)
+// then e0b (16 -> 16), both 3x3 with dilation 1.  Per output tile of MB rows
+// x 64 pixels the 4-channel input region with a two-pixel halo, (MB + 4) x 68
+// pixels at 8 bytes each, is staged in LDS; e0a is evaluated on the tile with
+// a one-pixel halo, (MB + 2) x 66 pixels taken as a flat list of 16-pixel
+// blocks (lane kq of K step s covers tap 4 s + kq, steps past tap 8 having
+// zero weights), rounded to BF16 (zero outside the grid, the zero padding e0b
+// sees when the two run apart) into a second LDS buffer that e0b reads.  So
+// the 16-channel intermediate never goes through memory, and the result has
+// the bits of the two separate launches (the same BF16 rounding points and
+// MFMA K order).  Persistent as k_convp: both layers' weights staged once per
+// workgroup, tiles walked in XCD row bands, the next tile's input loaded
+// into registers while the current one is computed.  (Fusing two 16-channel
+// layers the same way measured slower than their two launches: the halo
+// recomputation and the LDS traffic of the first layer outweigh the saved
+// round trip of the intermediate.)
+struct Conv2Args {
+  const uint16_t* in;  // e0a input, 4 channels at in[pixel * in_cs + in_c0 + c]
+  int in_cs, in_c0;
+  const uint16_t* wa;  // packed forward weights of e0a [16][kpa], k = tap * 4 + ci
+  const uint16_t* wb;  // of e0b [16][kpb], k = tap * 16 + ci
+  int kpa, kpb;
+  const float* bias_a;  // [16] or null
+  const float* bias_b;  // [16] or null
+  int relu_a, relu_b;
+  uint16_t* out;  // e0b output (BF16)
+  int out_cs, out_c0, cout;
+  int H, W;
+};
+
+template <int MB>
+__global__ __launch_bounds__(256) void k_conv2(Conv2Args A, int ntiles) {
+  constexpr int IW = 68, IH = MB + 4, MW = 66, MH = MB + 2;
+  constexpr int CSA = 4, CSM = 20;  // LDS pixel strides (4 input, 16 + 4 mid channels)
+  constexpr int KA = 48, KPA = KA + 4, KPB = 9 * 16 + 4;
+  constexpr int NMID = MH * MW, MBLK = (NMID + 15) / 16;
+  constexpr int NIN = IH * IW, UIN = (NIN + 255) / 256;  // 8-byte staging units
+  extern __shared__ uint16_t smem[];
+  uint16_t* wa_s = smem;
+  uint16_t* wb_s = wa_s + 16 * KPA;
+  uint16_t* in_s = wb_s + 16 * KPB;
+  uint16_t* mid_s = in_s + IH * IW * CSA;
+  const int band = (ntiles + 7) / 8, xcd = (int)(blockIdx.x % 8);
+  const int per_xcd = (int)(gridDim.x / 8), local0 = (int)(blockIdx.x / 8);
+  const int tile_lo = xcd * band, tile_hi = min(ntiles, tile_lo + band);
+  const int tiles_x = (A.W + 63) / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  // both layers' weights, once (8-byte units; rows padded by 4 entries)
+  for (int i = threadIdx.x; i < 16 * (KA / 4); i += 256) {
+    const int co = i / (KA / 4), k = 4 * (i - co * (KA / 4));
+    *(uint2*)&wa_s[co * KPA + k] = *(const uint2*)&A.wa[(size_t)co * A.kpa + k];
+  }
+  for (int i = threadIdx.x; i < 16 * 36; i += 256) {
+    const int co = i / 36, k = 4 * (i - co * 36);
+    *(uint2*)&wb_s[co * KPB + k] = *(const uint2*)&A.wb[(size_t)co * A.kpb + k];
+  }
+  // layer a: lane m of mid block b is mid pixel min(16 b + m, NMID - 1); its
+  // tap (0, 0) input pixel sits at the same (row, col) of the input region
+  uint2 vi[UIN];
+  auto load_stage = [&](int tile) {
+    const int tx0 = (tile % tiles_x) * 64 - 2, ty0 = (tile / tiles_x) * MB - 2;
+#pragma unroll
+    for (int u = 0; u < UIN; u++) {
+      const int i = threadIdx.x + 256 * u;
+      vi[u] = uint2{0u, 0u};
+      if (i < NIN) {
+        const int gx = tx0 + i % IW, gy = ty0 + i / IW;
+        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
+          vi[u] = *(const uint2*)&A.in[(gy * A.W + gx) * A.in_cs + A.in_c0];
+      }
+    }
+  };
+  int tile = tile_lo + local0;
+  if (tile < tile_hi) load_stage(tile);
+  for (; tile < tile_hi; tile += per_xcd) {
+    const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
+    __syncthreads();  // the previous tile's LDS reads are done (and the weights stored)
+#pragma unroll
+    for (int u = 0; u < UIN; u++) {
+      const int i = threadIdx.x + 256 * u;
+      if (i < NIN) *(uint2*)&in_s[i * CSA] = vi[u];
+    }
+    __syncthreads();
+    if (tile + per_xcd < tile_hi) load_stage(tile + per_xcd);
+    // layer a, two mid blocks at a time (independent MFMA chains)
+    for (int b0 = wave; b0 < MBLK; b0 += 8) {
+      f4 acc[2] = {f4{0.0f, 0.0f, 0.0f, 0.0f}, f4{0.0f, 0.0f, 0.0f, 0.0f}};
+      int q[2], base[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        q[h] = min(16 * (b0 + 4 * h) + m, NMID - 1);
+        const int row = q[h] / MW, col = q[h] - row * MW;
+        base[h] = (row * IW + col) * CSA;
+      }
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        const int tap = 4 * s + kq;
+        const int toff = tap < 9 ? ((tap / 3) * IW + tap % 3) * CSA : 0;
+        const s4 w = *(const s4*)&wa_s[m * KPA + 16 * s + 4 * kq];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+              w, *(const s4*)&in_s[base[h] + toff], acc[h], 0, 0, 0);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (b0 + 4 * h >= MBLK || 16 * (b0 + 4 * h) + m >= NMID) continue;
+        const int row = q[h] / MW, col = q[h] - row * MW;
+        const int gy = ty0 - 1 + row, gx = tx0 - 1 + col;
+        const bool inside = gx >= 0 && gx < A.W && gy >= 0 && gy < A.H;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          v[i] = acc[h][i] + (A.bias_a ? A.bias_a[4 * kq + i] : 0.0f);
+          if (A.relu_a) v[i] = fmaxf(v[i], 0.0f);
+          if (!inside) v[i] = 0.0f;
+        }
+        st4(&mid_s[q[h] * CSM + 4 * kq], v);
+      }
+    }
+    __syncthreads();
+    // layer b on the tile: wave w owns 16-pixel blocks w MB .. w MB + MB - 1
+    f4 acc[MB];
+    int mbase[MB];
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+      const int mb = wave * MB + j;
+      mbase[j] = ((mb >> 2) * MW + (mb & 3) * 16 + m) * CSM + 4 * kq;
+      acc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+      const int toff = ((tap / 3) * MW + tap % 3) * CSM;
+      const s4 w = *(const s4*)&wb_s[m * KPB + 16 * tap + 4 * kq];
+#pragma unroll
+      for (int j = 0; j < MB; j++)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, *(const s4*)&mid_s[mbase[j] + toff],
+                                                            acc[j], 0, 0, 0);
+    }
+    const int co = 4 * kq;
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+      const int mb = wave * MB + j;
+      const int oy = ty0 + (mb >> 2), ox = tx0 + (mb & 3) * 16 + m;
+      if (oy >= A.H || ox >= A.W || co >= A.cout) continue;
+      const int op = oy * A.W + ox;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = acc[j][i] + (A.bias_b && co + i < A.cout ? A.bias_b[co + i] : 0.0f);
+        if (A.relu_b) v[i] = fmaxf(v[i], 0.0f);
+      }
+      st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
+    }
+  }
+}
+
 // Weight gradient dW[co][tap*cin + ci] = sum_p dout[p][co] in[p + off(tap)][ci]:
 // a GEMM over pixels (K), M = output channels, N = (tap, ci).  One row
 // segment of 64 pixels at a time is staged transposed in LDS (dout as
@@ -1365,6 +1523,45 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
   return PP_OK;
 }
 
+// LDS bytes of k_conv2<mb>: both layers' weights, the input region and the
+// intermediate tile
+static size_t conv2_lds(int mb) {
+  return ((size_t)16 * (48 + 4) + 16 * (9 * 16 + 4) + (size_t)(mb + 4) * 68 * 4 +
+          (size_t)(mb + 2) * 66 * 20) * 2;
+}
+
+bool Cnn::conv2_ok(const Layer& La, const Layer& Lb) const {
+  return La.cin == 4 && La.cout == 16 && Lb.cin == 16 && Lb.cout == 16 && La.dil == 1 &&
+         Lb.dil == 1 && La.kp == 48 && Lb.kp == 144 && conv2_lds(kConv2Rows) <= lds_max;
+}
+
+int Cnn::conv2_fwd(const Layer& La, const Layer& Lb, const TV& in, const TV& out, int lev) {
+  if (!conv2_ok(La, Lb) || in.cs % 4 || in.c0 % 4 || out.cs % 4 || out.c0 % 4)
+    return fail(PP_ERR_INVALID, "conv2: layer pair not supported");
+  Conv2Args A;
+  memset(&A, 0, sizeof(A));
+  A.in = in.p; A.in_cs = in.cs; A.in_c0 = in.c0;
+  A.wa = wf.p + La.wf_off; A.kpa = La.kp;
+  A.wb = wf.p + Lb.wf_off; A.kpb = Lb.kp;
+  A.bias_a = linear ? nullptr : params.p + La.b_off;
+  A.bias_b = linear ? nullptr : params.p + Lb.b_off;
+  A.relu_a = La.relu; A.relu_b = Lb.relu;
+  A.out = out.p; A.out_cs = out.cs; A.out_c0 = out.c0; A.cout = Lb.cout;
+  A.H = H[lev]; A.W = W[lev];
+  const int64_t cs_max = std::max((int64_t)in.cs, (int64_t)out.cs);
+  if ((int64_t)H[lev] * W[lev] * cs_max + cs_max >= (1LL << 31))
+    return fail(PP_ERR_INVALID, "conv2: tensor too large for 32-bit offsets");
+  constexpr int R = kConv2Rows;
+  const int nt = ((W[lev] + 63) / 64) * ((H[lev] + R - 1) / R);
+  const size_t lds = conv2_lds(R);
+  int occ = 0;
+  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_conv2<R>, 256, lds));
+  const int g = 8 * std::max(1, std::min((nt + 7) / 8, std::max(1, occ) * cu / 8));
+  hipLaunchKernelGGL(k_conv2<R>, dim3(g), dim3(256), lds, stream, A, nt);
+  PP_HIP(hipGetLastError());
+  return PP_OK;
+}
+
 int Cnn::conv_wgrad(const Layer& L, const TV& in, const TV& dout, int lev) {
   WgArgs A;
   A.in = in.p; A.in_cs = in.cs; A.in_c0 = in.c0; A.cin = L.cin;
@@ -1423,6 +1620,7 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
   st = cfg && cfg->rotation_scale > 0 ? cfg->rotation_scale : 0.3;
   linear = cfg && cfg->relu ? 0 : 1;
   line_search = cfg && cfg->fixed_step ? 0 : 1;
+  fuse_pairs = !(cfg && cfg->unfused_forward);
   // FCG(8) by default; FCG(4) with the multigrid smoother, whose stronger
   // steps need fewer kept directions (1024^2: 217 iterations against 242)
   fcg_window = cfg && cfg->fcg_window > 0 ? cfg->fcg_window : cfg && cfg->multigrid ? 4 : 8;
@@ -1543,7 +1741,7 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
                       (const void*)k_convf<32, 2, 4>, (const void*)k_convf<32, 4, 2>,
                       (const void*)k_convp<16, 1, 4>, (const void*)k_convp<16, 2, 4>,
                       (const void*)k_convp<32, 2, 4>, (const void*)k_convp<32, 2, 2>,
-                      (const void*)k_convp<32, 4, 2>};
+                      (const void*)k_convp<32, 4, 2>, (const void*)k_conv2<kConv2Rows>};
   for (auto k : kf)
     PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
   // the forward pass without backward reads d1a's and d0a's concatenated
@@ -1606,13 +1804,20 @@ int Cnn::apply(const double* r, double* out, bool keep) {
       vb{bb.p, C2, 0}, vpb{cat1.p, cat1C, 0}, vcat1{cat1.p, cat1C, 0}, vc1{c1b.p, C1, 0},
       vd1{d1.p, C1, 0}, vpd1{cat0.p, cat0C, 0}, vcat0{cat0.p, cat0C, 0}, vc0{c0.p, C0, 0},
       vc0b{c0b.p, C0, 0};
-  if ((rc = conv_fwd(L[0], vx, va0, nullptr, 0, 0, false))) return rc;
+  // consecutive convolutions at one level fused into one launch (the
+  // solver's forward pass; the training pass keeps every activation)
+  const bool pair0 = !keep && fuse_pairs && conv2_ok(L[0], L[1]) && split_fwd;
+  if (!pair0 && (rc = conv_fwd(L[0], vx, va0, nullptr, 0, 0, false))) return rc;
   if (!keep && split_fwd) {
     // skip and prolonged tensors stored alone (whole cache lines), resampled
     // in 16-byte accesses; the decoder's first layers read the two parts of
     // their concatenated input in place: the concatenations are never written
     const TV vs0a{s0a.p, C0, 0}, vs1a{s1a.p, C1, 0}, vp0a{p0a.p, C1, 0}, vp1a{p1a.p, C2, 0};
-    if ((rc = conv_fwd(L[1], va0, vs0a, nullptr, 0, 0, false))) return rc;
+    if (pair0) {
+      if ((rc = conv2_fwd(L[0], L[1], vx, vs0a, 0))) return rc;
+    } else if ((rc = conv_fwd(L[1], va0, vs0a, nullptr, 0, 0, false))) {
+      return rc;
+    }
     hipLaunchKernelGGL(k_restrict8, dim3(nblk((int64_t)H[1] * W[1] * (C0 / 8))), dim3(256), 0,
                        stream, (const uint16_t*)s0a.p, H[0], W[0], r1.p, H[1], W[1], C0);
     if ((rc = conv_fwd(L[2], vr1, va1, nullptr, 0, 1, false))) return rc;

@@ -70,7 +70,8 @@ class CnnConfig(C.Structure):
                 ("rotation_scale", C.c_double), ("smoother_weight", C.c_double),
                 ("relu", C.c_int32), ("fixed_step", C.c_int32),
                 ("fcg_window", C.c_int32), ("multigrid", C.c_int32), ("mg_sweeps", C.c_int32),
-                ("mg_coarse_sweeps", C.c_int32), ("mg_jacobi", C.c_int32)]
+                ("mg_coarse_sweeps", C.c_int32), ("mg_jacobi", C.c_int32),
+                ("unfused_forward", C.c_int32)]
 
 
 class TrainOpts(C.Structure):

@@ -183,17 +183,20 @@ class Net:
     def __init__(self, plan, seed=1, channels=(16, 32, 64), bottleneck_dilation=1,
                  rotation_scale=0.3, smoother_weight=0.0, relu=False, fixed_step=False,
                  fcg_window=0, multigrid=False, mg_sweeps=0, mg_coarse_sweeps=0,
-                 mg_jacobi=False):
+                 mg_jacobi=False, unfused_forward=False):
         """multigrid: the smoother around the network is a Galerkin V-cycle
         (mg_sweeps smoothing steps on the finest level before and after the
         coarse correction, 0 = 6 Chebyshev steps or 3 damped block-Jacobi
         sweeps with mg_jacobi; mg_coarse_sweeps on the coarser levels, 0 = 1)
-        instead of w B."""
+        instead of w B.  unfused_forward: every convolution of the solver's
+        forward pass as its own launch (the default runs the first two as
+        one)."""
         self.plan, self.lib = plan, plan.lib
         cfg = _lib.CnnConfig((C.c_int32 * 3)(*channels), int(bottleneck_dilation),
                              float(rotation_scale), float(smoother_weight), int(bool(relu)),
                              int(bool(fixed_step)), int(fcg_window), int(bool(multigrid)),
-                             int(mg_sweeps), int(mg_coarse_sweeps), int(bool(mg_jacobi)))
+                             int(mg_sweeps), int(mg_coarse_sweeps), int(bool(mg_jacobi)),
+                             int(bool(unfused_forward)))
         h = C.c_void_p()
         check(self.lib.pp_cnn_create(plan.h, C.byref(cfg), int(seed), C.byref(h)))
         self.h = h

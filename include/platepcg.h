@@ -300,6 +300,10 @@ typedef struct {
   int32_t mg_jacobi;           /* smoothing steps: 0 = Chebyshev semi-iteration for B A on
                                   [lambda_max / 16, lambda_max] (the default), 1 = damped block
                                   Jacobi w B, w = 4 / (3 lambda_max) */
+  int32_t unfused_forward;     /* 1: every convolution of the solver's forward pass runs as its
+                                  own launch; 0 (default): the first two (4 -> 16 -> 16
+                                  channels) run as one launch with the intermediate tensor in
+                                  LDS (the same bits) */
 } pp_cnn_config;
 
 #define PP_TRAIN_CURVE 64

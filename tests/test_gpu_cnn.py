@@ -91,6 +91,26 @@ def test_forward_matches_oracle(ctx, n, relu):
     net.close()
 
 
+@pytest.mark.parametrize("n,relu", [(30, False), (130, False), (130, True), (257, False)])
+def test_fused_first_layers_equal_separate_launches(ctx, n, relu):
+    """The solver's forward pass runs e0a and e0b as one launch (the
+    intermediate tensor kept in LDS); it must give the bits of separate
+    launches, on padded grids, several tiles per row and a partial last row
+    of tiles, with and without ReLU and biases."""
+    mesh, mask, et, plan, op = plate(ctx, n)
+    fused = Net(plan, seed=1, relu=relu)
+    apart = Net(plan, seed=1, relu=relu, unfused_forward=True)
+    p = rand_params(U.layer_shapes(), seed=n + 7)
+    fused.set_params(p)
+    apart.set_params(p)
+    e = U.training_error(mesh["nodes"], n, n, mask, seed=3)
+    r = op.apply_k(e)
+    np.testing.assert_array_equal(fused.forward(r), apart.forward(r))
+    np.testing.assert_array_equal(fused.apply(r), apart.apply(r))
+    fused.close()
+    apart.close()
+
+
 @pytest.mark.parametrize("relu", [False, True])
 def test_loss_and_gradient_match_oracle(ctx, relu):
     n = 32
