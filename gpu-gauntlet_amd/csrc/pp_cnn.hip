@@ -496,26 +496,65 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
       }
     }
   }
+  // Staging without branches: every thread loads from an address clamped
+  // into the tensor (units past the tile repeat the last one, and LDS then
+  // receives the same value twice) and keeps a bit per unit that zeroes it
+  // when it is written to LDS.  With no divergent loads the compiler's
+  // waits are exact counts, so waiting for one stage's loads does not also
+  // wait for the stores issued after them.
   uint4 vi[UIN];
+  uint32_t inside = 0;
   auto load_stage = [&](int tile, int c0) {
     const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
     const bool first = c0 < A.p_ch;
     const uint16_t* src = first ? A.pin + A.p_c0 + c0 : A.in + A.in_c0 + (c0 - A.p_ch);
     const int cs = first ? A.p_cs : A.in_cs;
+    inside = 0;
 #pragma unroll
     for (int u = 0; u < UIN; u++) {
-      const int i = threadIdx.x + 256 * u;
-      vi[u] = uint4{0u, 0u, 0u, 0u};
-      if (i < NIN) {
-        const int pix = i / Q8, part = i % Q8;
-        const int gx = tx0 - 1 + pix % TW, gy = ty0 - 1 + pix / TW;
-        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
-          vi[u] = *(const uint4*)&src[(gy * A.W + gx) * cs + 8 * part];
+      const int i = min((int)threadIdx.x + 256 * u, NIN - 1);
+      const int pix = i / Q8, part = i % Q8;
+      const int gx = tx0 - 1 + pix % TW, gy = ty0 - 1 + pix / TW;
+      const bool ok = gx >= 0 && gx < A.W && gy >= 0 && gy < A.H;
+      inside |= (uint32_t)ok << u;
+      const int cx = min(max(gx, 0), A.W - 1), cy = min(max(gy, 0), A.H - 1);
+      vi[u] = *(const uint4*)&src[(cy * A.W + cx) * cs + 8 * part];
+    }
+  };
+  // the bias, staged in LDS once: a global load in the epilogue would make
+  // the epilogue's wait also wait for the next tile's prefetch
+  float* bias_s = (float*)(in_s + TH * TW * CS);
+  if ((int)threadIdx.x < NB * 16) {
+    const int co = (int)threadIdx.x;
+    bias_s[co] = A.bias && co < A.cout ? A.bias[co] : 0.0f;
+  }
+  // (the halo outside the tensor is zeroed by a second LDS write, not by a
+  // select on the loaded value, which the compiler would schedule, with
+  // its wait, right after the loads)
+  auto store_stage = [&]() {
+#pragma unroll
+    for (int u = 0; u < UIN; u++) {
+      const int i = min((int)threadIdx.x + 256 * u, NIN - 1);
+      uint16_t* d = &in_s[(i / Q8) * CS + 8 * (i % Q8)];
+      *(uint2*)d = uint2{vi[u].x, vi[u].y};
+      *(uint2*)(d + 4) = uint2{vi[u].z, vi[u].w};
+      if (!((inside >> u) & 1)) {
+        *(uint2*)d = uint2{0u, 0u};
+        *(uint2*)(d + 4) = uint2{0u, 0u};
       }
     }
   };
+  // Stage order: the next stage's loads are issued before this stage's
+  // multiplies and written to LDS after them (behind a barrier), and only
+  // then does the epilogue store the tile's outputs.  So a wait for the
+  // loads never waits for stores issued after them (the counter of
+  // outstanding vector memory operations covers loads and stores in order).
   int tile = tile_lo + local0;
-  if (tile < tile_hi) load_stage(tile, 0);
+  if (tile < tile_hi) {
+    load_stage(tile, 0);
+    store_stage();
+  }
+  __syncthreads();
   for (; tile < tile_hi; tile += per_xcd) {
     f4 acc[MB][NB];
 #pragma unroll
@@ -523,19 +562,12 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
 #pragma unroll
       for (int nb = 0; nb < NB; nb++) acc[j][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int ch = 0; ch < nch; ch++) {
-      __syncthreads();  // the previous stage's LDS reads are done
-#pragma unroll
-      for (int u = 0; u < UIN; u++) {
-        const int i = threadIdx.x + 256 * u;
-        if (i < NIN) {
-          uint16_t* d = &in_s[(i / Q8) * CS + 8 * (i % Q8)];
-          *(uint2*)d = uint2{vi[u].x, vi[u].y};
-          *(uint2*)(d + 4) = uint2{vi[u].z, vi[u].w};
-        }
-      }
-      __syncthreads();
-      if (ch + 1 < nch) load_stage(tile, (ch + 1) * CC);
-      else if (tile + per_xcd < tile_hi) load_stage(tile + per_xcd, 0);
+      // LDS holds stage (tile, ch); the next one is loaded into registers.
+      // Every stage loads and stores one (the workgroup's last reloads its
+      // own first chunk, never read), so no load or wait is conditional.
+      const bool more = ch + 1 < nch;
+      const int nt = more ? tile : tile + per_xcd < tile_hi ? tile + per_xcd : tile;
+      load_stage(nt, more ? (ch + 1) * CC : 0);
       const int kc0 = ch * CC + KQ * kq;
 #pragma unroll
       for (int tap = 0; tap < 9; tap++) {
@@ -571,6 +603,9 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
           }
         }
       }
+      __syncthreads();  // every wave's reads of this stage are done
+      store_stage();
+      if (more) __syncthreads();  // the tile's next chunk is in LDS
     }
     const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
 #pragma unroll
@@ -586,7 +621,7 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          v[i] = acc[j][nb][i] + (A.bias && co + i < A.cout ? A.bias[co + i] : 0.0f);
+          v[i] = acc[j][nb][i] + bias_s[co + i];
           if (A.relu) v[i] = fmaxf(v[i], 0.0f);
         }
         // a float output with room for four channels takes the zero pad channels too
@@ -601,6 +636,7 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
         }
       }
     }
+    __syncthreads();  // the next tile's first stage is in LDS
   }
 }
 
@@ -663,32 +699,51 @@ __global__ __launch_bounds__(256) void k_conv2(Conv2Args A, int ntiles) {
   }
   // layer a: lane m of mid block b is mid pixel min(16 b + m, NMID - 1); its
   // tap (0, 0) input pixel sits at the same (row, col) of the input region
+  // staging without branches, as in k_convp: clamped loads, a bit per unit
+  // outside the tensor, zeroed by a second LDS write
   uint2 vi[UIN];
+  uint32_t inside = 0;
   auto load_stage = [&](int tile) {
     const int tx0 = (tile % tiles_x) * 64 - 2, ty0 = (tile / tiles_x) * MB - 2;
+    inside = 0;
 #pragma unroll
     for (int u = 0; u < UIN; u++) {
-      const int i = threadIdx.x + 256 * u;
-      vi[u] = uint2{0u, 0u};
-      if (i < NIN) {
-        const int gx = tx0 + i % IW, gy = ty0 + i / IW;
-        if (gx >= 0 && gx < A.W && gy >= 0 && gy < A.H)
-          vi[u] = *(const uint2*)&A.in[(gy * A.W + gx) * A.in_cs + A.in_c0];
-      }
+      const int i = min((int)threadIdx.x + 256 * u, NIN - 1);
+      const int gx = tx0 + i % IW, gy = ty0 + i / IW;
+      inside |= (uint32_t)(gx >= 0 && gx < A.W && gy >= 0 && gy < A.H) << u;
+      const int cx = min(max(gx, 0), A.W - 1), cy = min(max(gy, 0), A.H - 1);
+      vi[u] = *(const uint2*)&A.in[(cy * A.W + cx) * A.in_cs + A.in_c0];
     }
   };
-  int tile = tile_lo + local0;
-  if (tile < tile_hi) load_stage(tile);
-  for (; tile < tile_hi; tile += per_xcd) {
-    const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
-    __syncthreads();  // the previous tile's LDS reads are done (and the weights stored)
+  auto store_stage = [&]() {
 #pragma unroll
     for (int u = 0; u < UIN; u++) {
-      const int i = threadIdx.x + 256 * u;
-      if (i < NIN) *(uint2*)&in_s[i * CSA] = vi[u];
+      uint2* d = (uint2*)&in_s[min((int)threadIdx.x + 256 * u, NIN - 1) * CSA];
+      *d = vi[u];
+      if (!((inside >> u) & 1)) *d = uint2{0u, 0u};
     }
-    __syncthreads();
-    if (tile + per_xcd < tile_hi) load_stage(tile + per_xcd);
+  };
+  // both layers' bias, staged in LDS once (a global load in an epilogue
+  // would make its wait also wait for the next tile's prefetch)
+  float* bias_s = (float*)(mid_s + NMID * CSM);
+  if (threadIdx.x < 16) bias_s[threadIdx.x] = A.bias_a ? A.bias_a[threadIdx.x] : 0.0f;
+  else if (threadIdx.x < 32) {
+    const int co = (int)threadIdx.x - 16;
+    bias_s[16 + co] = A.bias_b && co < A.cout ? A.bias_b[co] : 0.0f;
+  }
+  // Per tile: the next tile's input is loaded while e0a runs and written to
+  // LDS once e0a's reads are done (before e0b and its stores), so waiting
+  // for those loads never waits for stores issued after them.  Every tile
+  // loads one (the workgroup's last reloads its own, never read).
+  int tile = tile_lo + local0;
+  if (tile < tile_hi) {
+    load_stage(tile);
+    store_stage();
+  }
+  __syncthreads();  // the first tile and the weights are in LDS
+  for (; tile < tile_hi; tile += per_xcd) {
+    const int tx0 = (tile % tiles_x) * 64, ty0 = (tile / tiles_x) * MB;
+    load_stage(tile + per_xcd < tile_hi ? tile + per_xcd : tile);
     // layer a, two mid blocks at a time (independent MFMA chains)
     for (int b0 = wave; b0 < MBLK; b0 += 8) {
       f4 acc[2] = {f4{0.0f, 0.0f, 0.0f, 0.0f}, f4{0.0f, 0.0f, 0.0f, 0.0f}};
@@ -718,14 +773,15 @@ __global__ __launch_bounds__(256) void k_conv2(Conv2Args A, int ntiles) {
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          v[i] = acc[h][i] + (A.bias_a ? A.bias_a[4 * kq + i] : 0.0f);
+          v[i] = acc[h][i] + bias_s[4 * kq + i];
           if (A.relu_a) v[i] = fmaxf(v[i], 0.0f);
           if (!inside) v[i] = 0.0f;
         }
         st4(&mid_s[q[h] * CSM + 4 * kq], v);
       }
     }
-    __syncthreads();
+    __syncthreads();  // e0a's reads of the input are done, its output is in LDS
+    store_stage();
     // layer b on the tile: wave w owns 16-pixel blocks w MB .. w MB + MB - 1
     f4 acc[MB];
     int mbase[MB];
@@ -754,11 +810,12 @@ __global__ __launch_bounds__(256) void k_conv2(Conv2Args A, int ntiles) {
       float v[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        v[i] = acc[j][i] + (A.bias_b && co + i < A.cout ? A.bias_b[co + i] : 0.0f);
+        v[i] = acc[j][i] + bias_s[16 + co + i];
         if (A.relu_b) v[i] = fmaxf(v[i], 0.0f);
       }
       st4(&A.out[op * A.out_cs + A.out_c0 + co], v);
     }
+    __syncthreads();  // e0b's reads are done, the next tile's input is in LDS
   }
 }
 
@@ -1390,10 +1447,10 @@ int Cnn::convp_pick(int cin, int nb) const {
   return -1;
 }
 
-// the weights and the input tile
+// the weights, the input tile and the bias
 size_t Cnn::convp_lds(int cin, int k) const {
   const int C = kConvpShapes[k][0], N = kConvpShapes[k][1], R = kConvpShapes[k][2];
-  return ((size_t)N * 16 * (9 * cin + 4) + (size_t)(R + 2) * 66 * (C + 4)) * 2;
+  return ((size_t)N * 16 * (9 * cin + 4) + (size_t)(R + 2) * 66 * (C + 4)) * 2 + N * 16 * 4;
 }
 
 int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int out32_cs,
@@ -1527,7 +1584,7 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
 // intermediate tile
 static size_t conv2_lds(int mb) {
   return ((size_t)16 * (48 + 4) + 16 * (9 * 16 + 4) + (size_t)(mb + 4) * 68 * 4 +
-          (size_t)(mb + 2) * 66 * 20) * 2;
+          (size_t)(mb + 2) * 66 * 20) * 2 + 32 * 4;
 }
 
 bool Cnn::conv2_ok(const Layer& La, const Layer& Lb) const {
