@@ -104,6 +104,9 @@ struct ConvArgs {
   // concatenation of two tensors stored apart is read in place
   const uint16_t* pin;
   int p_cs, p_c0, p_ch;
+  // k_convp: output channels in groups of NB * 16, each workgroup one group
+  // (weights [groups * NB * 16][kp]); 1 = all output blocks per workgroup
+  int groups;
 };
 
 // ((a + b) + (c + d)) / 4 per BF16 channel, in FP32, rounded to BF16: the
@@ -455,12 +458,18 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
   const int cin = A.cin, nch = cin / CC, K9 = 9 * cin, KP = K9 + 4;
   uint16_t* w_s = smem;
   uint16_t* in_s = smem + NB * 16 * KP;
+  // workgroup b runs on XCD b % 8; the workgroups of an XCD are dealt to
+  // the output-channel groups in turn, and each walks the XCD's band of
+  // tiles for its group
   const int band = (ntiles + 7) / 8, xcd = (int)(blockIdx.x % 8);
-  const int per_xcd = (int)(gridDim.x / 8), local0 = (int)(blockIdx.x / 8);
+  const int grp = (int)(blockIdx.x / 8) % A.groups;
+  const int per_xcd = (int)(gridDim.x / 8) / A.groups, local0 = (int)(blockIdx.x / 8) / A.groups;
   const int tile_lo = xcd * band, tile_hi = min(ntiles, tile_lo + band);
   const int tiles_x = (A.W + 63) / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
+  const int co0 = grp * NB * 16;  // the group's first output channel
+  const uint16_t* wg = A.w + (size_t)co0 * A.kp;
   // 32-channel chunks multiply on v_mfma_f32_16x16x32_bf16 (lane kq holds K
   // 8 kq .. 8 kq + 7 of A and B), 16-channel chunks on the K = 16 form (4 kq ..)
   constexpr int KQ = CC == 32 ? 8 : 4;
@@ -481,7 +490,7 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
         const int i = i0 + 256 * u;
         if (i < nw) {
           const int co = i / units, kk = (i - co * units) * 8;
-          v[u] = *(const uint4*)&A.w[(size_t)co * A.kp + kk];
+          v[u] = *(const uint4*)&wg[(size_t)co * A.kp + kk];
         }
       }
 #pragma unroll
@@ -525,8 +534,8 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
   // the epilogue's wait also wait for the next tile's prefetch
   float* bias_s = (float*)(in_s + TH * TW * CS);
   if ((int)threadIdx.x < NB * 16) {
-    const int co = (int)threadIdx.x;
-    bias_s[co] = A.bias && co < A.cout ? A.bias[co] : 0.0f;
+    const int co = co0 + (int)threadIdx.x;
+    bias_s[threadIdx.x] = A.bias && co < A.cout ? A.bias[co] : 0.0f;
   }
   // (the halo outside the tensor is zeroed by a second LDS write, not by a
   // select on the loaded value, which the compiler would schedule, with
@@ -616,12 +625,12 @@ __global__ __launch_bounds__(256) void k_convp(ConvArgs A, int ntiles) {
       const int op = oy * A.W + ox;
 #pragma unroll
       for (int nb = 0; nb < NB; nb++) {
-        const int co = nb * 16 + 4 * kq;
+        const int co = co0 + nb * 16 + 4 * kq;
         if (co >= A.cout) continue;
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          v[i] = acc[j][nb][i] + bias_s[co + i];
+          v[i] = acc[j][nb][i] + bias_s[nb * 16 + 4 * kq + i];
           if (A.relu) v[i] = fmaxf(v[i], 0.0f);
         }
         // a float output with room for four channels takes the zero pad channels too
@@ -1431,17 +1440,26 @@ constexpr int kRowsPerChunk = 8;
 
 // The persistent kernel's compile-time shapes (input channels per chunk,
 // 16-channel output blocks, rows per tile), in the order they are tried.
-static const int kConvpShapes[5][3] = {{16, 1, 4}, {16, 2, 4}, {32, 2, 4}, {32, 2, 2}, {32, 4, 2}};
+static const int kConvpShapes[6][3] = {{16, 1, 4}, {16, 2, 4}, {32, 2, 4},
+                                        {32, 2, 2}, {32, 1, 4}, {32, 1, 2}};
+constexpr int kNConvp = 6;
 
 // The k_convp shape for cin input channels and nb output blocks: the first
-// whose weights and tile leave room for two workgroups per CU, else one; -1
-// if none fits.
+// whose output blocks divide nb (the rest in groups over the workgroups), at
+// most two of them, and whose weights and tile leave room for two
+// workgroups per CU, else one; -1 if none fits.  Measured at 1025^2: the
+// 64-channel layers on the 257^2 grid in two groups of 32 channels, ba 13.6
+// -> 12.2 us and bb 22.2 -> 19.0 us (their weights take half the LDS, so
+// more workgroups stay resident); one block per workgroup was slower for the
+// 32-channel layers (d1a 40 -> 46 us).
 int Cnn::convp_pick(int cin, int nb) const {
   if (cin % 16) return -1;
   const int cc = cin % 32 == 0 ? 32 : 16;
+  const int nb_max = 2;
   for (size_t cap : {lds_max / 2, lds_max})
-    for (int k = 0; k < 5; k++) {
-      if (cc == kConvpShapes[k][0] && nb == kConvpShapes[k][1] && convp_lds(cin, k) <= cap)
+    for (int k = 0; k < kNConvp; k++) {
+      const int N = kConvpShapes[k][1];
+      if (cc == kConvpShapes[k][0] && N <= nb_max && nb % N == 0 && convp_lds(cin, k) <= cap)
         return k;
     }
   return -1;
@@ -1501,20 +1519,24 @@ int Cnn::conv_fwd(const Layer& L, const TV& in, const TV& out, float* out32, int
       const size_t lds = convp_lds(cin, shape);
       // workgroups: what stays resident (kernel occupancy x CUs), whole XCD
       // groups, at most one per tile
-#define PP_CONVP(K, C, N, R)                                                             \
-  if (shape == K) {                                                                        \
-    int occ = 0;                                                                           \
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_convp<C, N, R>, 256, lds)); \
-    const int g = 8 * std::max(1, std::min((nt + 7) / 8, std::max(1, occ) * cu / 8));      \
-    hipLaunchKernelGGL((k_convp<C, N, R>), dim3(g), dim3(256), lds, s, A, nt);             \
-    PP_HIP(hipGetLastError());                                                             \
-    return PP_OK;                                                                          \
+      // output blocks beyond the shape's in groups, one per workgroup
+      A.groups = nb / kConvpShapes[shape][1];
+      const int gr = A.groups;
+#define PP_CONVP(K, C, N, R)                                                                  \
+  if (shape == K) {                                                                             \
+    int occ = 0;                                                                                \
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_convp<C, N, R>, 256, lds));      \
+    const int g = 8 * gr * std::max(1, std::min((nt + 7) / 8, std::max(1, occ) * cu / (8 * gr))); \
+    hipLaunchKernelGGL((k_convp<C, N, R>), dim3(g), dim3(256), lds, s, A, nt);                  \
+    PP_HIP(hipGetLastError());                                                                  \
+    return PP_OK;                                                                               \
   }
       PP_CONVP(0, 16, 1, 4)
       PP_CONVP(1, 16, 2, 4)
       PP_CONVP(2, 32, 2, 4)
       PP_CONVP(3, 32, 2, 2)
-      PP_CONVP(4, 32, 4, 2)
+      PP_CONVP(4, 32, 1, 4)
+      PP_CONVP(5, 32, 1, 2)
 #undef PP_CONVP
     }
     auto tiles_of = [&](int mb) { return ((W[lev] + 63) / 64) * ((H[lev] + mb - 1) / mb); };
@@ -1798,7 +1820,8 @@ int Cnn::setup(pp_plan* plan, const pp_cnn_config* cfg, uint64_t seed) {
                       (const void*)k_convf<32, 2, 4>, (const void*)k_convf<32, 4, 2>,
                       (const void*)k_convp<16, 1, 4>, (const void*)k_convp<16, 2, 4>,
                       (const void*)k_convp<32, 2, 4>, (const void*)k_convp<32, 2, 2>,
-                      (const void*)k_convp<32, 4, 2>, (const void*)k_conv2<kConv2Rows>};
+                      (const void*)k_convp<32, 1, 4>,
+                      (const void*)k_convp<32, 1, 2>, (const void*)k_conv2<kConv2Rows>};
   for (auto k : kf)
     PP_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
   // the forward pass without backward reads d1a's and d0a's concatenated
