@@ -122,6 +122,32 @@ class OracleProblem:
     def true_rel_residual(self, x, F):
         return float(np.linalg.norm(F - self.apply_k(x)) / np.linalg.norm(F))
 
+    def exact_rel_residual(self, x, F):
+        """|F - K x| / |F| with every product and sum exact (rationals): the
+        FP64 element matrices, x and F taken as the exact values they hold,
+        the constraint rows as in apply_k (y = x on constrained DOFs).  Only
+        the final square root and division round."""
+        from fractions import Fraction
+        nd = self.nd
+        Ke = self.Ke.reshape(self.ne, nd, nd)
+        el = self.elems.reshape(self.ne, self.npe)
+        xf = [Fraction(float(v)) for v in np.asarray(x, np.float64)]
+        y = [Fraction(0)] * (3 * self.nn)
+        for e in range(self.ne):
+            dof = [3 * int(n) + d for n in el[e] for d in range(3)]
+            xl = [xf[j] for j in dof]
+            K = [[Fraction(float(v)) for v in row] for row in Ke[e]]
+            for a in range(nd):
+                y[dof[a]] += sum(K[a][b] * xl[b] for b in range(nd))
+        num = Fraction(0)
+        den = Fraction(0)
+        for i, f in enumerate(np.asarray(F, np.float64)):
+            fi = Fraction(float(f))
+            ri = fi - (xf[i] if self.cdof[i] else y[i])
+            num += ri * ri
+            den += fi * fi
+        return float(np.sqrt(float(num / den)))
+
 
 def test_system():
     x = np.zeros(10)

@@ -102,6 +102,29 @@ def test_solve_c1_matches_oracles(ctx, path):
     assert O.max_nodal_w_error(x32, xj) < 1e-3
 
 
+@pytest.mark.parametrize("precond", [_lib.PRECOND_BLOCK_JACOBI, _lib.PRECOND_MULTIGRID])
+def test_device_true_residual_against_exact(ctx, precond):
+    """The device's FP64 evaluation of |F - K x| / |F|, which decides the
+    1e-8 stop and the residual-replacement restarts, against the exact
+    (rational) evaluation on the oracle's element matrices: within 1e-10 on
+    a 32 x 32 plate with log-normal thickness (t^3 spanning about 20x), for
+    the block-Jacobi and the multigrid solver."""
+    n = 32
+    mesh = P.rect_mesh(n)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    rng = np.random.default_rng(11)
+    et = (0.1 * np.exp(0.5 * rng.standard_normal(n * n))).astype(np.float32)
+    plan = Plan(ctx, mesh, E, NU, 0.1, mask, element_t=et)
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask, element_t=et)
+    F = plan.pressure_load(1e4, np.array([[3e4, 4.0, 6.0, 1.5]]))
+    x64, _, st = plan.solve(F, tolerance=1e-8, max_iterations=20000, precond=precond)
+    assert st["converged"]
+    exact = op.exact_rel_residual(x64, F.astype(np.float64))
+    print("device", st["trueRelResidual"], "exact", exact)
+    assert exact <= 1.0001e-8
+    assert abs(st["trueRelResidual"] - exact) < 1e-10, (st["trueRelResidual"], exact)
+
+
 def test_solve_c2_matches_golden(ctx):
     """61,347 DOF: FP64 solve vs the FP64 oracle and the float32 solver.ts
     restatement (golden vectors)."""

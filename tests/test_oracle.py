@@ -256,6 +256,39 @@ def test_zero_rhs_and_breakdown_rules():
     assert not st["converged"] and st["iterations"] == 1
 
 
+def test_fp64_residual_evaluation_against_exact():
+    """The 1e-8 stopping test is decided by an FP64 evaluation of
+    |F - K x| / |F|.  On a 24 x 24 plate with log-normal thickness solved to
+    1e-8, that evaluation agrees with the exact (rational) one to 1e-12 of
+    |F|, far below the tolerance.  The exact evaluator is pinned first: on a
+    random x it agrees with an independent 80-bit (numpy longdouble)
+    assembly of K x to 1e-17, and x = 0 gives exactly 1."""
+    n = 24
+    mesh = P.rect_mesh(n)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    rng = np.random.default_rng(3)
+    et = (0.1 * np.exp(0.5 * rng.standard_normal(n * n))).astype(np.float32)
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask, element_t=et)
+    F = op.pressure_load(1e4, np.array([[3e4, 3.0, 6.0, 1.5]]))
+    assert op.exact_rel_residual(np.zeros_like(F), F) == 1.0
+    # independent extended-precision assembly: sum of Ke x_e in longdouble
+    xr = rng.standard_normal(F.size) * 1e-4
+    Ke = op.Ke.reshape(op.ne, 12, 12).astype(np.longdouble)
+    dof = (3 * op.elems.reshape(op.ne, 4)[:, :, None].astype(np.int64) + np.arange(3)).reshape(op.ne, 12)
+    y = np.zeros(F.size, np.longdouble)
+    np.add.at(y, dof, np.einsum("eab,eb->ea", Ke, xr.astype(np.longdouble)[dof]))
+    y[op.cdof == 1] = xr[op.cdof == 1]
+    r = F.astype(np.longdouble) - y
+    ref = float(np.sqrt(np.sum(r * r) / np.sum(F.astype(np.longdouble) ** 2)))
+    assert abs(op.exact_rel_residual(xr, F) - ref) <= 1e-17 * ref
+    x, st = op.pcg(F, tol=1e-8)
+    assert st["converged"]
+    exact = op.exact_rel_residual(x, F)
+    fp64 = op.true_rel_residual(x, F)
+    assert 1e-10 < exact <= 1.01e-8
+    assert abs(fp64 - exact) < 1e-12, (fp64, exact)
+
+
 # ----------------------------------------------------- closed-form plates
 
 @pytest.mark.parametrize("support,coef", [("fixed", 0.00126), ("pinned", 0.00406)])
