@@ -176,10 +176,42 @@ __device__ __forceinline__ double w1(int f, int c, int n) {
   return ((f & 1) && f < n && (d == 1 || d == -1)) ? 0.5 : 0.0;
 }
 
-// P x_c at fine node (fi, fj): bilinear in index space over the coarse
-// element's corners (fine node f sits between coarse nodes f/2 and (f+1)/2).
-__device__ __forceinline__ void prolong_node(int fi, int fj, int fnx, int fny, const double* xc,
-                                             int cnx, double (&s)[3]) {
+// The coarse nodes a staged fine block of PW x PW nodes from (fi0, fj0)
+// interpolates from: CP x CP nodes from (fi0 / 2, fj0 / 2) (fi0, fj0 >= 0).
+template <int PW>
+struct CoarseBlock {
+  static constexpr int CP = PW / 2 + 2;
+};
+
+// Stage the coarse solution's block into LDS (SoA xc_s[d * CP^2 + local],
+// zero past the coarse grid), so that prolonging the staged fine nodes reads
+// LDS instead of up to four coarse nodes from memory per fine node.
+template <int PW, int NT, bool COH>
+__device__ __forceinline__ void stage_coarse(int fi0, int fj0, const double* xc, int cnx,
+                                             int cny, double* xc_s) {
+  constexpr int CP = CoarseBlock<PW>::CP;
+  const int ci0 = fi0 / 2, cj0 = fj0 / 2;
+  for (int idx = threadIdx.x; idx < CP * CP; idx += NT) {
+    const int ci = ci0 + idx % CP, cj = cj0 + idx / CP;
+    const bool ok = ci <= cnx && cj <= cny;
+    const int64_t m = (int64_t)(ok ? cj : 0) * (cnx + 1) + (ok ? ci : 0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double v;
+      if constexpr (COH) v = ld_coh(xc + 3 * m + d);
+      else v = xc[3 * m + d];
+      xc_s[d * CP * CP + idx] = ok ? v : 0.0;
+    }
+  }
+}
+
+// P xc at fine node (fi, fj), bilinear in index space over the coarse
+// element's corners (fine node f sits between coarse nodes f/2 and
+// (f+1)/2), read from the staged coarse block.
+template <int PW>
+__device__ __forceinline__ void prolong_node_lds(int fi, int fj, int fnx, int fny, int fi0,
+                                                 int fj0, const double* xc_s, double (&s)[3]) {
+  constexpr int CP = CoarseBlock<PW>::CP;
   const int i0 = fi / 2, i1 = (fi + 1) / 2, j0 = fj / 2, j1 = (fj + 1) / 2;
   s[0] = s[1] = s[2] = 0.0;
   const int ci[2] = {i0, i1}, cj[2] = {j0, j1};
@@ -187,8 +219,8 @@ __device__ __forceinline__ void prolong_node(int fi, int fj, int fnx, int fny, c
   for (int b = 0; b < nj; b++)
     for (int a = 0; a < ni; a++) {
       const double w = w1(fi, ci[a], fnx) * w1(fj, cj[b], fny);
-      const int64_t m = (int64_t)cj[b] * (cnx + 1) + ci[a];
-      for (int d = 0; d < 3; d++) s[d] += w * xc[3 * m + d];
+      const int l = (cj[b] - fj0 / 2) * CP + (ci[a] - fi0 / 2);
+      for (int d = 0; d < 3; d++) s[d] += w * xc_s[d * CP * CP + l];
     }
 }
 
@@ -197,8 +229,13 @@ __device__ __forceinline__ void prolong_node(int fi, int fj, int fnx, int fny, c
 template <int TS, int H, int NT>
 __device__ __forceinline__ void stage(int i0, int j0, int nx, int ny, const double* src,
                                       const double* y, double beta, const double* xc, int cnx,
-                                      const uint8_t* mask, double* u_s) {
+                                      const uint8_t* mask, double* u_s, double* xc_s = nullptr) {
   using T = TG<TS, H>;
+  const int fi0 = max(i0 - H, 0), fj0 = max(j0 - H, 0);
+  if (xc) {
+    stage_coarse<T::PW, NT, false>(fi0, fj0, xc, cnx, (ny + 1) / 2, xc_s);
+    __syncthreads();
+  }
   for (int idx = threadIdx.x; idx < T::NP; idx += NT) {
     const int lx = idx % T::PW, ly = idx / T::PW;
     const int gi = i0 - H + lx, gj = j0 - H + ly;
@@ -210,7 +247,7 @@ __device__ __forceinline__ void stage(int i0, int j0, int nx, int ny, const doub
         for (int d = 0; d < 3; d++) v[d] += beta * y[3 * n + d];
       if (xc) {
         double s[3];
-        prolong_node(gi, gj, nx, ny, xc, cnx, s);
+        prolong_node_lds<T::PW>(gi, gj, nx, ny, fi0, fj0, xc_s, s);
         const unsigned msk = mask[n];
         for (int d = 0; d < 3; d++)
           if (!(msk >> d & 1)) v[d] += s[d];
@@ -304,6 +341,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   __shared__ uint8_t m_s[MODE == M_RES ? T::NC : 1];
   __shared__ double red_s[2 * (NT / 64) + 2];
   __shared__ double in_s[2];
+  __shared__ double xc_s[PRO ? 3 * CoarseBlock<T::PW>::CP * CoarseBlock<T::PW>::CP : 1];
   if (A.S->done) return;
   const int nx = A.L.nx, ny = A.L.ny;
   const int ntx = (nx + 1 + TS - 1) / TS;
@@ -362,7 +400,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   double cv[1] = {0.0};
   if (A.cin && threadIdx.x < 64) wave_sum_parts<1>(A.cin, A.cn, cv);
   stage<TS, H, NT>(i0, j0, nx, ny, A.x, nullptr, 0.0, PRO ? A.pro.xc : nullptr, A.pro.cnx,
-                   A.L.mask, u_s);
+                   A.L.mask, u_s, xc_s);
   if (MODE == M_KP)
     stage<TS, H, NT>(i0, j0, nx, ny, A.y, nullptr, 0.0, nullptr, 0, A.L.mask, y_s);
   if (threadIdx.x == 0) in_s[0] = cv[0];
@@ -516,6 +554,7 @@ __device__ __forceinline__ void stv(double* p, double v) {
 struct StSmem {
   double u[3 * 18 * 18];
   double t[3 * 16 * 16];
+  double xc[3 * CoarseBlock<18>::CP * CoarseBlock<18>::CP];  // PRO: the coarse block
   uint8_t m[16 * 16];
 };
 
@@ -563,6 +602,11 @@ __device__ __forceinline__ void st_tile(const TileArgs& A, int tile, StSmem& sm)
     }
   }
   // stage x (+ P xc, masked) on the tile and its halo
+  const int fi0 = max(i0 - H, 0), fj0 = max(j0 - H, 0);
+  if (PRO) {
+    stage_coarse<T::PW, 256, COH>(fi0, fj0, A.pro.xc, A.pro.cnx, (ny + 1) / 2, sm.xc);
+    __syncthreads();
+  }
   for (int idx = threadIdx.x; idx < T::NP; idx += 256) {
     const int lx = idx % T::PW, ly = idx / T::PW;
     const int hi = i0 - H + lx, hj = j0 - H + ly;
@@ -571,16 +615,8 @@ __device__ __forceinline__ void st_tile(const TileArgs& A, int tile, StSmem& sm)
       const int64_t m = (int64_t)hj * (nx + 1) + hi;
       for (int d = 0; d < 3; d++) v[d] = ldv<COH>(A.x + 3 * m + d);
       if (PRO) {
-        const int q0 = hi / 2, q1 = (hi + 1) / 2, r0 = hj / 2, r1 = (hj + 1) / 2;
-        double sp[3] = {0.0, 0.0, 0.0};
-        const int ci[2] = {q0, q1}, cj[2] = {r0, r1};
-        const int ni = q0 == q1 ? 1 : 2, nj = r0 == r1 ? 1 : 2;
-        for (int bb = 0; bb < nj; bb++)
-          for (int aa = 0; aa < ni; aa++) {
-            const double w = w1(hi, ci[aa], nx) * w1(hj, cj[bb], ny);
-            const int64_t mc = (int64_t)cj[bb] * (A.pro.cnx + 1) + ci[aa];
-            for (int d = 0; d < 3; d++) sp[d] += w * ldv<COH>(A.pro.xc + 3 * mc + d);
-          }
+        double sp[3];
+        prolong_node_lds<T::PW>(hi, hj, nx, ny, fi0, fj0, sm.xc, sp);
         const unsigned msk = A.L.mask[m];
         for (int d = 0; d < 3; d++)
           if (!(msk >> d & 1)) v[d] += sp[d];
