@@ -2015,9 +2015,7 @@ int Cnn::precondition(const double* r, double* out, bool line_search, bool keep)
   int rc;
   const int64_t nd = pl->n_dof;
   if ((rc = smooth(r, nullptr, z0v.p))) return rc;
-  if ((rc = struct_apply_k(pl, z0v.p, kvec.p))) return rc;
-  hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, r, -1.0,
-                     (const double*)kvec.p, r1v.p);
+  if ((rc = struct_sweep(pl, 2, z0v.p, r, nullptr, 0.0, 0.0, r1v.p))) return rc;  // r - K z0
   if ((rc = apply(r1v.p, mout.p, keep))) return rc;
   if ((rc = struct_apply_k(pl, mout.p, kvec.p))) return rc;  // K u
   double* g = scal.p + 10;
@@ -2067,9 +2065,8 @@ int Cnn::loss_grad(const double* e, double* loss_dev) {
   hipLaunchKernelGGL(k_loss, dim3(1), dim3(64), 0, stream, (const double*)scal.p + 2, loss_dev,
                      scal.p + 4);  // scal[4] = 2 / eKe
   if ((rc = smooth(kvec.p, nullptr, z0v.p))) return rc;     // S q1
-  if ((rc = struct_apply_k(pl, z0v.p, z1v.p))) return rc;  // K S q1
-  hipLaunchKernelGGL(k_lin, dim3(nblk(nd)), dim3(256), 0, stream, nd, (const double*)kvec.p,
-                     -1.0, (const double*)z1v.p, r1v.p);  // E^T K e_f
+  // E^T K e_f = q1 - K S q1
+  if ((rc = struct_sweep(pl, 2, z0v.p, kvec.p, nullptr, 0.0, 0.0, r1v.p))) return rc;
   // dL/do = -(2 / eKe) S^T K e_f * s / scale_c
   hipLaunchKernelGGL(k_lin, dim3(1), dim3(1), 0, stream, (int64_t)1, (const double*)scal.p + 15,
                      -1.0, (const double*)scal.p + 4, scal.p + 5);  // scal[5] = 0 - scal[4]
