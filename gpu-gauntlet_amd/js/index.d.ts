@@ -121,3 +121,29 @@ export declare function solvePlateGPU(
   constrainedDOFs: Set<number>,
   options?: GPUSolverOptions
 ): Promise<PlateResultGPU>;
+
+/** PlateGeometry (types.ts): outer boundary and holes as flat [x0, y0, x1, y1, ...]. */
+export interface PlateGeometry {
+  boundary: Float32Array;
+  holes: Float32Array[];
+}
+
+export interface UnstructuredMeshOptions {
+  /** Spacing of the Steiner lattice (mesher-unstructured.ts:97-146); none: polygon vertices only. */
+  targetEdgeLength?: number;
+  /** Extension: rotate each triangle's vertex order (still counter-clockwise) so that no node is
+   *  only ever a third vertex, which the triangle as written would leave singular (default true). */
+  balanceVertexOrder?: boolean;
+}
+
+/** generateUnstructuredMesh (mesher-unstructured.ts:36-78): constrained Delaunay triangles of the
+ *  boundary minus the holes (this library's triangulator in place of poly2tri). */
+export declare function generateUnstructuredMesh(
+  geometry: PlateGeometry,
+  options?: UnstructuredMeshOptions
+): PlateMesh;
+
+/** generateCirclePolygon (mesher-utils.ts:122-137). */
+export declare function generateCirclePolygon(
+  cx: number, cy: number, radius: number, segments?: number
+): Float32Array;

@@ -208,8 +208,11 @@ async function solveGPU(mesh, material, coloring, F, constrainedDOFs, options) {
   }
 }
 
+// the unstructured mesher (mesher-unstructured.ts:36-78) for the coloured path
+const { generateUnstructuredMesh, generateCirclePolygon } = require('./mesher');
+
 module.exports = {
   solveGPU, solvePlateGPU, extractVerticalDisplacements, isGPUAvailable, PlateSolver,
-  destroyContext, constrainedMask, packColouring,
+  destroyContext, constrainedMask, packColouring, generateUnstructuredMesh, generateCirclePolygon,
   PRECISION, PRECOND, PATH, FIX,
 };

@@ -343,3 +343,44 @@ def test_unstructured_triangulation_coloured_path(balance):
         np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-7 * np.abs(want).max())
     plan.close()
     ctx.close()
+
+
+def test_plate_with_holes_from_the_mesher():
+    """A plate with a circular and a quadrilateral hole, meshed by the Node
+    host's generateUnstructuredMesh (constrained Delaunay, Steiner lattice
+    of mesher-unstructured.ts:97-146), greedy-coloured (coloring.ts:92-177),
+    clamped on every boundary node (the outer edge and the holes' edges, as
+    the reference's boundaryNodes give them), solved on the coloured DKT
+    path: K.x, the pressure load, the converged solve and the moments equal
+    the oracle's."""
+    from test_mesher import circle, js_mesh, sampled_rect
+
+    boundary = sampled_rect(0.0, 0.0, 10.0, 6.0, 0.5)
+    holes = [circle(3.0, 3.0, 1.2, 40),
+             np.array([6.5, 2.0, 8.5, 2.6, 8.0, 4.5, 6.2, 4.0], np.float32)]
+    mesh = js_mesh(boundary, holes, 0.4)
+    ctx = Context(0)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    col = P.colour_greedy(mesh)
+    plan = Plan(ctx, mesh, E, NU, 0.05, mask, col)
+    assert not plan.structured
+    op = O.OracleProblem(mesh, E, NU, 0.05, mask, order=col["elements_by_colour"])
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(plan.n_dof)
+    assert rel(plan.apply_k(x), op.apply_k(x)) < 1e-12
+    patch = np.array([[2e4, 5.0, 1.5, 1.0]])
+    F = plan.pressure_load(1e4, patch)
+    np.testing.assert_allclose(F, op.pressure_load(1e4, patch).astype(np.float32), rtol=2e-7,
+                               atol=0)
+    x64, _, st = plan.solve(F, tolerance=1e-8, max_iterations=20000)
+    xo, so = op.pcg(F.astype(np.float64), tol=1e-8)
+    print("mesh", mesh["nodeCount"], "nodes", mesh["elementCount"], "triangles,",
+          col["colour_count"], "colours; iterations", st["iterations"], so["iterations"])
+    assert st["converged"] and so["converged"]
+    assert abs(st["iterations"] - so["iterations"]) <= 2
+    assert op.true_rel_residual(x64, F.astype(np.float64)) <= 1e-8
+    assert O.max_nodal_w_error(x64, xo) < 1e-6
+    for got, want in zip(plan.moments(x64), op.moments(x64)):
+        np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-7 * np.abs(want).max())
+    plan.close()
+    ctx.close()
