@@ -59,12 +59,15 @@ def parse():
                          "150 for c3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (profiler calibration)")
-    ap.add_argument("--probe", choices=["copy", "pcg", "mg142", "pcg1024", "mg1024", "unet1024"],
+    ap.add_argument("--probe", choices=["copy", "pcg", "mg142", "pcg1024", "mg1024", "unet1024",
+                                        "dkt400"],
                     help="only run a probe for the rocprofv3 runs: a 256 MiB copy, one "
                          "142 x 142 block-Jacobi solve, five 142 x 142 multigrid-preconditioned "
                          "solves, 1,500 block-Jacobi iterations on the 1024 x 1024 plate, its "
-                         "multigrid-preconditioned solve, or 20 applications of the learned "
-                         "preconditioner there (U-net forward + block-Jacobi smoothing)")
+                         "multigrid-preconditioned solve, 20 applications of the learned "
+                         "preconditioner there (U-net forward + block-Jacobi smoothing), or "
+                         "the 400 x 400 split-quad DKT plate on the coloured path (three solves "
+                         "with the persistent kernel, one with the launch-per-operation loop)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs (traffic: null)")
     ap.add_argument("--no-c5", action="store_true", help="default run without the 1024 x 1024 legs")
@@ -144,6 +147,26 @@ def probe_unet1024():
     for _ in range(20):
         net.apply(r)
     net.close()
+    plan.close()
+    ctx.close()
+
+
+def probe_dkt400():
+    """The 400 x 400 clamped plate split into 320,000 DKT triangles (160,801
+    nodes, greedy colouring), coloured path: three block-Jacobi solves to 1e-8
+    with the persistent kernel, one with the launch-per-operation loop."""
+    from platepcg import Context, Plan, _lib
+    from platepcg import problem as P
+    mesh = P.split_quads(P.rect_mesh(400))
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    ctx = Context(0)
+    plan = Plan(ctx, mesh, 210e9, 0.3, 0.1, mask, P.colour_greedy(mesh))
+    plan.set_rhs(plan.pressure_load(1e4))
+    for flags in (0, 0, 0, _lib.FLAG_COLOURED_HOST_LOOP):
+        st = plan.solve_resident(plan.options(tolerance=1e-8, max_iterations=5000,
+                                              path=_lib.PATH_COLOURED, flags=flags))
+        print(json.dumps({"flags": flags, "iterations": st.iterations,
+                          "device_ms": st.device_ms}), flush=True)
     plan.close()
     ctx.close()
 
@@ -542,6 +565,9 @@ def main():
         return
     if a.probe == "unet1024":
         probe_unet1024()
+        return
+    if a.probe == "dkt400":
+        probe_dkt400()
         return
     if a.workload == "c4":
         bench_batch(a)

@@ -373,14 +373,8 @@ int pp_solve_resident(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
   } else {
     return fail(PP_ERR_INVALID, "unknown precision");
   }
-  if (rc) return rc;
-  // the structured solvers measure the true residual themselves
-  if ((o->flags & PP_FLAG_NO_TRUE_RESIDUAL) || structured) return PP_OK;
-  double rel = 0;
-  rc = structured ? struct_true_residual(pl, pl->x.p, &rel) : general_true_residual(pl, pl->x.p, &rel);
-  if (rc) return rc;
-  st->true_rel_residual = rel;
-  return PP_OK;
+  // every solver measures the FP64 true residual itself
+  return rc;
 }
 
 int pp_get_solution(pp_plan* pl, double* x64, float* x32) {

@@ -5,10 +5,14 @@
 // not tensor grids (triangles, general quads) and as the parity path that
 // mirrors the reference's own scatter order.
 //
-// The PCG loop is driven from the host without per-iteration read-back: all
-// scalars (alpha, beta, |r|, the iteration counter and the stop flag) live on
-// the device, every kernel returns early once the stop flag is set, and the
-// host only polls the flag once per batch of iterations.
+// The FP64 PCG runs as one persistent kernel (k_gen_pcg): K x gathered per
+// node from the node's 3x3 blocks of K, assembled at set-up from the element
+// matrices and stored sliced-ELL (64 nodes per slice, coalesced), with a
+// grid barrier between the preconditioner and the gather and one all-reduce
+// per iteration.  The launch-per-operation loop (PP_FLAG_COLOURED_HOST_LOOP)
+// keeps all scalars (alpha, beta, |r|, the iteration counter and the stop
+// flag) on the device, every kernel returning early once the stop flag is
+// set, and the host polls the flag once per batch of iterations.
 
 #include <math.h>
 
@@ -417,6 +421,340 @@ __global__ __launch_bounds__(256) void k_pressure_csr(int64_t nn, int npe, const
   F[3 * n + 2] = 0.0f;
 }
 
+// ------------------------------------------------- persistent FP64 PCG
+//
+// The single-reduction (Chronopoulos-Gear) recurrence of pcg.ts:98-215 on
+// the coloured path, one persistent launch:
+//   A (per node):  p = u + b p, s = w + b s;  x += a p;  r -= a s;  u = M r
+//   -- grid barrier: every node's u visible --
+//   B (per node):  w = K u (gathered, y[c] = u[c] on constrained DOFs);
+//                  partials r.u, w.u, r.r
+//   -- all-reduce --  b = g'/g, pAp = d' - b g'/a, a = g'/pAp
+// s = K p is carried, not recomputed.  u is the only vector other
+// workgroups read (device-coherent stores and loads); x, r, p, s, w stay
+// with the thread that owns the node.  Iteration count, stopping rule and
+// breakdown rule as the launch-per-operation loop (pcg.ts:118-181).
+constexpr int kGenNT = 512;
+
+struct GenArgs {
+  int64_t nn;
+  const int64_t* slc;     // per 64-node slice: first slot, width (slots per node)
+  const uint32_t* col;    // slot -> neighbour node; slot (slice, k, lane) at first + 64 k + lane
+  const double* blk;      // 9 doubles per slot, entry j of slot (slice, k, lane) at
+                          //   9 first + 64 (9 k + j) + lane
+  const uint8_t* cmask;   // PP_FIX bits per node
+  const double* minv;
+  const double* F;
+  const double* r0;       // restart: residual F - K x of the x held in x (else null)
+  double bnorm_in;        // restart: |F|
+  double *x, *r, *p, *s, *w, *u;
+  double* part;           // 8 * grid tagged all-reduce slots, zeroed
+  GridSync* gs;
+  DevStats* st;
+  double tol;
+  int maxit;
+};
+
+// (K u) of node n's three rows: the node's blocks times the neighbours' u.
+// The 64 lanes of a wave hold the 64 nodes of one slice (node blocks start
+// at multiples of 64), so the slice's width and offset are wave-uniform and
+// every block and column load is coalesced.  Padding slots carry a zero
+// block and the node itself.
+__device__ __forceinline__ void gen_kx(const GenArgs& A, int64_t n, double (&v)[3]) {
+  const int sl = __builtin_amdgcn_readfirstlane((int)(n >> 6));
+  const int l = (int)(n & 63);
+  const int64_t first = A.slc[2 * (int64_t)sl];
+  const int width = (int)A.slc[2 * (int64_t)sl + 1];
+  const uint32_t* cp = A.col + first + l;
+  const double* bp = A.blk + 9 * first + l;
+  v[0] = v[1] = v[2] = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < width; k++) {
+    const uint32_t m = cp[64 * k];
+    const double* b = bp + (size_t)576 * k;
+    double bk[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) bk[j] = b[64 * j];
+    const double u0 = ld_coh(&A.u[3 * (size_t)m]), u1 = ld_coh(&A.u[3 * (size_t)m + 1]),
+                 u2 = ld_coh(&A.u[3 * (size_t)m + 2]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) v[d] += bk[3 * d] * u0 + bk[3 * d + 1] * u1 + bk[3 * d + 2] * u2;
+  }
+}
+
+// Slots of a node's row whose columns a thread keeps in registers.
+constexpr int kGenCols = 9;
+
+__global__ __launch_bounds__(kGenNT) void k_gen_pcg(GenArgs A) {
+  __shared__ double red[80];
+  const unsigned me = blockIdx.x, nwg = gridDim.x;
+  const int64_t n0 = (int64_t)me * kGenNT + threadIdx.x, step = (int64_t)nwg * kGenNT;
+  unsigned epoch = 0;
+  const bool timing = me == 0 && threadIdx.x == 0;
+  unsigned long long tk[4] = {0, 0, 0, 0};
+  auto allreduce = [&](double (&v)[3]) -> bool {
+    publish();
+    block_sum<3, kGenNT>(v, red + 16);
+    return grid_allreduce<3, 4>(v, A.part, A.gs, epoch, red, nwg, me);
+  };
+  auto barrier = [&]() -> bool {
+    publish();
+    __syncthreads();
+    double z[1] = {0.0};
+    return grid_allreduce<1, 4>(z, A.part, A.gs, epoch, red, nwg, me);
+  };
+
+  // The thread's first node n0 keeps its state (x, r, p, s, w, u), its
+  // block inverse, constraint bits and the columns of its first kGenCols
+  // slots in registers for the whole solve; further nodes (meshes larger
+  // than one pass of the grid) go through memory.  Every u load of the
+  // resident row is then independent of any other load.
+  const bool res = n0 < A.nn;
+  double X[3] = {0, 0, 0}, Rr[3] = {0, 0, 0}, Pp[3] = {0, 0, 0}, Ss[3] = {0, 0, 0};
+  double W[3] = {0, 0, 0}, U[3] = {0, 0, 0}, Mi[6] = {0, 0, 0, 0, 0, 0};
+  unsigned fix0 = 0;
+  uint32_t rc[kGenCols];
+  int64_t rfirst = 0;
+  int rwidth = 0;
+#pragma unroll
+  for (int k = 0; k < kGenCols; k++) rc[k] = 0;
+  if (res) {
+    const int sl = __builtin_amdgcn_readfirstlane((int)(n0 >> 6));
+    rfirst = A.slc[2 * (int64_t)sl];
+    rwidth = (int)A.slc[2 * (int64_t)sl + 1];
+#pragma unroll
+    for (int k = 0; k < kGenCols; k++)
+      rc[k] = k < rwidth ? A.col[rfirst + 64 * k + (n0 & 63)] : (uint32_t)n0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) Mi[k] = A.minv[6 * n0 + k];
+    fix0 = A.cmask[n0];
+  }
+  // K u of the resident node: the register columns' u and blocks need no
+  // other load first, then the slots beyond kGenCols from memory; slots
+  // past the width add nothing, so the sums equal gen_kx's
+  auto kx_res = [&](double (&v)[3]) {
+    const double* bp = A.blk + 9 * rfirst + (n0 & 63);
+    v[0] = v[1] = v[2] = 0.0;
+    // three slots at a time; slots past the row's width load slot 0's
+    // block (clamped, branch-free) and use zero
+#pragma unroll
+    for (int k0 = 0; k0 < kGenCols; k0 += 3) {
+      double um[3][3], bk[3][9];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int k = k0 + q;
+#pragma unroll
+        for (int d = 0; d < 3; d++) um[q][d] = ld_coh(&A.u[3 * (size_t)rc[k] + d]);
+        const double* b = bp + (size_t)576 * (k < rwidth ? k : 0);
+#pragma unroll
+        for (int j = 0; j < 9; j++) bk[q][j] = b[64 * j];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const bool live = k0 + q < rwidth;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double t = bk[q][3 * d] * um[q][0] + bk[q][3 * d + 1] * um[q][1] +
+                           bk[q][3 * d + 2] * um[q][2];
+          v[d] += live ? t : 0.0;
+        }
+      }
+    }
+    for (int k = kGenCols; k < rwidth; k++) {
+      const uint32_t m = A.col[rfirst + 64 * k + (n0 & 63)];
+      const double* b = bp + (size_t)576 * k;
+      double bk[9];
+#pragma unroll
+      for (int j = 0; j < 9; j++) bk[j] = b[64 * j];
+      const double u0 = ld_coh(&A.u[3 * (size_t)m]), u1 = ld_coh(&A.u[3 * (size_t)m + 1]),
+                   u2 = ld_coh(&A.u[3 * (size_t)m + 2]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) v[d] += bk[3 * d] * u0 + bk[3 * d + 1] * u1 + bk[3 * d + 2] * u2;
+    }
+  };
+  // w = K u on the own nodes, partials r.u, w.u, r.r
+  auto phase_b = [&](double (&acc)[3]) {
+    acc[0] = acc[1] = acc[2] = 0.0;
+    if (res) {
+      double v[3];
+      kx_res(v);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        W[d] = ((fix0 >> d) & 1u) ? U[d] : v[d];
+        acc[0] += Rr[d] * U[d];
+        acc[1] += W[d] * U[d];
+        acc[2] += Rr[d] * Rr[d];
+      }
+    }
+    for (int64_t n = n0 + step; n < A.nn; n += step) {
+      double v[3];
+      gen_kx(A, n, v);
+      const unsigned fix = A.cmask[n];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double u = ld_coh(&A.u[3 * n + d]), r = A.r[3 * n + d];
+        const double w = ((fix >> d) & 1u) ? u : v[d];
+        A.w[3 * n + d] = w;
+        acc[0] += r * u;
+        acc[1] += w * u;
+        acc[2] += r * r;
+      }
+    }
+  };
+
+  // r = F, x = 0 (restart: r = r0, x kept), u = M r
+  const bool restart = A.r0 != nullptr;
+  const double* src = restart ? A.r0 : A.F;
+  if (res) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      Rr[d] = src[3 * n0 + d];
+      X[d] = restart ? A.x[3 * n0 + d] : 0.0;
+    }
+    sym3_apply(Mi, Rr, U);
+#pragma unroll
+    for (int d = 0; d < 3; d++) st_coh(&A.u[3 * n0 + d], U[d]);
+  }
+  for (int64_t n = n0 + step; n < A.nn; n += step) {
+    double r[3], u[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      r[d] = src[3 * n + d];
+      A.r[3 * n + d] = r[d];
+      if (!restart) A.x[3 * n + d] = 0.0;
+    }
+    sym3_apply(A.minv + 6 * n, r, u);
+#pragma unroll
+    for (int d = 0; d < 3; d++) st_coh(&A.u[3 * n + d], u[d]);
+  }
+  if (!barrier()) return;
+  double acc[3];
+  phase_b(acc);
+  if (!allreduce(acc)) return;
+  double g = acc[0], pap = acc[1], rho = acc[2];
+  const double bnorm = restart ? A.bnorm_in : sqrt(rho);
+  int it = 0, converged = 0, breakdown = 0;
+  if (restart ? (sqrt(rho) < A.tol || sqrt(rho) / bnorm < A.tol) : bnorm < 1e-30) {
+    converged = 1;
+    if (!restart) rho = 0.0;
+  } else if (A.maxit > 0 && fabs(pap) < 1e-30) {
+    breakdown = 1;
+    it = 1;
+  } else {
+    double a = g / pap, b = 0.0;
+    bool first = true;
+    for (; it < A.maxit;) {
+      const unsigned long long t0 = timing ? wall_clock64() : 0;
+      // p = u + b p, s = w + b s; x += a p; r -= a s; u = M r
+      if (res) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          Pp[d] = first ? U[d] : fma(b, Pp[d], U[d]);
+          Ss[d] = first ? W[d] : fma(b, Ss[d], W[d]);
+          X[d] = fma(a, Pp[d], X[d]);
+          Rr[d] = fma(-a, Ss[d], Rr[d]);
+        }
+        sym3_apply(Mi, Rr, U);
+#pragma unroll
+        for (int d = 0; d < 3; d++) st_coh(&A.u[3 * n0 + d], U[d]);
+      }
+      for (int64_t n = n0 + step; n < A.nn; n += step) {
+        double r[3], u[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const size_t i = 3 * n + d;
+          const double uo = ld_coh(&A.u[i]), wo = A.w[i];
+          const double p = first ? uo : fma(b, A.p[i], uo);
+          const double s = first ? wo : fma(b, A.s[i], wo);
+          A.p[i] = p;
+          A.s[i] = s;
+          A.x[i] = fma(a, p, A.x[i]);
+          r[d] = fma(-a, s, A.r[i]);
+          A.r[i] = r[d];
+        }
+        sym3_apply(A.minv + 6 * n, r, u);
+#pragma unroll
+        for (int d = 0; d < 3; d++) st_coh(&A.u[3 * n + d], u[d]);
+      }
+      first = false;
+      const unsigned long long t1 = timing ? wall_clock64() : 0;
+      if (!barrier()) return;
+      const unsigned long long t2 = timing ? wall_clock64() : 0;
+      phase_b(acc);
+      const unsigned long long t3 = timing ? wall_clock64() : 0;
+      if (!allreduce(acc)) return;
+      if (timing) {
+        const unsigned long long t4 = wall_clock64();
+        tk[0] += t1 - t0; tk[1] += t2 - t1; tk[2] += t3 - t2; tk[3] += t4 - t3;
+      }
+      it++;
+      const double g1 = acc[0], d1 = acc[1];
+      rho = acc[2];
+      const double rn = sqrt(rho);
+      if (rn < A.tol || rn / bnorm < A.tol) { converged = 1; break; }
+      b = g1 / g;
+      pap = d1 - b * g1 / a;
+      if (fabs(pap) < 1e-30) { breakdown = 1; it++; break; }
+      a = g1 / pap;
+      g = g1;
+    }
+  }
+  if (res)
+#pragma unroll
+    for (int d = 0; d < 3; d++) A.x[3 * n0 + d] = X[d];
+  if (me == 0 && threadIdx.x == 0) {
+    A.st->iterations = it;
+    A.st->converged = converged;
+    A.st->breakdown = breakdown;
+    A.st->final_residual = sqrt(rho);
+    A.st->bnorm = bnorm;
+    for (int k = 0; k < 4; k++) A.st->ticks[k] = tk[k];
+  }
+}
+
+// Clears the barrier words, the statistics and the all-reduce slots with
+// stores (no memset node where a caller captures the stream).
+__global__ void k_gen_zero(GridSync* gs, DevStats* st, double* part, int64_t n_part) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_part) part[i] = 0.0;
+  if (i == 0) {
+    *st = DevStats{};
+    gs->count = 0;
+    gs->abort = 0;
+  }
+}
+
+// Node blocks of K for the persistent solve (one thread per node): for
+// every (element, corner) of the node, in ascending element order, each
+// corner block of the element's rows added into the slot of that corner's
+// node.  The diagonal block is summed in the order k_block_csr sums it.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_ell_fill(int64_t nn, const int64_t* off,
+                                                  const uint32_t* adj, const uint32_t* elems,
+                                                  const double* Ke, const int64_t* slc,
+                                                  const uint32_t* col, double* blk) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= nn) return;
+  constexpr int ND = 3 * NPE;
+  const int64_t sl = n >> 6, l = n & 63;
+  const int64_t first = slc[2 * sl];
+  const int width = (int)slc[2 * sl + 1];
+  for (int64_t k = off[n]; k < off[n + 1]; k++) {
+    const uint32_t ec = adj[k];
+    const uint32_t e = ec / NPE;
+    const int c = (int)(ec - e * NPE);
+    const double* K = Ke + (size_t)e * ND * ND;
+    for (int i = 0; i < NPE; i++) {
+      const uint32_t m = elems[(size_t)e * NPE + i];
+      int q = 0;
+      while (q < width - 1 && col[first + 64 * q + l] != m) q++;
+      double* b = blk + 9 * first + 576 * (int64_t)q + l;
+      for (int d = 0; d < 3; d++)
+        for (int j = 0; j < 3; j++) b[64 * (3 * d + j)] += K[(3 * c + d) * ND + 3 * i + j];
+    }
+  }
+}
+
 }  // namespace
 
 // Set-up on the device: every element's FP64 Ke, then the nodal block
@@ -436,6 +774,44 @@ int general_setup(pp_plan* pl, const pp_colouring* col) {
       for (int i = 0; i < npe; i++)
         adj[fill[pl->h_elems[(size_t)e * npe + i]]++] = (uint32_t)(e * npe + i);
   }
+  // node-block rows of the persistent solve: each node's neighbours (the
+  // nodes of its elements, itself included) in increasing order, in slices
+  // of 64 nodes padded to the slice's longest row with the node itself
+  const int64_t nsl = (nn + 63) / 64;
+  std::vector<int64_t> slc(2 * std::max<int64_t>(1, nsl), 0);
+  std::vector<uint32_t> ecol;
+  {
+    std::vector<std::vector<uint32_t>> nb(64);
+    int64_t slots = 0;
+    for (int64_t sl = 0; sl < nsl; sl++) {
+      size_t width = 0;
+      for (int l = 0; l < 64; l++) {
+        const int64_t n = sl * 64 + l;
+        std::vector<uint32_t>& v = nb[l];
+        v.clear();
+        if (n >= nn) continue;
+        for (int64_t k = off[n]; k < off[n + 1]; k++) {
+          const uint32_t e = adj[k] / (uint32_t)npe;
+          for (int i = 0; i < npe; i++) v.push_back(pl->h_elems[(size_t)e * npe + i]);
+        }
+        if (v.empty()) v.push_back((uint32_t)n);
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        width = std::max(width, v.size());
+      }
+      slc[2 * sl] = slots;
+      slc[2 * sl + 1] = (int64_t)width;
+      ecol.resize((size_t)(slots + 64 * (int64_t)width));
+      for (size_t k = 0; k < width; k++)
+        for (int l = 0; l < 64; l++) {
+          const int64_t n = sl * 64 + l;
+          ecol[(size_t)slots + 64 * k + l] =
+              k < nb[l].size() ? nb[l][k] : (uint32_t)(n < nn ? n : 0);
+        }
+      slots += 64 * (int64_t)width;
+    }
+    if (ecol.empty()) ecol.push_back(0);
+  }
   std::vector<uint8_t> cdof((size_t)nn * 3);
   for (int64_t n = 0; n < nn; n++)
     for (int d = 0; d < 3; d++) cdof[(size_t)n * 3 + d] = (pl->h_cmask[n] >> d) & 1u;
@@ -447,12 +823,17 @@ int general_setup(pp_plan* pl, const pp_colouring* col) {
       (rc = pl->cmask_dof.alloc(cdof.size())) || (rc = pl->elems.alloc(pl->h_elems.size())) ||
       (rc = pl->by_colour.alloc(std::max<size_t>(1, col->colour_offsets[col->colour_count]))) ||
       (rc = pl->scal.alloc(S_N)) || (rc = pl->partials.alloc(2 * kRedBlocks)) ||
-      (rc = nodes.alloc(2 * nn)) || (rc = doff.alloc(nn + 1)) || (rc = dadj.alloc(adj.size())))
+      (rc = nodes.alloc(2 * nn)) || (rc = doff.alloc(nn + 1)) || (rc = dadj.alloc(adj.size())) ||
+      (rc = pl->ell_slc.alloc(slc.size())) || (rc = pl->ell_col.alloc(ecol.size())) ||
+      (rc = pl->ell_blk.alloc(9 * ecol.size())))
     return rc;
   hipStream_t s = pl->ctx->stream;
   PP_HIP(hipMemcpyAsync(nodes.p, pl->h_nodes.data(), 2 * nn * 4, hipMemcpyHostToDevice, s));
   PP_HIP(hipMemcpyAsync(doff.p, off.data(), (nn + 1) * 8, hipMemcpyHostToDevice, s));
   PP_HIP(hipMemcpyAsync(dadj.p, adj.data(), adj.size() * 4, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(pl->ell_slc.p, slc.data(), slc.size() * 8, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemcpyAsync(pl->ell_col.p, ecol.data(), ecol.size() * 4, hipMemcpyHostToDevice, s));
+  PP_HIP(hipMemsetAsync(pl->ell_blk.p, 0, pl->ell_blk.n * 8, s));
   PP_HIP(hipMemcpyAsync(pl->cmask_dof.p, cdof.data(), cdof.size(), hipMemcpyHostToDevice, s));
   PP_HIP(hipMemcpyAsync(pl->elems.p, pl->h_elems.data(), pl->h_elems.size() * 4,
                         hipMemcpyHostToDevice, s));
@@ -473,6 +854,16 @@ int general_setup(pp_plan* pl, const pp_colouring* col) {
   hipLaunchKernelGGL(k_block_csr, gn, dim3(256), 0, s, nn, npe, (const int64_t*)doff.p,
                      (const uint32_t*)dadj.p, (const double*)pl->ke.p,
                      (const uint8_t*)pl->cmask.p, pl->minv.p);
+  if (npe == 4)
+    hipLaunchKernelGGL(k_ell_fill<4>, gn, dim3(256), 0, s, nn, (const int64_t*)doff.p,
+                       (const uint32_t*)dadj.p, (const uint32_t*)pl->elems.p,
+                       (const double*)pl->ke.p, (const int64_t*)pl->ell_slc.p,
+                       (const uint32_t*)pl->ell_col.p, pl->ell_blk.p);
+  else
+    hipLaunchKernelGGL(k_ell_fill<3>, gn, dim3(256), 0, s, nn, (const int64_t*)doff.p,
+                       (const uint32_t*)dadj.p, (const uint32_t*)pl->elems.p,
+                       (const double*)pl->ke.p, (const int64_t*)pl->ell_slc.p,
+                       (const uint32_t*)pl->ell_col.p, pl->ell_blk.p);
   PP_HIP(hipGetLastError());
   PP_HIP(hipStreamSynchronize(s));
   return PP_OK;
@@ -505,8 +896,109 @@ int general_true_residual(pp_plan* pl, const double* dx, double* rel) {
   return launch_norm_ratio(pl, pl->F.p, pl->tmp.p, rel);
 }
 
-// Host-driven device-resident PCG (pcg.ts:98-215), no per-iteration read-back.
+namespace {
+
+// One persistent launch (k_gen_pcg) sized to what is co-resident: one
+// 512-thread workgroup per CU at most, fewer on small meshes (cold start
+// when r0 is null).  If the workgroups cannot all run at once (another
+// process holds CUs) the barrier spin budget runs out and the call fails
+// with PP_ERR_TIMEOUT.
+int launch_gen_pcg(pp_plan* pl, const double* r0, double bnorm, double tol, int maxit,
+                   DevStats* ds) {
+  hipStream_t s = pl->ctx->stream;
+  const int64_t nn = pl->n_nodes;
+  int per_cu = 0;
+  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gen_pcg, kGenNT, 0));
+  const int64_t cap = (int64_t)std::min(per_cu, 1) * pl->ctx->cu_count;
+  const int grid = (int)std::min<int64_t>((nn + kGenNT - 1) / kGenNT, cap);
+  if (grid <= 0) return fail(PP_ERR_HIP, "persistent PCG kernel cannot be made resident");
+  const size_t npart = (size_t)8 * grid;
+  int rc;
+  if (pl->partials.n < npart && (rc = pl->partials.alloc(npart))) return rc;
+  GenArgs A;
+  A.nn = nn;
+  A.slc = (const int64_t*)pl->ell_slc.p;
+  A.col = (const uint32_t*)pl->ell_col.p;
+  A.blk = pl->ell_blk.p;
+  A.cmask = (const uint8_t*)pl->cmask.p;
+  A.minv = pl->minv.p;
+  A.F = pl->F.p;
+  A.r0 = r0;
+  A.bnorm_in = bnorm;
+  A.x = pl->x.p; A.r = pl->r.p; A.p = pl->p0.p; A.s = pl->Ap.p; A.w = pl->p1.p; A.u = pl->z.p;
+  A.part = pl->partials.p;
+  A.gs = pl->gsync.p;
+  A.st = pl->dstats.p;
+  A.tol = tol;
+  A.maxit = maxit;
+  hipLaunchKernelGGL(k_gen_zero, dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, s, pl->gsync.p,
+                     pl->dstats.p, pl->partials.p, (int64_t)npart);
+  hipLaunchKernelGGL(k_gen_pcg, dim3(grid), dim3(kGenNT), 0, s, A);
+  PP_HIP(hipGetLastError());
+  GridSync gsh;
+  PP_HIP(hipMemcpyAsync(ds, pl->dstats.p, sizeof(*ds), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipMemcpyAsync(&gsh, pl->gsync.p, sizeof(gsh), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipStreamSynchronize(s));
+  if (gsh.abort) return fail(PP_ERR_TIMEOUT, "grid barrier spin limit reached");
+  return PP_OK;
+}
+
+// The persistent solve, then |F - K x| / |F| in FP64 and, while it is above
+// the tolerance, up to four restarts from r = F - K x (residual
+// replacement), as the structured solver does.
+int general_solve_persistent(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  constexpr int kMaxRestarts = 4;
+  hipStream_t s = pl->ctx->stream;
+  PP_HIP(hipEventRecord(pl->ctx->ev0, s));
+  DevStats ds;
+  int rc = launch_gen_pcg(pl, nullptr, 0.0, o->tolerance, o->max_iterations, &ds);
+  if (rc) return rc;
+  const double bnorm = ds.bnorm;
+  int total = ds.iterations;
+  for (int k = 0; k < 4; k++) st->phase_us[k] = 0.01 * (double)ds.ticks[k];
+  st->sweeps = 1;
+  st->sweep_iterations[0] = ds.iterations;
+  double rel = 0;
+  if (ds.converged && bnorm > 0 && !(o->flags & PP_FLAG_NO_TRUE_RESIDUAL)) {
+    for (int k = 0;; k++) {
+      if ((rc = apply_all(pl, pl->x.p, pl->tmp.p, nullptr))) return rc;
+      double res = 0, fn = 0;
+      if ((rc = launch_norms(pl, pl->F.p, pl->tmp.p, &res, &fn))) return rc;
+      rel = fn > 0 ? res / fn : 0.0;
+      st->sweep_residual[std::min(st->sweeps, PP_MAX_SWEEPS) - 1] = rel;
+      if (rel <= o->tolerance || k == kMaxRestarts || total >= o->max_iterations) break;
+      if ((rc = launch_residual(pl, pl->F.p, pl->tmp.p, pl->tmp.p))) return rc;
+      rc = launch_gen_pcg(pl, pl->tmp.p, bnorm, o->tolerance, o->max_iterations - total, &ds);
+      if (rc) return rc;
+      total += ds.iterations;
+      for (int k = 0; k < 4; k++) st->phase_us[k] += 0.01 * (double)ds.ticks[k];
+      if (st->sweeps < PP_MAX_SWEEPS) st->sweep_iterations[st->sweeps] = ds.iterations;
+      st->sweeps++;
+      if (!ds.converged) break;
+    }
+  }
+  PP_HIP(hipEventRecord(pl->ctx->ev1, s));
+  PP_HIP(hipEventSynchronize(pl->ctx->ev1));
+  float ms = 0;
+  PP_HIP(hipEventElapsedTime(&ms, pl->ctx->ev0, pl->ctx->ev1));
+  st->iterations = total;
+  st->converged = ds.converged;
+  st->breakdown = ds.breakdown;
+  st->final_residual = ds.final_residual;
+  st->true_rel_residual = rel;
+  st->device_ms = ms;
+  return PP_OK;
+}
+
+}  // namespace
+
+// FP64 PCG on the coloured path (pcg.ts:98-215), no per-iteration read-back:
+// the persistent kernel with residual replacement, or with
+// PP_FLAG_COLOURED_HOST_LOOP the launch-per-operation loop (the recursive
+// residual alone decides); both report the FP64 true residual unless
+// PP_FLAG_NO_TRUE_RESIDUAL.
 int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) {
+  if (!(o->flags & PP_FLAG_COLOURED_HOST_LOOP)) return general_solve_persistent(pl, o, st);
   hipStream_t s = pl->ctx->stream;
   const int64_t nd = pl->n_dof, nn = pl->n_nodes;
   double* sc = pl->scal.p;
@@ -560,6 +1052,13 @@ int general_solve_fp64(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats* st) 
   st->device_ms = ms;
   st->sweeps = 1;
   st->sweep_iterations[0] = st->iterations;
+  if (!(o->flags & PP_FLAG_NO_TRUE_RESIDUAL)) {
+    double rel = 0;
+    int rc = general_true_residual(pl, pl->x.p, &rel);
+    if (rc) return rc;
+    st->true_rel_residual = rel;
+    st->sweep_residual[0] = rel;
+  }
   return PP_OK;
 }
 

@@ -130,6 +130,9 @@ struct pp_plan {
   pp::DevBuf<float> gen_nodes;        // coloured path: f32 coordinates
   pp::DevBuf<int64_t> gen_off;        // coloured path: node -> (element, corner) list,
   pp::DevBuf<uint32_t> gen_adj;       //   CSR, entries element * npe + corner
+  pp::DevBuf<int64_t> ell_slc;        // coloured path: K as 3x3 node blocks, sliced ELL
+  pp::DevBuf<uint32_t> ell_col;       //   (64 nodes per slice): slice offsets and widths,
+  pp::DevBuf<double> ell_blk;         //   neighbour per slot, 9 doubles per slot
 
   // solve state (FP64)
   pp::DevBuf<double> F, x, r, z, p0, p1, Ap, tmp;

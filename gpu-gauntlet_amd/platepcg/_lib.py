@@ -28,6 +28,7 @@ FLAG_SMALL_TILES = 2
 FLAG_CLASSIC_PCG = 4
 FLAG_RELIABLE_UPDATE = 8
 FLAG_MG_UNFUSED = 16
+FLAG_COLOURED_HOST_LOOP = 32
 
 
 class Mesh(C.Structure):

@@ -121,6 +121,11 @@ typedef struct {
  * launches (19), for comparison; same arithmetic up to the order of the
  * dot-product sums. */
 #define PP_FLAG_MG_UNFUSED 16
+/* Coloured path: the launch-per-operation PCG loop (a launch per colour of
+ * every K apply, device scalars, the host polling a stop flag every 64
+ * iterations) instead of the persistent kernel, which applies K from 3x3
+ * node blocks assembled at set-up; the same iterates up to rounding. */
+#define PP_FLAG_COLOURED_HOST_LOOP 32
 
 #define PP_MAX_SWEEPS 32
 
@@ -139,7 +144,8 @@ typedef struct {
   /* Wall time (us) workgroup 0 of the resident persistent kernel spent in
    * each part of an iteration, summed over iterations: halo/stage, element
    * apply, node sums, all-reduce of p.Ap, vector updates, all-reduce of
-   * r.z and r.r.  Zero where not measured. */
+   * r.z and r.r; on the coloured path's persistent kernel: vector updates
+   * and M r, grid barrier, K u, all-reduce.  Zero where not measured. */
   double phase_us[6];
 } pp_solve_stats;
 

@@ -384,3 +384,49 @@ def test_plate_with_holes_from_the_mesher():
         np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-7 * np.abs(want).max())
     plan.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("case", ["q4", "dkt", "unstructured", "dkt_strided"])
+def test_coloured_persistent_kernel_equals_launch_loop(ctx, case):
+    """The coloured path's persistent PCG kernel (K x gathered per node in
+    colour order, one grid barrier and one all-reduce per iteration) against
+    the launch-per-operation loop (a launch per colour, device scalars,
+    PP_FLAG_COLOURED_HOST_LOOP): the same first iterates and the same
+    converged answer, and both against the oracle.  dkt_strided has more
+    nodes (160,801) than one pass of the grid covers, so every workgroup
+    strides over several node blocks."""
+    loop = _lib.FLAG_COLOURED_HOST_LOOP
+    if case == "unstructured":
+        mesh = O.unstructured_triangles(20, seed=11, jitter=0.3, balance=True)
+    else:
+        mesh = P.rect_mesh({"q4": 32, "dkt": 64, "dkt_strided": 400}[case])
+        if case != "q4":
+            mesh = P.split_quads(mesh)
+    mask = P.constrained_mask(mesh, [{"type": "fixed", "location": "all_edges"}])
+    col = P.colour_greedy(mesh) if case != "q4" else P.colour_structured(mesh)
+    plan = Plan(ctx, mesh, E, NU, 0.1, mask, col)
+    path = _lib.PATH_COLOURED
+    F = plan.pressure_load(1e4, np.array([[3e4, 4.0, 6.0, 1.5]]))
+    for k in (1, 5, 40):
+        xa, _, sa = plan.solve(F, tolerance=1e-30, max_iterations=k, path=path,
+                               flags=_lib.FLAG_NO_TRUE_RESIDUAL)
+        xb, _, sb = plan.solve(F, tolerance=1e-30, max_iterations=k, path=path,
+                               flags=loop | _lib.FLAG_NO_TRUE_RESIDUAL)
+        assert sa["iterations"] == sb["iterations"] == k
+        assert rel(xa, xb) < 1e-9, (k, rel(xa, xb))
+    if case == "dkt_strided":
+        plan.close()
+        return
+    xa, _, sa = plan.solve(F, tolerance=1e-8, max_iterations=20000, path=path)
+    xb, _, sb = plan.solve(F, tolerance=1e-8, max_iterations=20000, path=path, flags=loop)
+    print(case, mesh["nodeCount"], "nodes: iterations", sa["iterations"], sb["iterations"],
+          "ms", sa["deviceMs"], sb["deviceMs"])
+    assert sa["converged"] and sb["converged"]
+    assert abs(sa["iterations"] - sb["iterations"]) <= 2
+    assert sa["trueRelResidual"] <= 1e-8
+    assert O.max_nodal_w_error(xa, xb) < 1e-6
+    op = O.OracleProblem(mesh, E, NU, 0.1, mask, order=col["elements_by_colour"])
+    xo, so = op.pcg(F.astype(np.float64), tol=1e-8)
+    assert abs(sa["iterations"] - so["iterations"]) <= 2
+    assert O.max_nodal_w_error(xa, xo) < 1e-6
+    plan.close()
