@@ -382,9 +382,22 @@ __global__ __launch_bounds__(kNT) void k_batch_pcg(const PlateDesc* descs, const
           for (int d = 0; d < 3; d++) {
             const double pd = p_s[d * kMaxNP + c];
             const double ad = ((fix >> d) & 1u) ? pd : v[d];
-            st_coh(&pnew[3 * n + d], pd);
             D.Ap[3 * n + d] = ad;
             pap[0] += pd * ad;
+          }
+        }
+        // p of the tile's own nodes goes out row by row with consecutive
+        // lanes on consecutive doubles: a device-coherent store is written
+        // through per instruction, so node-strided stores (lane = node, one
+        // instruction per component) would write each line three times
+        {
+          const int rw = min(D.TX, D.nx + 1 - i0), rows = min(D.TY, D.ny + 1 - j0);
+          const int W3 = 3 * D.TX;
+          for (int t = threadIdx.x; t < W3 * rows; t += kNT) {
+            const int r = t / W3, k = t % W3;
+            if (k < 3 * rw)
+              st_coh(&pnew[3 * ((size_t)(j0 + r) * (D.nx + 1) + i0) + k],
+                     p_s[(k % 3) * kMaxNP + (r + 1) * PW + k / 3 + 1]);
           }
         }
         __syncthreads();
@@ -393,23 +406,40 @@ __global__ __launch_bounds__(kNT) void k_batch_pcg(const PlateDesc* descs, const
       if (fabs(pap[0]) < 1e-30) { breakdown = 1; it++; break; }
       const double alpha = rz / pap[0];
       acc[0] = 0; acc[1] = 0;
+      // p comes in and z goes out through LDS row by row (whole lines per
+      // coherent access); the node's thread works on its own entries
       for (int tile = me; tile < D.ntiles; tile += nwg) {
-        const int gi = (tile % D.ntx) * D.TX + tx, gj = (tile / D.ntx) * D.TY + ty;
+        const int i0 = (tile % D.ntx) * D.TX, j0 = (tile / D.ntx) * D.TY;
+        const int rw = min(D.TX, D.nx + 1 - i0), rows = min(D.TY, D.ny + 1 - j0);
+        const int W3 = 3 * D.TX;
+        for (int t = threadIdx.x; t < W3 * rows; t += kNT) {
+          const int r = t / W3, k = t % W3;
+          if (k < 3 * rw) ye_s[t] = ld_coh(&pnew[3 * ((size_t)(j0 + r) * (D.nx + 1) + i0) + k]);
+        }
+        __syncthreads();
+        const int gi = i0 + tx, gj = j0 + ty;
         if (node_thread && gi <= D.nx && gj <= D.ny) {
           const int64_t n = (int64_t)gj * (D.nx + 1) + gi;
+          double* own = ye_s + 3 * (ty * D.TX + tx);
           double rv[3], zv[3];
           for (int d = 0; d < 3; d++) {
-            D.x[3 * n + d] += alpha * ld_coh(&pnew[3 * n + d]);
+            D.x[3 * n + d] += alpha * own[d];
             rv[d] = D.r[3 * n + d] - alpha * D.Ap[3 * n + d];
             D.r[3 * n + d] = rv[d];
           }
           sym3_apply(D.minv + 6 * n, rv, zv);
           for (int d = 0; d < 3; d++) {
-            st_coh(&D.z[3 * n + d], zv[d]);
+            own[d] = zv[d];
             acc[0] += rv[d] * zv[d];
             acc[1] += rv[d] * rv[d];
           }
         }
+        __syncthreads();
+        for (int t = threadIdx.x; t < W3 * rows; t += kNT) {
+          const int r = t / W3, k = t % W3;
+          if (k < 3 * rw) st_coh(&D.z[3 * ((size_t)(j0 + r) * (D.nx + 1) + i0) + k], ye_s[t]);
+        }
+        __syncthreads();
       }
       if (!allreduce(acc)) return;
       rr = acc[1];
