@@ -133,6 +133,7 @@ __device__ __forceinline__ void elem_scales(const KGrid& G, int i0, int j0, bool
 // loaded before that prefetch, because loads complete in issue order.
 template <class C>
 __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
+  static_assert(3 * C::TX * C::TY <= 12 * C::NE, "phase B stages a tile's p and z in ye_s");
   __shared__ float p_s[3 * C::NP];
   __shared__ float ye_s[12 * C::NE];
   __shared__ double red[80];
@@ -257,9 +258,21 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
           for (int d = 0; d < 3; d++) {
             const float pd = p_s[d * C::NP + c];
             const float ad = ((fix >> d) & 1u) ? pd : v[d];
-            st_coh(&pnew[3 * n + d], pd);
             A.Ap[3 * n + d] = ad;
             pap[0] += (double)pd * (double)ad;
+          }
+        }
+        // p of the tile's own nodes goes out row by row with consecutive
+        // lanes on consecutive words: a device-coherent store is written
+        // through per instruction, so node-strided stores (lane = node, one
+        // instruction per component) would write each line three times
+        {
+          const int rw = min(C::TX, G.nx + 1 - i0), rows = min(C::TY, G.ny + 1 - j0);
+          for (int t = threadIdx.x; t < 3 * C::TX * rows; t += C::NT) {
+            const int r = t / (3 * C::TX), k = t % (3 * C::TX);
+            if (k < 3 * rw)
+              st_coh(&pnew[3 * ((size_t)(j0 + r) * (G.nx + 1) + i0) + k],
+                     p_s[(k % 3) * C::NP + (r + 1) * C::PW + k / 3 + 1]);
           }
         }
         __syncthreads();
@@ -268,14 +281,25 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
       if (fabs(pap[0]) < 1e-30) { breakdown = 1; it++; break; }
       const float alpha = (float)(rz / pap[0]);
       // ---------------- phase B: e += alpha p, r -= alpha Ap, z = M^-1 r
+      // p comes in and z goes out through LDS row by row (whole lines per
+      // coherent access); the node's thread works on its own entries
       acc[0] = 0; acc[1] = 0;
       for (int tile = band.first; tile < band.last; tile += band.step) {
-        const int gi = (tile % ntx) * C::TX + tx, gj = (tile / ntx) * C::TY + ty;
+        const int i0 = (tile % ntx) * C::TX, j0 = (tile / ntx) * C::TY;
+        const int rw = min(C::TX, G.nx + 1 - i0), rows = min(C::TY, G.ny + 1 - j0);
+        constexpr int W3 = 3 * C::TX;
+        for (int t = threadIdx.x; t < W3 * rows; t += C::NT) {
+          const int r = t / W3, k = t % W3;
+          if (k < 3 * rw) ye_s[t] = ld_coh(&pnew[3 * ((size_t)(j0 + r) * (G.nx + 1) + i0) + k]);
+        }
+        __syncthreads();
+        const int gi = i0 + tx, gj = j0 + ty;
         if (node_thread && gi <= G.nx && gj <= G.ny) {
           const int64_t n = (int64_t)gj * (G.nx + 1) + gi;
+          float* own = ye_s + 3 * (ty * C::TX + tx);
           double rv[3], zv[3];
           for (int d = 0; d < 3; d++) {
-            A.e[3 * n + d] = fmaf(alpha, ld_coh(&pnew[3 * n + d]), A.e[3 * n + d]);
+            A.e[3 * n + d] = fmaf(alpha, own[d], A.e[3 * n + d]);
             const float rf = fmaf(-alpha, A.Ap[3 * n + d], A.r[3 * n + d]);
             A.r[3 * n + d] = rf;
             rv[d] = (double)rf;
@@ -283,11 +307,17 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
           sym3_apply(A.minv + 6 * n, rv, zv);
           for (int d = 0; d < 3; d++) {
             const float zf = (float)zv[d];
-            st_coh(&A.z[3 * n + d], zf);
+            own[d] = zf;
             acc[0] += rv[d] * (double)zf;
             acc[1] += rv[d] * rv[d];
           }
         }
+        __syncthreads();
+        for (int t = threadIdx.x; t < W3 * rows; t += C::NT) {
+          const int r = t / W3, k = t % W3;
+          if (k < 3 * rw) st_coh(&A.z[3 * ((size_t)(j0 + r) * (G.nx + 1) + i0) + k], ye_s[t]);
+        }
+        __syncthreads();
       }
       if (!allreduce(acc)) return;
       rr = acc[1];
