@@ -233,8 +233,7 @@ __device__ __forceinline__ void batch_cg(const PlateDesc& D, unsigned me, unsign
     }
   };
   auto allreduce = [&](double (&v)[3]) -> bool {
-    publish();
-    block_sum<3, kNT>(v, red + 16);
+    block_sum_published<3, kNT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce<3, 2>(v, D.part, D.gs, epoch, red, nwg, me);
   };
   auto ring_w = [&](const double* wpub, double (&wv)[3]) {
@@ -328,8 +327,7 @@ __global__ __launch_bounds__(kNT) void k_batch_pcg(const PlateDesc* descs, const
   const int PW = D.TX + 2;
   unsigned epoch = 0;
   auto allreduce = [&](auto& v) -> bool {
-    publish();
-    block_sum<sizeof(v) / sizeof(double), kNT>(v, red + 16);
+    block_sum_published<sizeof(v) / sizeof(double), kNT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce(v, D.part, D.gs, epoch, red, nwg, me);
   };
   // init: r = F (or r0), x = 0 (or kept), z = M^-1 r

@@ -49,6 +49,21 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
+// The workgroup part of an all-reduce's arrival: the block sum of the
+// partials, then every thread waits for its own outstanding stores, then
+// the workgroup meets once more, so thread 0's arrival (grid_allreduce)
+// still follows every store of the workgroup while those stores drain
+// during the block sum.  On the 142 x 142 plate one device took 3.86 us per
+// all-reduce waiting before the sum and 3.45 us waiting after it; on others
+// both orders measure the same.  publish() is defined below.
+__device__ __forceinline__ void publish();
+template <int NV, int NT>
+__device__ __forceinline__ void block_sum_published(double (&v)[NV], double* lds) {
+  block_sum<NV, NT>(v, lds);
+  publish();
+  __syncthreads();
+}
+
 // Cross-workgroup exchange inside a persistent kernel.  Every value that
 // another workgroup reads while the kernel runs is written with st_coh and
 // read with ld_coh: relaxed atomic accesses at device scope (sc1), served by

@@ -493,8 +493,7 @@ __global__ __launch_bounds__(kGenNT) void k_gen_pcg(GenArgs A) {
   const bool timing = me == 0 && threadIdx.x == 0;
   unsigned long long tk[4] = {0, 0, 0, 0};
   auto allreduce = [&](double (&v)[3]) -> bool {
-    publish();
-    block_sum<3, kGenNT>(v, red + 16);
+    block_sum_published<3, kGenNT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce<3, 4>(v, A.part, A.gs, epoch, red, nwg, me);
   };
   auto barrier = [&]() -> bool {

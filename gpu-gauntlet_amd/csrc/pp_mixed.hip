@@ -145,8 +145,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg32(MixArgs A) {
   const bool node_thread = threadIdx.x < C::TX * C::TY;
   unsigned epoch = 0;
   auto allreduce = [&](auto& v) -> bool {
-    publish();  // every thread's stores complete before the arrival
-    block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
+    block_sum_published<sizeof(v) / sizeof(double), C::NT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
   };
   // MFMA A fragments (uniform plates fold bending + shear into one matrix)

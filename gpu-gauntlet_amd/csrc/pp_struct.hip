@@ -240,8 +240,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg(PcgArgs A) {
   const bool timing = me == 0 && threadIdx.x == 0;
   unsigned long long tk[4] = {0, 0, 0, 0}, t0 = 0, t1 = 0, t2 = 0, t3 = 0;
   auto allreduce = [&](auto& v) -> bool {
-    publish();  // every thread's stores complete before the arrival
-    block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
+    block_sum_published<sizeof(v) / sizeof(double), C::NT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
   };
 
@@ -504,8 +503,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg_res(PcgArgs A) {
   const bool timing = me == 0 && threadIdx.x == 0;
   unsigned long long tk[6] = {0, 0, 0, 0, 0, 0};
   auto allreduce = [&](auto& v) -> bool {
-    publish();
-    block_sum<sizeof(v) / sizeof(double), C::NT>(v, red + 16);
+    block_sum_published<sizeof(v) / sizeof(double), C::NT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce(v, A.part, A.gs, epoch, red, nwg, me);
   };
 
@@ -741,8 +739,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg_cg(PcgArgs A) {
     }
   };
   auto allreduce = [&](double (&v)[3]) -> bool {
-    publish();
-    block_sum<3, C::NT>(v, red + 16);
+    block_sum_published<3, C::NT>(v, red + 16);  // stores drain during the sum
     // at most a few hundred workgroups: two loads in flight per lane suffice
     return grid_allreduce<3, 2>(v, A.part, A.gs, epoch, red, nwg, me);
   };
@@ -976,8 +973,7 @@ __global__ __launch_bounds__(C::NT) void k_pcg_cheb(PcgArgs A, ChebArgs Q) {
     return true;
   };
   auto allreduce = [&](double (&v)[3]) -> bool {
-    publish();
-    block_sum<3, C::NT>(v, red + 16);
+    block_sum_published<3, C::NT>(v, red + 16);  // stores drain during the sum
     return grid_allreduce<3, 2>(v, A.part, A.gs, epoch, red, nwg, me);
   };
 
