@@ -460,13 +460,15 @@ struct GenArgs {
 // at multiples of 64), so the slice's width and offset are wave-uniform and
 // every block and column load is coalesced.  Padding slots carry a zero
 // block and the node itself.
-__device__ __forceinline__ void gen_kx(const GenArgs& A, int64_t n, double (&v)[3]) {
+template <bool COH = true>
+__device__ __forceinline__ void ell_kx(const int64_t* slc, const uint32_t* col, const double* blk,
+                                       const double* u, int64_t n, double (&v)[3]) {
   const int sl = __builtin_amdgcn_readfirstlane((int)(n >> 6));
   const int l = (int)(n & 63);
-  const int64_t first = A.slc[2 * (int64_t)sl];
-  const int width = (int)A.slc[2 * (int64_t)sl + 1];
-  const uint32_t* cp = A.col + first + l;
-  const double* bp = A.blk + 9 * first + l;
+  const int64_t first = slc[2 * (int64_t)sl];
+  const int width = (int)slc[2 * (int64_t)sl + 1];
+  const uint32_t* cp = col + first + l;
+  const double* bp = blk + 9 * first + l;
   v[0] = v[1] = v[2] = 0.0;
 #pragma unroll 4
   for (int k = 0; k < width; k++) {
@@ -475,11 +477,30 @@ __device__ __forceinline__ void gen_kx(const GenArgs& A, int64_t n, double (&v)[
     double bk[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) bk[j] = b[64 * j];
-    const double u0 = ld_coh(&A.u[3 * (size_t)m]), u1 = ld_coh(&A.u[3 * (size_t)m + 1]),
-                 u2 = ld_coh(&A.u[3 * (size_t)m + 2]);
+    const double* um = u + 3 * (size_t)m;
+    const double u0 = COH ? ld_coh(um) : um[0], u1 = COH ? ld_coh(um + 1) : um[1],
+                 u2 = COH ? ld_coh(um + 2) : um[2];
 #pragma unroll
     for (int d = 0; d < 3; d++) v[d] += bk[3 * d] * u0 + bk[3 * d + 1] * u1 + bk[3 * d + 2] * u2;
   }
+}
+__device__ __forceinline__ void gen_kx(const GenArgs& A, int64_t n, double (&v)[3]) {
+  ell_kx<true>(A.slc, A.col, A.blk, A.u, n, v);
+}
+
+// y = K x from the node blocks, y[c] = x[c] on constrained DOFs (one thread
+// per node): the persistent solve's residual check in one launch.
+__global__ __launch_bounds__(256) void k_ell_apply(int64_t nn, const int64_t* slc,
+                                                   const uint32_t* col, const double* blk,
+                                                   const uint8_t* cmask, const double* x,
+                                                   double* y) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= nn) return;
+  double v[3];
+  ell_kx<false>(slc, col, blk, x, n, v);
+  const unsigned fix = cmask[n];
+#pragma unroll
+  for (int d = 0; d < 3; d++) y[3 * n + d] = ((fix >> d) & 1u) ? x[3 * n + d] : v[d];
 }
 
 // Slots of a node's row whose columns a thread keeps in registers.
@@ -960,7 +981,11 @@ int general_solve_persistent(pp_plan* pl, const pp_solve_opts* o, pp_solve_stats
   double rel = 0;
   if (ds.converged && bnorm > 0 && !(o->flags & PP_FLAG_NO_TRUE_RESIDUAL)) {
     for (int k = 0;; k++) {
-      if ((rc = apply_all(pl, pl->x.p, pl->tmp.p, nullptr))) return rc;
+      hipLaunchKernelGGL(k_ell_apply, dim3((unsigned)((pl->n_nodes + 255) / 256)), dim3(256), 0, s,
+                         (int64_t)pl->n_nodes, (const int64_t*)pl->ell_slc.p,
+                         (const uint32_t*)pl->ell_col.p, (const double*)pl->ell_blk.p,
+                         (const uint8_t*)pl->cmask.p, (const double*)pl->x.p, pl->tmp.p);
+      PP_HIP(hipGetLastError());
       double res = 0, fn = 0;
       if ((rc = launch_norms(pl, pl->F.p, pl->tmp.p, &res, &fn))) return rc;
       rel = fn > 0 ? res / fn : 0.0;
