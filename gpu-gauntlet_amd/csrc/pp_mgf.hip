@@ -257,6 +257,28 @@ __device__ __forceinline__ void stage(int i0, int j0, int nx, int ny, const doub
   }
 }
 
+// Stage src into u_s and y into y_s (both SoA, tile plus H-node halo, zero
+// outside the grid) in one pass, so the loads of both are in flight together.
+template <int TS, int H, int NT>
+__device__ __forceinline__ void stage_two(int i0, int j0, int nx, int ny, const double* src,
+                                          const double* y, double* u_s, double* y_s) {
+  using T = TG<TS, H>;
+  for (int idx = threadIdx.x; idx < T::NP; idx += NT) {
+    const int lx = idx % T::PW, ly = idx / T::PW;
+    const int gi = i0 - H + lx, gj = j0 - H + ly;
+    double v[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
+    if (gi >= 0 && gi <= nx && gj >= 0 && gj <= ny) {
+      const int64_t n = (int64_t)gj * (nx + 1) + gi;
+      for (int d = 0; d < 3; d++) v[d] = src[3 * n + d];
+      for (int d = 0; d < 3; d++) w[d] = y[3 * n + d];
+    }
+    for (int d = 0; d < 3; d++) {
+      u_s[d * T::NP + idx] = v[d];
+      y_s[d * T::NP + idx] = w[d];
+    }
+  }
+}
+
 // Stencil row of a level >= 1 in the fused form: per node 84 floats (21
 // float4), the 9 blocks of its row of A in the order the apply adds them:
 // the centre and forward blocks E, NW, N, NE as stored in the planes, then
@@ -399,10 +421,11 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs A) {
   // ---- consumed partials (wave 0; the sum reaches the other waves through LDS)
   double cv[1] = {0.0};
   if (A.cin && threadIdx.x < 64) wave_sum_parts<1>(A.cin, A.cn, cv);
-  stage<TS, H, NT>(i0, j0, nx, ny, A.x, nullptr, 0.0, PRO ? A.pro.xc : nullptr, A.pro.cnx,
-                   A.L.mask, u_s, xc_s);
-  if (MODE == M_KP)
-    stage<TS, H, NT>(i0, j0, nx, ny, A.y, nullptr, 0.0, nullptr, 0, A.L.mask, y_s);
+  if constexpr (MODE == M_KP)
+    stage_two<TS, H, NT>(i0, j0, nx, ny, A.x, A.y, u_s, y_s);
+  else
+    stage<TS, H, NT>(i0, j0, nx, ny, A.x, nullptr, 0.0, PRO ? A.pro.xc : nullptr, A.pro.cnx,
+                     A.L.mask, u_s, xc_s);
   if (threadIdx.x == 0) in_s[0] = cv[0];
   __syncthreads();
   if (A.check && stop_rule(in_s[0], A.S, A.rec, A.nrec)) return;
